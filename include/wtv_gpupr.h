@@ -1,0 +1,147 @@
+/*
+ * wtv_gpupr.h -- C-ABI of the MI355X graph-cut solver (libwtv_gpupr.so).
+ *
+ * Plain C types only: device buffers are passed as raw pointers, streams as
+ * opaque handles (hipStream_t cast to void*, NULL = the null stream).  Every
+ * entry point returns a wtv_status code; the message of the last failure on
+ * the calling thread is available from wtv_last_error().
+ *
+ * Reference interfaces these entry points replace:
+ *   wtv_pr_solve        <- cellocut_gpu_push_relabel_fast
+ *                          (native/gpupr/gpu_push_relabel_fast.h:20-28,
+ *                           gpu_push_relabel_fast.cu:406-571)
+ *   wtv_graph_cut_fast  <- graph_cut_gpupr_fast_v630, exported to Python as
+ *                          wtivo_gpupr.graph_cut_fast and
+ *                          wtivo_gpupr.graph_cut_gpupr_fast_v630
+ *                          (native/gpupr/wtivo_gpupr_bindings.cpp:100-397,581-596)
+ *   wtv_batch_solve     <- no single reference function: many independent
+ *                          cellocut_gpu_push_relabel_fast calls fused into
+ *                          one launch sequence (batched mode)
+ */
+#ifndef WTV_GPUPR_H
+#define WTV_GPUPR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  Python mapping (pybind module wtivo_gpupr):
+ * WTV_EOVERFLOW -> OverflowError, everything else -> RuntimeError, as the
+ * reference's std::overflow_error / std::runtime_error
+ * (wtivo_gpupr_bindings.cpp:95-96; gpu_push_relabel_fast.cu:23-29,349-358). */
+typedef enum wtv_status {
+    WTV_OK = 0,
+    WTV_EINVAL = 1,     /* invalid argument or graph                          */
+    WTV_EOVERFLOW = 2,  /* capacity outside [0, 2^30)                         */
+    WTV_EINVARIANT = 3, /* device invariant (reference codes 2..5 in message) */
+    WTV_ENOMEM = 4,     /* insufficient device memory                         */
+    WTV_ELIMIT = 5,     /* max_rounds or the 240 s wall limit exceeded        */
+    WTV_EDEVICE = 6     /* HIP runtime error                                  */
+} wtv_status;
+
+/* Solver statistics.  The first six fields are CellocutGpuPRFastStats
+ * (gpu_push_relabel_fast.h:11-18); the rest are this solver's work counters. */
+typedef struct wtv_stats {
+    double solve_seconds;          /* discharge loop, after the first relabel */
+    double final_relabel_seconds;  /* the residual BFS that fixes labels      */
+    uint64_t solver_workspace_bytes;
+    uint64_t rounds;               /* macro rounds (local_steps sub-rounds)   */
+    uint64_t global_relabels;
+    int64_t raw_flow;
+    /* work counters (device-side, exact) */
+    uint64_t visits;     /* discharge vertex-visits                           */
+    uint64_t pushes;     /* successful pushes (arc or sink)                   */
+    uint64_t relabels;   /* height increases                                  */
+    uint64_t bfs_pops;   /* BFS frontier expansions, all global relabels      */
+    uint64_t sub_rounds; /* synchronous push/relabel sub-rounds               */
+    uint64_t bfs_levels; /* BFS levels expanded, all global relabels          */
+    double t_solve_ms;   /* device time: preflow init -> partition written    */
+    double t_bfs_ms;     /* device time in global relabels (incl. final)      */
+} wtv_stats;
+
+/* Solver knobs.  global_relabel_period, max_rounds and local_steps keep the
+ * reference's meaning and clamps (gpu_push_relabel_fast.cu:421-424,476-480):
+ * local_steps in [1,32]; period < 0 -> 0 (never); max_rounds 0 -> 2,000,000;
+ * a global relabel runs every max(1, ceil(period/local_steps)) macro rounds. */
+typedef struct wtv_params {
+    int32_t global_relabel_period; /* reference default 1024 */
+    int32_t local_steps;           /* reference default 8    */
+    uint64_t max_rounds;           /* reference default 2e6  */
+    int32_t wg_size;               /* 0 = default (256)      */
+    int32_t grid_wgs;              /* 0 = default            */
+    double time_limit_s;           /* 0 = 240 s (reference)  */
+} wtv_params;
+
+/* Fill *p with the reference defaults. */
+void wtv_params_default(wtv_params* p);
+
+/*
+ * Push-relabel max-flow + final residual BFS on a reduced graph resident in
+ * device memory (replaces cellocut_gpu_push_relabel_fast).
+ *   n           free nodes (n == 0 returns zero stats)
+ *   d_nbr       int32[n*4] neighbour ids, -1 = no arc          (read only)
+ *   d_res       uint32[n*4] residual capacities                (in/out)
+ *   d_term      int32[n] >0 source supply, <0 sink capacity    (in/out)
+ *   partition   uint8[n]: 1 = reaches the sink (outside), 0 = source side.
+ *               Host memory when partition_on_device == 0, else device.
+ *   stream      hipStream_t or NULL
+ */
+int wtv_pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
+                 uint8_t* partition, int partition_on_device,
+                 const wtv_params* params, wtv_stats* stats, void* stream);
+
+/* The 17 scalars of graph_cut_fast's 18-tuple, in tuple order
+ * (wtivo_gpupr_bindings.cpp:377-396); labels are written to labels_out. */
+typedef struct wtv_cut_result {
+    int64_t adjusted_flow;
+    int64_t raw_flow;
+    uint64_t graph_bytes;
+    uint64_t workspace_bytes;
+    uint64_t full_edges;
+    uint64_t free_edges;
+    uint64_t fixed_source;
+    uint64_t fixed_sink;
+    uint64_t terminal_edges;
+    int64_t threads;
+    uint64_t macro_rounds;
+    uint64_t global_relabels;
+    uint64_t map_bytes;
+    double build_upload_seconds;
+    int64_t local_steps;
+    double solve_seconds;
+    double native_total_seconds;
+} wtv_cut_result;
+
+/*
+ * Full graph cut (replaces graph_cut_gpupr_fast_v630): classify tets, build the
+ * reduced graph with the exact capacity formula, fold terminals, solve, and
+ * scatter the partition back to tet order.
+ *   V          float64[nv*3] host vertices
+ *   d_tets     int32[N*4] device tets      (borrowed, not modified)
+ *   d_nbr      int32[N*4] device neighbours (borrowed, not modified)
+ *   labels     uint8[N] host input labels
+ *   labels_out uint8[N] host output labels
+ */
+int wtv_graph_cut_fast(const double* V, size_t nv, const int32_t* d_tets,
+                       const int32_t* d_nbr, size_t N, const uint8_t* labels,
+                       double fill_T, int threads, int global_relabel_period,
+                       uint64_t max_rounds, int local_steps, uint8_t* labels_out,
+                       wtv_cut_result* out, void* stream);
+
+/* Stats of the last wtv_pr_solve / wtv_graph_cut_fast on this thread. */
+void wtv_last_stats(wtv_stats* out);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* wtv_last_error(void);
+
+/* Library version string. */
+const char* wtv_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WTV_GPUPR_H */

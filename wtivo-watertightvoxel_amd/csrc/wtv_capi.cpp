@@ -1,0 +1,95 @@
+// wtv_capi.cpp -- extern "C" entry points of libwtv_gpupr.so (include/wtv_gpupr.h).
+// C++ exceptions never cross this boundary: they become status codes plus a
+// thread-local message (wtv_last_error).
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+
+#include "wtv_internal.h"
+
+namespace wtv {
+void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t* dN, size_t N,
+                    const uint8_t* lp, double fill_T, int requested_threads, int period, uint64_t max_rounds,
+                    int local_steps, uint8_t* out_labels, wtv_cut_result& R, wtv_stats& st, void* stream_v);
+}
+
+namespace {
+thread_local std::string g_err;
+thread_local wtv_stats g_stats{};
+
+template <class Fn>
+int guarded(Fn fn) {
+    g_err.clear();
+    try {
+        fn();
+        return WTV_OK;
+    } catch (const wtv::Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return WTV_ENOMEM;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return WTV_EINVAL;
+    } catch (...) {
+        g_err = "unknown error";
+        return WTV_EINVAL;
+    }
+}
+}  // namespace
+
+namespace wtv {
+void set_last_stats(const wtv_stats& st) { g_stats = st; }
+}  // namespace wtv
+
+extern "C" {
+
+void wtv_params_default(wtv_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->global_relabel_period = 1024;
+    p->local_steps = 8;
+    p->max_rounds = 2000000ull;
+    p->wg_size = 0;
+    p->grid_wgs = 0;
+    p->time_limit_s = 0.0;
+}
+
+int wtv_pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,
+                 int partition_on_device, const wtv_params* params, wtv_stats* stats, void* stream) {
+    return guarded([&]() {
+        wtv_params p;
+        wtv_params_default(&p);
+        if (params) p = *params;
+        wtv_stats st{};
+        wtv::pr_solve(n, d_nbr, d_res, d_term, partition, partition_on_device != 0, p, st, stream);
+        wtv::set_last_stats(st);
+        if (stats) *stats = st;
+    });
+}
+
+int wtv_graph_cut_fast(const double* V, size_t nv, const int32_t* d_tets, const int32_t* d_nbr, size_t N,
+                       const uint8_t* labels, double fill_T, int threads, int global_relabel_period,
+                       uint64_t max_rounds, int local_steps, uint8_t* labels_out, wtv_cut_result* out,
+                       void* stream) {
+    return guarded([&]() {
+        wtv_cut_result r{};
+        wtv_stats st{};
+        wtv::graph_cut_fast(V, nv, d_tets, d_nbr, N, labels, fill_T, threads, global_relabel_period,
+                            max_rounds, local_steps, labels_out, r, st, stream);
+        wtv::set_last_stats(st);
+        if (out) *out = r;
+    });
+}
+
+void wtv_last_stats(wtv_stats* out) {
+    if (out) *out = g_stats;
+}
+
+const char* wtv_last_error(void) { return g_err.c_str(); }
+
+const char* wtv_version(void) { return "wtv_gpupr 0.1.0 gfx950"; }
+
+}  // extern "C"

@@ -1,0 +1,564 @@
+// wtv_solver.hip -- push-relabel max-flow + residual BFS partition for gfx950.
+//
+// Replaces cellocut_gpu_push_relabel_fast (native/gpupr/gpu_push_relabel_fast.cu:406-571).
+// Same arithmetic as the reference: uint32 residuals, int32 terminals
+// (>0 source supply, <0 remaining sink capacity), int64 excess, uint64 sink
+// flow; source height n, unreached height n+1; active = excess > 0 and
+// height < n (gpu_push_relabel_fast.cu:149-166); relabel to 1 + min height over
+// residual arcs, the sink counting as height 0 when term < 0 and the source as
+// height n when term > 0 (gpu_push_relabel_fast.cu:274-302).
+//
+// Schedule (this file's own design, not the reference's): every sub-round is
+// two kernels over device-resident worklists.
+//   push    : heights are read-only.  Each active vertex pushes to the sink
+//             (h == 1) and along every admissible arc (h(v) == h(u) + 1) as
+//             long as it has excess.  Its own residual slots and terminal are
+//             written with plain stores (no other thread writes them in this
+//             phase: admissibility is antisymmetric); the receiver's reverse
+//             residual and excess take device-scope atomic adds.
+//   relabel : every vertex that was active or received flow recomputes its
+//             height from its residual arcs and re-enters the worklist.
+// Because heights never change during a push phase and only grow in the
+// relabel phase, the height function stays a valid labelling
+// (h(v) <= h(u) + 1 on every residual arc) with any number of threads, which
+// is what makes the final BFS partition the unique minimal sink set.
+//
+// Worklist appends are wave64-aggregated (ballot + mbcnt prefix, one atomic
+// per wave).  Counters live in device memory and are zeroed by the kernel
+// before the one that appends to them, so the host never waits per round:
+// it reads the active count once per macro round (local_steps sub-rounds).
+//
+// Global relabel = level-synchronous BFS from the sink over residual arcs
+// (gpu_push_relabel_fast.cu:85-147,360-402), with reverse slots precomputed
+// once per solve into one byte per vertex (2 bits per slot) instead of the
+// reference's per-arc 4-slot search (gpu_push_relabel_fast.cu:56-67).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <sstream>
+#include <vector>
+
+#include "wtv_internal.h"
+
+namespace wtv {
+namespace {
+
+constexpr uint32_t kSent = 0xFFFFFFFFu;
+constexpr int kTPB = 256;
+
+// Device control block: worklist counters, flow, work counters, error word.
+struct Ctl {
+    uint32_t wl_cnt[2];   // active lists W[0], W[1]
+    uint32_t rl_cnt[2];   // relabel list counts (ping-pong)
+    uint32_t bfs_cnt[3];  // BFS frontier counts (ring of 3)
+    uint32_t bfs_levels;
+    unsigned long long sink_flow;
+    unsigned long long visits, pushes, relabels, bfs_pops;
+    int32_t error;
+    uint32_t violations;
+};
+
+// ---------------------------------------------------------------- atomics
+__device__ __forceinline__ uint32_t atom_add(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ long long atom_add(long long* p, long long v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long atom_add(unsigned long long* p, unsigned long long v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t atom_xchg(uint32_t* p, uint32_t v) {
+    return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool atom_cas(uint32_t* p, uint32_t expected, uint32_t desired) {
+    return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED,
+                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ long long atom_load(const long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void set_error(Ctl* c, int code) {
+    __hip_atomic_exchange(&c->error, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ------------------------------------------------------- wave64 primitives
+__device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
+// Append val to list for every active lane with pred set: one atomic per wave.
+// All active lanes of the wave must reach this call.
+__device__ __forceinline__ void wave_append(bool pred, uint32_t val, uint32_t* list, uint32_t* cnt) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const uint32_t rank = lane_rank(m);
+    const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+    uint32_t base = 0;
+    if (pred && rank == 0) base = atom_add(cnt, static_cast<uint32_t>(__popcll(m)));
+    base = __shfl(base, leader);
+    if (pred) list[base + rank] = val;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+// Sum per-lane counters over the wave and add them once per wave.
+__device__ __forceinline__ void flush_counter(unsigned long long* dst, unsigned long long x) {
+    x = wave_sum(x);
+    if (__lane_id() == 0 && x) atom_add(dst, x);
+}
+
+__device__ __forceinline__ int slot_of(const int4& v, int k) {
+    return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
+}
+__device__ __forceinline__ uint32_t slot_of(const uint4& v, int k) {
+    return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
+}
+
+// ---------------------------------------------------------------- kernels
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+
+// Reverse slots, preflow init (gpu_push_relabel_fast.cu:69-83) and mark reset.
+__global__ void k_prepare(uint32_t n, const int4* __restrict__ nbr4, const int32_t* __restrict__ term,
+                          uint8_t* __restrict__ rslot, long long* __restrict__ excess,
+                          uint32_t* __restrict__ mark, Ctl* ctl) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const int4 nb = nbr4[v];
+        uint32_t code = 0;
+        int err = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int u = slot_of(nb, k);
+            if (u < 0) continue;
+            if (static_cast<uint32_t>(u) >= n) { err = 3; continue; }
+            const int4 nu = nbr4[u];
+            const int iv = static_cast<int>(v);
+            const int j = nu.x == iv ? 0 : nu.y == iv ? 1 : nu.z == iv ? 2 : nu.w == iv ? 3 : -1;
+            if (j < 0) { err = 2; continue; }
+            code |= static_cast<uint32_t>(j) << (2 * k);
+        }
+        if (err) set_error(ctl, err);
+        rslot[v] = static_cast<uint8_t>(code);
+        const int32_t t = term[v];
+        excess[v] = t > 0 ? static_cast<long long>(t) : 0LL;
+        mark[v] = 0u;
+    }
+}
+
+// BFS seeds: vertices with remaining sink capacity get height 1
+// (gpu_push_relabel_fast.cu:85-101); every other vertex starts unreached.
+__global__ void k_bfs_seed(uint32_t n, const int32_t* __restrict__ term, uint32_t* __restrict__ height,
+                           uint32_t* __restrict__ frontier, Ctl* ctl) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->bfs_cnt[2] = 0u;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const bool s = term[v] < 0;
+        height[v] = s ? 1u : kSent;
+        wave_append(s, v, frontier, &ctl->bfs_cnt[1]);
+    }
+}
+
+// One BFS level: frontier at height d -> unreached v with residual v->u > 0
+// gets height d+1 (gpu_push_relabel_fast.cu:103-139).
+__global__ void k_bfs_level(uint32_t d, const uint32_t* __restrict__ fin, uint32_t* __restrict__ fout,
+                            const int4* __restrict__ nbr4, const uint8_t* __restrict__ rslot,
+                            const uint32_t* __restrict__ res, uint32_t* height, Ctl* ctl) {
+    const uint32_t cnt = ctl->bfs_cnt[d % 3u];
+    uint32_t* out_cnt = &ctl->bfs_cnt[(d + 1u) % 3u];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->bfs_cnt[(d + 2u) % 3u] = 0u;
+        if (cnt) ctl->bfs_levels += 1u;
+    }
+    const uint32_t stride = gridDim.x * blockDim.x;
+    unsigned long long pops = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += stride) {
+        const uint32_t u = fin[i];
+        const int4 nb = nbr4[u];
+        const uint32_t rs = rslot[u];
+        ++pops;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int v = slot_of(nb, k);
+            bool claim = false;
+            if (v >= 0 && height[v] == kSent) {
+                const uint32_t kv = (rs >> (2 * k)) & 3u;
+                if (res[static_cast<size_t>(v) * 4u + kv] != 0u)
+                    claim = atom_cas(&height[v], kSent, d + 1u);
+            }
+            wave_append(claim, static_cast<uint32_t>(v), fout, out_cnt);
+        }
+    }
+    flush_counter(&ctl->bfs_pops, pops);
+}
+
+// Active set {excess > 0, height < n} (gpu_push_relabel_fast.cu:149-166);
+// unreached vertices are normalised to n+1 (gpu_push_relabel_fast.cu:141-147).
+__global__ void k_build_active(uint32_t n, const long long* __restrict__ excess, uint32_t* __restrict__ height,
+                               uint32_t* __restrict__ wl, uint32_t p, Ctl* ctl) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rl_cnt[p] = 0u;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        uint32_t h = height[v];
+        if (h == kSent) {
+            h = n + 1u;
+            height[v] = h;
+        }
+        const bool a = excess[v] > 0 && h < n;
+        wave_append(a, v, wl, &ctl->wl_cnt[p]);
+    }
+}
+
+// Push phase of sub-round with parity p (heights read-only).
+__global__ void k_push(uint32_t n, uint32_t p, const uint32_t* __restrict__ wl, uint32_t* __restrict__ rl,
+                       const int4* __restrict__ nbr4, const uint8_t* __restrict__ rslot, uint32_t* res,
+                       int32_t* __restrict__ term, long long* excess, const uint32_t* __restrict__ height,
+                       uint32_t* mark, Ctl* ctl) {
+    const uint32_t cnt = ctl->wl_cnt[p];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->wl_cnt[p ^ 1u] = 0u;
+    uint32_t* rl_cnt = &ctl->rl_cnt[p];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    unsigned long long visits = 0, pushes = 0, sinkf = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += stride) {
+        const uint32_t v = wl[i];
+        // every worklist vertex is relabel-checked exactly once this sub-round
+        const bool first = atom_xchg(&mark[v], 1u) == 0u;
+        wave_append(first, v, rl, rl_cnt);
+
+        long long e = atom_load(&excess[v]);
+        const uint32_t h = height[v];
+        const bool act = e > 0 && h < n;
+        int4 nb = make_int4(-1, -1, -1, -1);
+        uint4 r = make_uint4(0u, 0u, 0u, 0u);
+        uint32_t rs = 0;
+        uint32_t hu[4] = {kSent, kSent, kSent, kSent};
+        long long pushed = 0;
+        if (act) {
+            ++visits;
+            nb = nbr4[v];
+            r = reinterpret_cast<const uint4*>(res)[v];
+            rs = rslot[v];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int u = slot_of(nb, k);
+                if (u >= 0) hu[k] = height[u];
+            }
+            // v -> sink (gpu_push_relabel_fast.cu:220-236)
+            const int32_t t = term[v];
+            if (t < 0 && h == 1u) {
+                const long long cap = -static_cast<long long>(t);
+                const long long dlt = e < cap ? e : cap;
+                term[v] = static_cast<int32_t>(static_cast<long long>(t) + dlt);
+                e -= dlt;
+                pushed += dlt;
+                sinkf += static_cast<unsigned long long>(dlt);
+                ++pushes;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int u = slot_of(nb, k);
+            const uint32_t rk = slot_of(r, k);
+            bool cand = false;
+            if (e > 0 && u >= 0 && rk != 0u && h == hu[k] + 1u) {
+                const uint32_t dlt = static_cast<long long>(rk) < e ? rk : static_cast<uint32_t>(e);
+                res[static_cast<size_t>(v) * 4u + k] = rk - dlt;
+                const uint32_t back = (rs >> (2 * k)) & 3u;
+                atom_add(&res[static_cast<size_t>(u) * 4u + back], dlt);
+                const long long old = atom_add(&excess[u], static_cast<long long>(dlt));
+                e -= dlt;
+                pushed += dlt;
+                ++pushes;
+                if (old == 0) cand = atom_xchg(&mark[u], 1u) == 0u;
+            }
+            wave_append(cand, static_cast<uint32_t>(u), rl, rl_cnt);
+        }
+        if (pushed) atom_add(&excess[v], -pushed);
+    }
+    flush_counter(&ctl->visits, visits);
+    flush_counter(&ctl->pushes, pushes);
+    flush_counter(&ctl->sink_flow, sinkf);
+}
+
+// Relabel phase of sub-round with parity p: recompute heights of every vertex
+// that was active or received flow, and rebuild the active list.
+__global__ void k_relabel(uint32_t n, uint32_t p, const uint32_t* __restrict__ rl, uint32_t* __restrict__ wl_next,
+                          const int4* __restrict__ nbr4, const uint32_t* __restrict__ res,
+                          const int32_t* __restrict__ term, const long long* __restrict__ excess,
+                          uint32_t* height, uint32_t* __restrict__ mark, Ctl* ctl) {
+    const uint32_t cnt = ctl->rl_cnt[p];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rl_cnt[p ^ 1u] = 0u;
+    uint32_t* wl_cnt = &ctl->wl_cnt[p ^ 1u];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    unsigned long long relabels = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += stride) {
+        const uint32_t v = rl[i];
+        mark[v] = 0u;
+        const long long e = excess[v];
+        uint32_t h = height[v];
+        bool act = false;
+        if (e > 0 && h < n) {
+            const int32_t t = term[v];
+            uint32_t mh = kSent;
+            if (t < 0) mh = 0u;
+            else if (t > 0) mh = n;
+            const int4 nb = nbr4[v];
+            const uint4 r = reinterpret_cast<const uint4*>(res)[v];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int u = slot_of(nb, k);
+                if (u >= 0 && slot_of(r, k) != 0u) {
+                    const uint32_t x = height[u];
+                    mh = x < mh ? x : mh;
+                }
+            }
+            if (mh == kSent) {
+                set_error(ctl, 5);
+            } else if (mh + 1u > h) {
+                h = mh + 1u;
+                height[v] = h;
+                ++relabels;
+            }
+            act = h < n;
+        }
+        wave_append(act, v, wl_next, wl_cnt);
+    }
+    flush_counter(&ctl->relabels, relabels);
+}
+
+// partition = reaches the sink (gpu_push_relabel_fast.cu:315-324); counts any
+// vertex still active after the final BFS (an invariant violation).
+__global__ void k_partition(uint32_t n, const uint32_t* __restrict__ height, const long long* __restrict__ excess,
+                            uint8_t* __restrict__ part, Ctl* ctl) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    unsigned long long bad = 0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const uint32_t h = height[v];
+        part[v] = h < n + 1u ? 1u : 0u;
+        if (excess[v] > 0 && h < n) ++bad;
+    }
+    bad = wave_sum(bad);
+    if (__lane_id() == 0 && bad) atom_add(&ctl->violations, static_cast<uint32_t>(bad));
+}
+
+// --------------------------------------------------------------- host side
+inline void ck(hipError_t e, const char* where) {
+    if (e == hipSuccess) return;
+    std::ostringstream oss;
+    oss << kPrefix << "HIP failure at " << where << ": " << hipGetErrorString(e);
+    fail(WTV_EDEVICE, oss.str());
+}
+
+struct DeviceBuf {
+    void* p = nullptr;
+    ~DeviceBuf() { if (p) (void)hipFree(p); }
+};
+struct HostBuf {
+    void* p = nullptr;
+    ~HostBuf() { if (p) (void)hipHostFree(p); }
+};
+struct Event {
+    hipEvent_t e = nullptr;
+    Event() { ck(hipEventCreate(&e), "event create"); }
+    ~Event() { if (e) (void)hipEventDestroy(e); }
+};
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+struct Layout {
+    size_t height, excess, mark, rslot, wl0, wl1, rl, ctl, total;
+};
+
+Layout layout_for(uint32_t n) {
+    Layout L{};
+    size_t o = 0;
+    auto take = [&](size_t bytes) { size_t at = o; o = align_up(o + bytes, 256); return at; };
+    const size_t N = n;
+    L.ctl = take(sizeof(Ctl));
+    L.excess = take(N * 8u);
+    L.height = take(N * 4u);
+    L.mark = take(N * 4u);
+    L.wl0 = take(N * 4u);
+    L.wl1 = take(N * 4u);
+    L.rl = take(N * 4u);
+    L.rslot = take(N);
+    L.total = o;
+    return L;
+}
+
+}  // namespace
+
+void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,
+              bool partition_on_device, const wtv_params& prm, wtv_stats& st, void* stream_v) {
+    using Clock = std::chrono::steady_clock;
+    std::memset(&st, 0, sizeof(st));
+    if (!n) return;
+    if (!d_nbr || !d_res || !d_term || !partition) fail(WTV_EINVAL, std::string(kPrefix) + "null graph pointer");
+    if (n > 0x7FFFFFFFu) fail(WTV_EINVAL, std::string(kPrefix) + ">INT32_MAX reduced nodes unsupported");
+    int local_steps = prm.local_steps;
+    if (local_steps < 1) local_steps = 1;
+    if (local_steps > 32) local_steps = 32;
+    int period = prm.global_relabel_period < 0 ? 0 : prm.global_relabel_period;
+    uint64_t max_rounds = prm.max_rounds ? prm.max_rounds : 2000000ull;
+    const double time_limit = prm.time_limit_s > 0 ? prm.time_limit_s : 240.0;
+    const int tpb = kTPB;
+    int grid = prm.grid_wgs > 0 ? prm.grid_wgs : 1024;
+    const int sweep_grid = static_cast<int>(std::min<uint64_t>((static_cast<uint64_t>(n) + tpb - 1) / tpb, 8192));
+    hipStream_t s = static_cast<hipStream_t>(stream_v);
+
+    const Layout L = layout_for(n);
+    DeviceBuf ws;
+    ck(hipMalloc(&ws.p, L.total), "workspace malloc");
+    st.solver_workspace_bytes = L.total;
+    char* base = static_cast<char*>(ws.p);
+    Ctl* ctl = reinterpret_cast<Ctl*>(base + L.ctl);
+    long long* excess = reinterpret_cast<long long*>(base + L.excess);
+    uint32_t* height = reinterpret_cast<uint32_t*>(base + L.height);
+    uint32_t* mark = reinterpret_cast<uint32_t*>(base + L.mark);
+    uint32_t* wl[2] = {reinterpret_cast<uint32_t*>(base + L.wl0), reinterpret_cast<uint32_t*>(base + L.wl1)};
+    uint32_t* rl = reinterpret_cast<uint32_t*>(base + L.rl);
+    uint8_t* rslot = reinterpret_cast<uint8_t*>(base + L.rslot);
+    const int4* nbr4 = reinterpret_cast<const int4*>(d_nbr);
+
+    HostBuf hctl_buf;
+    ck(hipHostMalloc(&hctl_buf.p, sizeof(Ctl), hipHostMallocDefault), "pinned ctl");
+    Ctl* hctl = static_cast<Ctl*>(hctl_buf.p);
+
+    auto read_ctl = [&](const char* where) {
+        ck(hipMemcpyAsync(hctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s), where);
+        ck(hipStreamSynchronize(s), where);
+    };
+    auto check_err = [&](const char* stage) {
+        read_ctl(stage);
+        if (hctl->error) {
+            std::ostringstream oss;
+            oss << kPrefix << "device invariant failed at " << stage << " code=" << hctl->error;
+            fail(WTV_EINVARIANT, oss.str());
+        }
+    };
+
+    Event ev_t0, ev_t1, ev_b0, ev_b1;
+    double bfs_ms = 0.0;
+
+    ck(hipMemsetAsync(ctl, 0, sizeof(Ctl), s), "ctl clear");
+    ck(hipEventRecord(ev_t0.e, s), "event");
+    k_prepare<<<sweep_grid, tpb, 0, s>>>(n, nbr4, d_term, rslot, excess, mark, ctl);
+    ck(hipGetLastError(), "prepare");
+    check_err("prepare");
+
+    uint64_t sub = 0;  // global sub-round counter; parity selects the worklist
+    // BFS from the sink; leaves the frontier buffers dead (wl[] reused).
+    auto global_relabel = [&]() {
+        ck(hipEventRecord(ev_b0.e, s), "event");
+        k_set_u32<<<1, 1, 0, s>>>(&ctl->bfs_cnt[1], 0u);
+        k_bfs_seed<<<sweep_grid, tpb, 0, s>>>(n, d_term, height, wl[1], ctl);
+        ck(hipGetLastError(), "bfs seed");
+        uint32_t d = 1;
+        const uint32_t batch = 8;
+        for (;;) {
+            for (uint32_t b = 0; b < batch; ++b, ++d)
+                k_bfs_level<<<grid, tpb, 0, s>>>(d, wl[d & 1u], wl[(d + 1u) & 1u], nbr4, rslot, d_res, height, ctl);
+            ck(hipGetLastError(), "bfs expand");
+            read_ctl("bfs count");
+            if (hctl->bfs_cnt[d % 3u] == 0u) break;
+            if (static_cast<uint64_t>(d) > static_cast<uint64_t>(n) + 1ull)
+                fail(WTV_EINVARIANT, std::string(kPrefix) + "BFS exceeded n+1 levels");
+        }
+        ck(hipEventRecord(ev_b1.e, s), "event");
+        ck(hipEventSynchronize(ev_b1.e), "event sync");
+        float ms = 0.f;
+        ck(hipEventElapsedTime(&ms, ev_b0.e, ev_b1.e), "event time");
+        bfs_ms += ms;
+        ++st.global_relabels;
+    };
+    auto build_active = [&]() -> uint32_t {
+        const uint32_t p = static_cast<uint32_t>(sub & 1u);
+        k_set_u32<<<1, 1, 0, s>>>(&ctl->wl_cnt[p], 0u);
+        k_build_active<<<sweep_grid, tpb, 0, s>>>(n, excess, height, wl[p], p, ctl);
+        ck(hipGetLastError(), "build active");
+        read_ctl("active count");
+        return hctl->wl_cnt[p];
+    };
+
+    global_relabel();
+    check_err("global relabel");
+    uint32_t active = build_active();
+
+    int macro_period = 0;
+    if (period > 0) macro_period = std::max(1, (period + local_steps - 1) / local_steps);
+
+    const auto solve0 = Clock::now();
+    while (active) {
+        if (st.rounds >= max_rounds) fail(WTV_ELIMIT, std::string(kPrefix) + "max macro rounds exceeded");
+        for (int j = 0; j < local_steps; ++j, ++sub) {
+            const uint32_t p = static_cast<uint32_t>(sub & 1u);
+            k_push<<<grid, tpb, 0, s>>>(n, p, wl[p], rl, nbr4, rslot, d_res, d_term, excess, height, mark, ctl);
+            k_relabel<<<grid, tpb, 0, s>>>(n, p, rl, wl[p ^ 1u], nbr4, d_res, d_term, excess, height, mark, ctl);
+        }
+        ck(hipGetLastError(), "discharge");
+        ++st.rounds;
+        read_ctl("next count");
+        active = hctl->wl_cnt[sub & 1u];
+        if (hctl->error) {
+            std::ostringstream oss;
+            oss << kPrefix << "device invariant failed at discharge code=" << hctl->error;
+            fail(WTV_EINVARIANT, oss.str());
+        }
+        if (macro_period > 0 && active && (st.rounds % static_cast<uint64_t>(macro_period)) == 0ull) {
+            global_relabel();
+            active = build_active();
+        }
+        if ((st.rounds & 255ull) == 0ull) {
+            const double el = std::chrono::duration<double>(Clock::now() - solve0).count();
+            if (el > time_limit) fail(WTV_ELIMIT, std::string(kPrefix) + "solve exceeded 240s safety limit");
+        }
+    }
+    check_err("solve completion");
+    st.solve_seconds = std::chrono::duration<double>(Clock::now() - solve0).count();
+
+    const auto fr0 = Clock::now();
+    global_relabel();
+    st.final_relabel_seconds = std::chrono::duration<double>(Clock::now() - fr0).count();
+
+    uint8_t* d_part = partition_on_device ? partition : reinterpret_cast<uint8_t*>(rl);
+    k_partition<<<sweep_grid, tpb, 0, s>>>(n, height, excess, d_part, ctl);
+    ck(hipGetLastError(), "partition");
+    ck(hipEventRecord(ev_t1.e, s), "event");
+    if (!partition_on_device)
+        ck(hipMemcpyAsync(partition, d_part, static_cast<size_t>(n), hipMemcpyDeviceToHost, s), "partition D2H");
+    read_ctl("final");
+    if (hctl->error) {
+        std::ostringstream oss;
+        oss << kPrefix << "device invariant failed at final code=" << hctl->error;
+        fail(WTV_EINVARIANT, oss.str());
+    }
+    if (hctl->violations) {
+        std::ostringstream oss;
+        oss << kPrefix << "device invariant failed at partition: " << hctl->violations
+            << " vertices still active after the final BFS";
+        fail(WTV_EINVARIANT, oss.str());
+    }
+    if (hctl->sink_flow > static_cast<unsigned long long>(INT64_MAX))
+        fail(WTV_EINVARIANT, std::string(kPrefix) + "flow exceeds int64");
+    st.raw_flow = static_cast<int64_t>(hctl->sink_flow);
+    st.visits = hctl->visits;
+    st.pushes = hctl->pushes;
+    st.relabels = hctl->relabels;
+    st.bfs_pops = hctl->bfs_pops;
+    st.bfs_levels = hctl->bfs_levels;
+    st.sub_rounds = sub;
+    float ms = 0.f;
+    ck(hipEventElapsedTime(&ms, ev_t0.e, ev_t1.e), "event time");
+    st.t_solve_ms = ms;
+    st.t_bfs_ms = bfs_ms;
+}
+
+}  // namespace wtv

@@ -1,0 +1,458 @@
+"""Synthetic tet-graph workloads for the graph-cut solver.
+
+Builds jittered Kuhn (Freudenthal) tetrahedralisations of a k^3 grid with the
+reference's 4-slot neighbour layout, shape signed-distance functions, tet
+labels, and the two capacity models the solver is exercised with:
+
+* the reference model: graph_cut_fast computes capacities itself from the
+  vertices, tets and labels (wtivo_gpupr_bindings.cpp:65-98);
+* the soft-seed model: every tet is a free node with a terminal arc of
+  lambda_fill * volume on its labelled side and facet arcs of
+  area * exp(-|UDF(facet centroid)| / h), quantised to integers with one
+  power-of-two scale per graph (the solver's arithmetic is integer,
+  gpu_push_relabel_fast.h:23, wtivo_gpupr_bindings.cpp:217-220).
+
+Slot convention (wtivo_core.cpp:68-71, wtivo_gpupr_bindings.cpp:70): slot fi is
+the face {012, 013, 023, 123}[fi] of the tet's own vertex order, i.e. the face
+opposite local vertex 3-fi; a hull face has neighbour -1.
+
+Kuhn tet (cell c, axis permutation p) has vertices v0=c, v1=v0+e[p0],
+v2=v1+e[p1], v3=c+(1,1,1).  Its neighbours:
+  slot 0 (opposite v3): cell c-e[p2], perm (p2,p0,p1), their slot 3
+  slot 1 (opposite v2): cell c,       perm (p0,p2,p1), their slot 1
+  slot 2 (opposite v1): cell c,       perm (p1,p0,p2), their slot 2
+  slot 3 (opposite v0): cell c+e[p0], perm (p1,p2,p0), their slot 0
+
+All randomness comes from numpy's PCG64 with an explicit seed; heavy
+arithmetic runs in torch (CPU threads, or the GPU when device="cuda").
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+try:  # torch is used as an array engine only
+    import torch
+except Exception:  # pragma: no cover
+    torch = None
+
+PERMS = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+_PIDX = {p: i for i, p in enumerate(PERMS)}
+FACE_LUT = ((0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3))
+KUHN_REV_SLOT = (3, 1, 2, 0)
+
+# per (slot, perm): neighbour perm index and cell offset
+_NB_PERM = np.zeros((4, 6), np.int64)
+_NB_OFF = np.zeros((4, 6, 3), np.int64)
+for _pi, (_a, _b, _c) in enumerate(PERMS):
+    _e = np.eye(3, dtype=np.int64)
+    _NB_PERM[0, _pi] = _PIDX[(_c, _a, _b)]
+    _NB_OFF[0, _pi] = -_e[_c]
+    _NB_PERM[1, _pi] = _PIDX[(_a, _c, _b)]
+    _NB_PERM[2, _pi] = _PIDX[(_b, _a, _c)]
+    _NB_PERM[3, _pi] = _PIDX[(_b, _c, _a)]
+    _NB_OFF[3, _pi] = _e[_a]
+
+
+@dataclass
+class TetMesh:
+    k: int
+    V: np.ndarray      # float64 [nv,3]
+    tets: np.ndarray   # int32 [N,4]
+    nbrs: np.ndarray   # int32 [N,4]
+
+    @property
+    def N(self) -> int:
+        return int(self.tets.shape[0])
+
+
+def kuhn_mesh(k: int, jitter: float = 0.2, seed: int = 0, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)) -> TetMesh:
+    """Jittered Kuhn 6-tet subdivision of a k^3 grid: 6k^3 tets, (k+1)^3 vertices."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    k1 = k + 1
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    cell = (hi - lo) / k
+    g = np.arange(k1, dtype=np.float64)
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    V = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    V += rng.uniform(-jitter, jitter, size=V.shape)
+    V = lo + V * cell
+    tets, nbrs = kuhn_topology(k)
+    return TetMesh(k, V, tets, nbrs)
+
+
+def kuhn_topology(k: int, chunk_cells: int = 1 << 21):
+    """Tets and 4-slot neighbours of the Kuhn subdivision, tet id = cell*6 + perm."""
+    k1 = k + 1
+    ncell = k * k * k
+    tets = np.empty((ncell * 6, 4), np.int32)
+    nbrs = np.empty((ncell * 6, 4), np.int32)
+    e = np.eye(3, dtype=np.int64)
+    vstride = np.array([1, k1, k1 * k1], np.int64)
+    cstride = np.array([1, k, k * k], np.int64)
+    perms = np.array(PERMS, np.int64)
+    for c0 in range(0, ncell, chunk_cells):
+        c1 = min(ncell, c0 + chunk_cells)
+        c = np.arange(c0, c1, dtype=np.int64)
+        cx = c % k
+        cy = (c // k) % k
+        cz = c // (k * k)
+        cc = np.stack([cx, cy, cz], 1)  # [m,3]
+        base_v = cc @ vstride
+        m = c.shape[0]
+        T = np.empty((m, 6, 4), np.int64)
+        Nb = np.empty((m, 6, 4), np.int64)
+        for pi in range(6):
+            p = perms[pi]
+            v1 = base_v + vstride[p[0]]
+            v2 = v1 + vstride[p[1]]
+            v3 = base_v + vstride.sum()
+            T[:, pi, 0] = base_v
+            T[:, pi, 1] = v1
+            T[:, pi, 2] = v2
+            T[:, pi, 3] = v3
+            for s in range(4):
+                off = _NB_OFF[s, pi]
+                nc = cc + off
+                ok = np.all((nc >= 0) & (nc < k), axis=1)
+                nid = (nc @ cstride) * 6 + _NB_PERM[s, pi]
+                Nb[:, pi, s] = np.where(ok, nid, -1)
+        tets[c0 * 6:c1 * 6] = T.reshape(-1, 4)
+        nbrs[c0 * 6:c1 * 6] = Nb.reshape(-1, 4)
+    return tets, nbrs
+
+
+def reverse_slots(nbrs: np.ndarray) -> np.ndarray:
+    """rs[i,k] = slot of i in nbrs[nbrs[i,k]] (or -1 for hull / missing)."""
+    N = nbrs.shape[0]
+    rs = np.full((N, 4), -1, np.int8)
+    idx = np.arange(N, dtype=np.int64)
+    for k in range(4):
+        j = nbrs[:, k].astype(np.int64)
+        ok = j >= 0
+        jj = np.where(ok, j, 0)
+        for r in range(4):
+            hit = ok & (nbrs[jj, r] == idx) & (rs[:, k] < 0)
+            rs[hit, k] = r
+    return rs
+
+
+# --------------------------------------------------------------------------
+# shapes (signed distance, negative inside)
+# --------------------------------------------------------------------------
+class Shape:
+    def sdf(self, p):  # p: torch [m,3] float64
+        raise NotImplementedError
+
+
+@dataclass
+class Sphere(Shape):
+    c: tuple
+    r: float
+
+    def sdf(self, p):
+        c = p.new_tensor(self.c)
+        return torch.linalg.vector_norm(p - c, dim=1) - self.r
+
+
+@dataclass
+class Torus(Shape):
+    """Torus around axis `ax` (unit vector) through centre c, radii R, r."""
+    c: tuple
+    ax: tuple
+    R: float
+    r: float
+
+    def sdf(self, p):
+        c = p.new_tensor(self.c)
+        a = p.new_tensor(self.ax)
+        a = a / torch.linalg.vector_norm(a)
+        d = p - c
+        h = d @ a
+        radial = torch.linalg.vector_norm(d - h[:, None] * a[None, :], dim=1)
+        q = radial - self.R
+        return torch.sqrt(q * q + h * h) - self.r
+
+
+@dataclass
+class Cylinder(Shape):
+    """Infinite cylinder of radius r along unit direction `ax` through point c."""
+    c: tuple
+    ax: tuple
+    r: float
+
+    def sdf(self, p):
+        c = p.new_tensor(self.c)
+        a = p.new_tensor(self.ax)
+        a = a / torch.linalg.vector_norm(a)
+        d = p - c
+        h = d @ a
+        return torch.linalg.vector_norm(d - h[:, None] * a[None, :], dim=1) - self.r
+
+
+@dataclass
+class Union(Shape):
+    parts: list
+
+    def sdf(self, p):
+        out = self.parts[0].sdf(p)
+        for s in self.parts[1:]:
+            out = torch.minimum(out, s.sdf(p))
+        return out
+
+
+@dataclass
+class Subtract(Shape):
+    a: Shape
+    holes: list
+
+    def sdf(self, p):
+        out = self.a.sdf(p)
+        for h in self.holes:
+            out = torch.maximum(out, -h.sdf(p))
+        return out
+
+
+@dataclass
+class Shell(Shape):
+    """|sdf(a)| - t/2 : a hollow wall of thickness t around a's surface."""
+    a: Shape
+    t: float
+
+    def sdf(self, p):
+        return torch.abs(self.a.sdf(p)) - 0.5 * self.t
+
+
+def _unit(rng):
+    v = rng.normal(size=3)
+    return tuple((v / np.linalg.norm(v)).tolist())
+
+
+def sphere_with_holes(seed: int, r=0.35, hole_r=0.05, holes=3) -> Shape:
+    """Sphere of radius r at the centre with `holes` seeded cylinders punched through it."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cyl = []
+    for _ in range(holes):
+        off = rng.uniform(-0.15, 0.15, size=3)
+        cyl.append(Cylinder(tuple((0.5 + off).tolist()), _unit(rng), hole_r))
+    return Subtract(Sphere((0.5, 0.5, 0.5), r), cyl)
+
+
+def genus3(seed: int = 0) -> Shape:
+    """Three fused tori in a row (a genus-3 solid)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    parts = []
+    for i in range(3):
+        cx = 0.5 + (i - 1) * 0.27 + rng.uniform(-0.01, 0.01)
+        parts.append(Torus((cx, 0.5, 0.5), (0.0, 0.0, 1.0), 0.14, 0.05))
+    return Union(parts)
+
+
+def thin_shell_with_tunnels(seed: int, cell: float, tunnels=40) -> Shape:
+    """Spherical shell with wall thickness 1.5 cells and seeded tunnels through the wall."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    holes = []
+    for _ in range(tunnels):
+        d = np.asarray(_unit(rng))
+        c = 0.5 + 0.35 * d
+        holes.append(Cylinder(tuple(c.tolist()), tuple(d.tolist()), 1.5 * cell))
+    return Subtract(Shell(Sphere((0.5, 0.5, 0.5), 0.35), 1.5 * cell), holes)
+
+
+def random_shape(rng: np.random.Generator) -> Shape:
+    """Union of 1-4 spheres or tori with 0-5 punched holes."""
+    parts = []
+    for _ in range(int(rng.integers(1, 5))):
+        c = tuple(rng.uniform(0.3, 0.7, size=3).tolist())
+        if rng.uniform() < 0.5:
+            parts.append(Sphere(c, float(rng.uniform(0.1, 0.25))))
+        else:
+            R = float(rng.uniform(0.1, 0.2))
+            parts.append(Torus(c, _unit(rng), R, float(rng.uniform(0.03, 0.6 * R))))
+    holes = [Cylinder(tuple(rng.uniform(0.35, 0.65, size=3).tolist()), _unit(rng),
+                      float(rng.uniform(0.02, 0.06))) for _ in range(int(rng.integers(0, 6)))]
+    shape = Union(parts) if len(parts) > 1 else parts[0]
+    return Subtract(shape, holes) if holes else shape
+
+
+# --------------------------------------------------------------------------
+# geometry helpers (torch, chunked)
+# --------------------------------------------------------------------------
+def _dev(device):
+    return torch.device(device if device is not None else "cpu")
+
+
+def tet_centroids_sdf(mesh: TetMesh, shape: Shape, device=None, chunk=1 << 22) -> np.ndarray:
+    dev = _dev(device)
+    Vt = torch.from_numpy(mesh.V).to(dev)
+    out = np.empty(mesh.N, np.float64)
+    for a in range(0, mesh.N, chunk):
+        b = min(mesh.N, a + chunk)
+        t = torch.from_numpy(mesh.tets[a:b].astype(np.int64)).to(dev)
+        cen = Vt[t].mean(dim=1)
+        out[a:b] = shape.sdf(cen).cpu().numpy()
+    return out
+
+
+def labels_from_sdf(sdf: np.ndarray, flip: float = 0.0, seed: int = 0) -> np.ndarray:
+    """0 = inside (source side, wtvo_vdb.cpp:1578 polarity), 1 = outside; then seeded flips."""
+    lab = (sdf >= 0.0).astype(np.uint8)
+    if flip > 0:
+        rng = np.random.Generator(np.random.PCG64(seed))
+        f = rng.uniform(size=lab.shape[0]) < flip
+        lab ^= f.astype(np.uint8)
+    return lab
+
+
+def band_labels(sdf: np.ndarray, band: float) -> np.ndarray:
+    """Reference labels: 0 = within the UDF band (|sdf| <= band), else 1 (wtivo_vdb.cpp:1578)."""
+    return (np.abs(sdf) > band).astype(np.uint8)
+
+
+@dataclass
+class SoftGraph:
+    """Soft-seed graph: every tet is a free node."""
+    nbr: np.ndarray        # int32 [N,4]
+    res: np.ndarray        # uint32 [N,4] quantised facet capacities
+    term: np.ndarray       # int32 [N] quantised terminal (+ source, - sink)
+    cap_f32: np.ndarray    # float32 [N,4]
+    term_f32: np.ndarray   # float32 [N] (+ source, - sink)
+    scale: float
+    meta: dict = field(default_factory=dict)
+
+
+def soft_graph(mesh: TetMesh, labels: np.ndarray, shape: Shape, lam: float, h: float,
+               device=None, chunk=1 << 21, max_cap=(1 << 30) - 1, max_term=(1 << 31) - 1) -> SoftGraph:
+    """Soft-seed capacity model: terminal = lam * |vol| on the labelled side, facet =
+    area * exp(-|UDF(facet centroid)|/h), fp32, quantised by one power-of-two scale."""
+    dev = _dev(device)
+    N = mesh.N
+    Vt = torch.from_numpy(mesh.V).to(dev)
+    cap = np.empty((N, 4), np.float32)
+    vol = np.empty(N, np.float32)
+    for a in range(0, N, chunk):
+        b = min(N, a + chunk)
+        t = torch.from_numpy(mesh.tets[a:b].astype(np.int64)).to(dev)
+        P = Vt[t]  # [m,4,3]
+        d1 = P[:, 1] - P[:, 0]
+        d2 = P[:, 2] - P[:, 0]
+        d3 = P[:, 3] - P[:, 0]
+        vol[a:b] = (torch.abs((torch.cross(d1, d2, dim=1) * d3).sum(1)) / 6.0).float().cpu().numpy()
+        cols = []
+        for fi in range(4):
+            i0, i1, i2 = FACE_LUT[fi]
+            e1 = P[:, i1] - P[:, i0]
+            e2 = P[:, i2] - P[:, i0]
+            area = 0.5 * torch.linalg.vector_norm(torch.cross(e1, e2, dim=1), dim=1)
+            cen = (P[:, i0] + P[:, i1] + P[:, i2]) / 3.0
+            udf = torch.abs(shape.sdf(cen))
+            cols.append((area * torch.exp(-udf / h)).float())
+        cap[a:b] = torch.stack(cols, 1).cpu().numpy()
+    nb = mesh.nbrs
+    # one value per facet: take the lower tet id's evaluation for both arcs
+    rs = reverse_slots(nb)
+    idx = np.arange(N, dtype=np.int64)[:, None]
+    j = nb.astype(np.int64)
+    take = (j >= 0) & (j < idx)
+    jj = np.where(take, j, 0)
+    rr = np.where(take, rs, 0).astype(np.int64)
+    cap = np.where(take, cap[jj, rr], cap).astype(np.float32)
+    cap[nb < 0] = 0.0
+    tf = (lam * vol).astype(np.float32)
+    term_f32 = np.where(labels == 0, tf, -tf).astype(np.float32)
+    mc = float(cap.max()) if cap.size else 0.0
+    mt = float(np.abs(term_f32).max()) if term_f32.size else 0.0
+    s = math.inf
+    if mc > 0:
+        s = min(s, max_cap / mc)
+    if mt > 0:
+        s = min(s, max_term / mt)
+    scale = 2.0 ** math.floor(math.log2(s)) if math.isfinite(s) else 1.0
+    res = np.floor(cap.astype(np.float64) * scale).astype(np.int64)
+    tq = np.trunc(term_f32.astype(np.float64) * scale).astype(np.int64)
+    assert res.max(initial=0) <= max_cap and np.abs(tq).max(initial=0) <= max_term
+    return SoftGraph(nb.copy(), res.astype(np.uint32), tq.astype(np.int32), cap, term_f32, scale,
+                     {"lam": lam, "h": h, "N": N})
+
+
+def cut_energy(g: SoftGraph, partition: np.ndarray) -> float:
+    """FP64 cut energy of a partition (1 = sink side) from the original fp32 capacities."""
+    p = partition.astype(bool)
+    t = g.term_f32.astype(np.float64)
+    e = float(np.sum(np.where(~p & (t < 0), -t, 0.0)) + np.sum(np.where(p & (t > 0), t, 0.0)))
+    j = g.nbr.astype(np.int64)
+    ok = j >= 0
+    pj = np.where(ok, p[np.where(ok, j, 0)], False)
+    cut = ok & (~p)[:, None] & pj
+    return e + float(np.sum(np.where(cut, g.cap_f32.astype(np.float64), 0.0)))
+
+
+def quantised_cut(nbr, res0, term0, partition) -> int:
+    """Integer cut capacity of a partition on the quantised graph (== max-flow at optimum)."""
+    p = partition.astype(bool)
+    t = term0.astype(np.int64)
+    e = int(np.sum(np.where(~p & (t < 0), -t, 0)) + np.sum(np.where(p & (t > 0), t, 0)))
+    j = nbr.astype(np.int64)
+    ok = j >= 0
+    pj = np.where(ok, p[np.where(ok, j, 0)], False)
+    cut = ok & (~p)[:, None] & pj
+    return e + int(np.sum(np.where(cut, res0.astype(np.int64), 0)))
+
+
+def lognormal_int_graph(mesh: TetMesh, labels: np.ndarray, lam: float, seed: int, sigma=2.0,
+                        cap_hi=(1 << 20)) -> SoftGraph:
+    """Integer stress model: facet capacities log-normal(sigma) quantised to [1, cap_hi]
+    (one value per facet), terminals lam * round(volume-proportional) integers."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    N = mesh.N
+    nb = mesh.nbrs
+    raw = rng.lognormal(mean=0.0, sigma=sigma, size=(N, 4))
+    q = np.clip(np.rint(raw * 64.0), 1, cap_hi).astype(np.int64)
+    rs = reverse_slots(nb)
+    idx = np.arange(N, dtype=np.int64)[:, None]
+    j = nb.astype(np.int64)
+    take = (j >= 0) & (j < idx)
+    q = np.where(take, q[np.where(take, j, 0), np.where(take, rs, 0).astype(np.int64)], q)
+    q[nb < 0] = 0
+    t = np.full(N, int(lam) * 64, np.int64)
+    term = np.where(labels == 0, t, -t)
+    return SoftGraph(nb.copy(), q.astype(np.uint32), term.astype(np.int32), q.astype(np.float32),
+                     term.astype(np.float32), 1.0, {"lam": lam, "sigma": sigma, "N": N})
+
+
+def random_graph(n: int, seed: int, edges_per_node: float = 1.6, cap_max: int = 20,
+                 term_max: int = 30, p_term: float = 0.6):
+    """Random symmetric graph with at most 4 arcs per node in random slots (tests)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nbr = np.full((n, 4), -1, np.int32)
+    res = np.zeros((n, 4), np.uint32)
+    deg = np.zeros(n, np.int64)
+    tries = int(edges_per_node * n * 4) + 8
+    target = int(edges_per_node * n)
+    m = 0
+    for _ in range(tries):
+        if m >= target or n < 2:
+            break
+        a, b = (int(x) for x in rng.integers(0, n, size=2))
+        if a == b or deg[a] >= 4 or deg[b] >= 4:
+            continue
+        if np.any(nbr[a] == b):
+            continue
+        sa = int(rng.choice(np.flatnonzero(nbr[a] < 0)))
+        sb = int(rng.choice(np.flatnonzero(nbr[b] < 0)))
+        nbr[a, sa] = b
+        nbr[b, sb] = a
+        res[a, sa] = int(rng.integers(0, cap_max + 1))
+        res[b, sb] = int(rng.integers(0, cap_max + 1))
+        deg[a] += 1
+        deg[b] += 1
+        m += 1
+    term = np.zeros(n, np.int32)
+    has = rng.uniform(size=n) < p_term
+    mag = rng.integers(1, term_max + 1, size=n)
+    sign = np.where(rng.uniform(size=n) < 0.5, 1, -1)
+    term[has] = (mag * sign)[has]
+    return nbr, res, term
