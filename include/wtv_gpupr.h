@@ -14,9 +14,6 @@
  *                          wtivo_gpupr.graph_cut_fast and
  *                          wtivo_gpupr.graph_cut_gpupr_fast_v630
  *                          (native/gpupr/wtivo_gpupr_bindings.cpp:100-397,581-596)
- *   wtv_batch_solve     <- no single reference function: many independent
- *                          cellocut_gpu_push_relabel_fast calls fused into
- *                          one launch sequence (batched mode)
  */
 #ifndef WTV_GPUPR_H
 #define WTV_GPUPR_H
@@ -60,6 +57,13 @@ typedef struct wtv_stats {
     uint64_t bfs_levels; /* BFS levels expanded, all global relabels          */
     double t_solve_ms;   /* device time: preflow init -> partition written    */
     double t_bfs_ms;     /* device time in global relabels (incl. final)      */
+    /* per-kernel device time, filled only when wtv_params.profile != 0 */
+    double t_push_ms;    /* sum over discharge push launches                  */
+    double t_relabel_ms; /* sum over discharge relabel launches               */
+    double t_level_ms;   /* sum over BFS level launches                       */
+    uint64_t n_push;     /* launches of each kind                             */
+    uint64_t n_relabel;
+    uint64_t n_level;
 } wtv_stats;
 
 /* Solver knobs.  global_relabel_period, max_rounds and local_steps keep the
@@ -73,6 +77,7 @@ typedef struct wtv_params {
     int32_t wg_size;               /* 0 = default (256)      */
     int32_t grid_wgs;              /* 0 = default            */
     double time_limit_s;           /* 0 = 240 s (reference)  */
+    int32_t profile;               /* 1 = time every kernel with HIP events */
 } wtv_params;
 
 /* Fill *p with the reference defaults. */

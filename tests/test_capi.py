@@ -1,0 +1,80 @@
+"""CPU-side checks of the drop-in boundary: the HIP library loads, exports every
+symbol include/wtv_gpupr.h declares, and rejects bad arguments before touching a
+device; the pybind module keeps the reference's Python signature."""
+import ctypes as C
+import inspect
+import os
+import re
+
+import pytest
+
+import wtv_capi as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "wtv_gpupr.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(wtv_[a-z_]+)\s*\(", src)))
+
+
+def test_header_symbols_exported():
+    names = _declared()
+    assert set(names) == set(W.EXPORTS)
+    lib = W.lib()
+    for nm in names:
+        assert hasattr(lib, nm), nm
+
+
+def test_params_defaults_match_reference():
+    p = W.params()
+    # wtivo_gpupr_bindings.cpp:587-590 defaults
+    assert p.global_relabel_period == 1024 and p.local_steps == 8 and p.max_rounds == 2000000
+    assert W.lib().wtv_version().startswith(b"wtv_gpupr")
+
+
+def test_struct_sizes_match_c_layout():
+    # 6 doubles + 14 uint64 (2 of them int64) in wtv_stats
+    assert C.sizeof(W.WtvStats) == 20 * 8
+    assert C.sizeof(W.WtvCutResult) == 17 * 8
+
+
+def test_zero_nodes_and_null_pointers_without_device():
+    st = W.pr_solve(0, 0, 0, 0, 0, 0)  # n == 0 returns zero stats (gpu_push_relabel_fast.cu:418)
+    assert st.raw_flow == 0 and st.rounds == 0
+    with pytest.raises(W.WtvError) as e:
+        W.pr_solve(5, 0, 0, 0, 0, 0)
+    assert e.value.code == W.WTV_EINVAL and "null graph pointer" in str(e.value)
+
+
+def test_graph_cut_null_args_rejected():
+    r = W.WtvCutResult()
+    rc = W.lib().wtv_graph_cut_fast(None, 0, None, None, 10, None, 10.0, 1, 1024, 0, 8, None, C.byref(r), None)
+    assert rc == W.WTV_EINVAL
+
+
+def test_pybind_module_signature():
+    import wtivo_gpupr as G
+    for fn in (G.graph_cut_fast, G.graph_cut_gpupr_fast_v630):
+        doc = fn.__doc__
+        assert "vertices" in doc and "tets_cuda" in doc and "neighbors_cuda" in doc and "labels" in doc
+        import re as _re
+        got = dict(_re.findall(r"(\w+): [^=,]*= (\d+)", doc))
+        assert got == {"threads": "16", "global_relabel_period": "1024", "max_rounds": "2000000",
+                       "local_steps": "8"}
+        order = _re.findall(r"(\w+): ", doc.split("(", 1)[1])
+        assert order == ["vertices", "tets_cuda", "neighbors_cuda", "labels", "fill_T", "threads",
+                         "global_relabel_period", "max_rounds", "local_steps"]
+
+
+def test_pybind_rejects_non_cuda_like_reference():
+    import numpy as np
+    import torch
+    import wtivo_gpupr as G
+    V = np.zeros((4, 3))
+    t = torch.zeros((1, 4), dtype=torch.int32)
+    with pytest.raises(RuntimeError, match="tets/neighbors must be CUDA tensors"):
+        G.graph_cut_fast(V, t, t, np.ones(1, np.uint8), 10.0)
+    with pytest.raises(RuntimeError, match="vertices must be float64 Nx3"):
+        G.graph_cut_fast(np.zeros((4, 2)), t, t, np.ones(1, np.uint8), 10.0)

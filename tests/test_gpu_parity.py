@@ -1,0 +1,221 @@
+"""GPU parity: the HIP solver, called through the C-ABI (libwtv_gpupr.so), against
+the CPU oracle on the same seeded inputs.  Integer work: flows, labels and final
+residual certificates must match exactly."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import wtv_capi as W
+import wtv_synth as S
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    W.lib()
+    return torch
+
+
+def gpu_solve(torch, nbr, res, term, device_partition=False, **kw):
+    """wtv_pr_solve through ctypes; returns (partition, stats dict, res, term) on the host."""
+    dn = torch.from_numpy(np.ascontiguousarray(nbr, np.int32)).cuda()
+    dr = torch.from_numpy(np.ascontiguousarray(res, np.uint32).view(np.int32)).cuda()
+    dt = torch.from_numpy(np.ascontiguousarray(term, np.int32)).cuda()
+    n = int(dt.numel())
+    stream = torch.cuda.current_stream().cuda_stream
+    prm = W.params(**kw)
+    if device_partition:
+        part_d = torch.empty(max(n, 1), dtype=torch.uint8, device="cuda")
+        st = W.pr_solve(n, dn.data_ptr(), dr.data_ptr(), dt.data_ptr(), part_d.data_ptr(), 1, prm, stream)
+        part = part_d.cpu().numpy()[:n]
+    else:
+        part = np.empty(max(n, 1), np.uint8)
+        st = W.pr_solve(n, dn.data_ptr(), dr.data_ptr(), dt.data_ptr(), part.ctypes.data, 0, prm, stream)
+        part = part[:n]
+    return part, st.as_dict(), dr.cpu().numpy().view(np.uint32), dt.cpu().numpy()
+
+
+def assert_same(orc, nbr, res, term, gpu, want_flow=None, want_part=None):
+    part, st, r_out, t_out = gpu
+    if want_flow is None:
+        sol = orc.bk_solve(nbr, res, term)
+        want_flow, want_part = sol.raw_flow, sol.partition
+    assert st["raw_flow"] == want_flow
+    assert np.array_equal(part, want_part)
+    assert orc.certificate(nbr, res, term, r_out, t_out, part, st["raw_flow"]) == 0
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_random_graphs(orc, torch_cuda, seed):
+    n = int(np.random.default_rng(seed).integers(6, 400))
+    nbr, res, term = S.random_graph(n, seed, cap_max=int(2 + seed * 7))
+    a = orc.pr_solve(nbr, res, term)
+    if orc.certificate(nbr, res, term, a.res, a.term, a.partition, a.raw_flow) != 0:
+        pytest.skip("instance hits the reference's source-height-n convention")
+    assert_same(orc, nbr, res, term, gpu_solve(torch_cuda, nbr, res, term), a.raw_flow, a.partition)
+
+
+def test_reference_source_height_quirk(orc, torch_cuda):
+    nbr = np.array([[1, -1, -1, -1], [0, -1, -1, -1]], np.int32)
+    res = np.array([[5, 0, 0, 0], [0, 0, 0, 0]], np.uint32)
+    term = np.array([5, -5], np.int32)
+    part, st, _, _ = gpu_solve(torch_cuda, nbr, res, term)
+    a = orc.pr_solve(nbr, res, term)
+    assert st["raw_flow"] == a.raw_flow == 0
+    assert part.tolist() == a.partition.tolist() == [1, 1]
+
+
+@pytest.mark.parametrize("name", sorted(f for f in os.listdir(GOLDEN) if f.endswith(".npz")))
+def test_golden(orc, torch_cuda, name):
+    import wtivo_gpupr as G
+    z = np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+    if "V" in z.files:
+        T = torch_cuda.from_numpy(z["tets"]).cuda()
+        Nb = torch_cuda.from_numpy(z["nbrs"]).cuda()
+        out = G.graph_cut_fast(z["V"], T, Nb, z["labels"], float(z["fill_T"]), 4)
+        assert len(out) == 18
+        assert np.array_equal(out[0], z["out_labels"])
+        assert out[1] == int(z["adjusted_flow"]) and out[2] == int(z["raw_flow"])
+        assert out[3] == int(z["free_n"]) * 36
+    else:
+        gpu = gpu_solve(torch_cuda, z["nbr"], z["res"], z["term"])
+        assert_same(orc, z["nbr"], z["res"], z["term"], gpu, int(z["raw_flow"]), z["partition"])
+
+
+def _c0_soft(k=32, seed=20260000):
+    m = S.kuhn_mesh(k, seed=seed)
+    shape = S.sphere_with_holes(seed)
+    lab = S.labels_from_sdf(S.tet_centroids_sdf(m, shape), 0.02, seed=seed + 1)
+    return m, lab, S.soft_graph(m, lab, shape, 10.0, 1.0 / k)
+
+
+def test_config0_soft_seed(orc, torch_cuda):
+    """196,608-tet jittered Kuhn grid, sphere with 3 holes, 2% flips, soft seeds."""
+    m, lab, g = _c0_soft()
+    a = orc.pr_solve(g.nbr, g.res, g.term)
+    gpu = gpu_solve(torch_cuda, g.nbr, g.res, g.term, device_partition=True)
+    assert_same(orc, g.nbr, g.res, g.term, gpu, a.raw_flow, a.partition)
+    # identical cut energy from the fp32 capacities (same labels -> same energy)
+    assert S.cut_energy(g, gpu[0]) == S.cut_energy(g, a.partition)
+
+
+def test_config0_reference_model_graph_cut_fast(orc, torch_cuda):
+    """The drop-in entry point on config 0 with the reference capacity formula."""
+    import wtivo_gpupr as G
+    m, lab, _ = _c0_soft()
+    r = orc.reduce(m.V, m.tets, m.nbrs, lab, 10.0)
+    sol = orc.pr_solve(r.nbr, r.res, r.term)
+    want = orc.scatter(lab, r.free_old, sol.partition)
+    T = torch_cuda.from_numpy(m.tets).cuda()
+    Nb = torch_cuda.from_numpy(m.nbrs).cuda()
+    out = G.graph_cut_fast(m.V, T, Nb, lab, 10.0, 8, 1024, 2000000, 8)
+    assert np.array_equal(out[0], want)
+    assert out[1] == sol.raw_flow + r.info["base_flow"] and out[2] == sol.raw_flow
+    inf = r.info
+    assert out[3] == r.n * 36                     # graph_bytes
+    assert out[5] == inf["full_refs"] // 2         # full_E
+    assert out[6] == inf["free_refs"] // 2         # free_E
+    assert out[7] == inf["fixed_source"] and out[8] == inf["fixed_sink"]
+    assert out[9] == inf["terminal_edges"] and out[10] == 8 and out[15] == 8
+    assert out[13] == m.N * 4 + r.n * 4            # map_bytes
+    # the same call through the raw C-ABI
+    lab_out = np.empty(m.N, np.uint8)
+    res = W.WtvCutResult()
+    rc = W.lib().wtv_graph_cut_fast(m.V.ctypes.data, m.V.shape[0], T.data_ptr(), Nb.data_ptr(), m.N,
+                                    lab.ctypes.data, 10.0, 4, 1024, 2000000, 8, lab_out.ctypes.data,
+                                    C.byref(res), torch_cuda.cuda.current_stream().cuda_stream)
+    W.check(rc)
+    assert np.array_equal(lab_out, want) and res.adjusted_flow == out[1]
+
+
+def test_stress_integer_shell(orc, torch_cuda):
+    """Scaled-down adversarial stress: thin shell + tunnels, log-normal caps in [1,2^20], lambda 100."""
+    k = 40
+    m = S.kuhn_mesh(k, seed=20260004)
+    shape = S.thin_shell_with_tunnels(20260004, 1.0 / k, tunnels=40)
+    lab = S.labels_from_sdf(S.tet_centroids_sdf(m, shape), 0.0)
+    g = S.lognormal_int_graph(m, lab, 100.0, seed=20260005)
+    a = orc.bk_solve(g.nbr, g.res, g.term)
+    gpu = gpu_solve(torch_cuda, g.nbr, g.res, g.term)
+    assert_same(orc, g.nbr, g.res, g.term, gpu, a.raw_flow, a.partition)
+
+
+@pytest.mark.parametrize("kw", [dict(local_steps=0), dict(local_steps=100), dict(global_relabel_period=-5),
+                                dict(global_relabel_period=1, local_steps=1), dict(grid_wgs=7),
+                                dict(max_rounds=0)])
+def test_clamped_parameters_same_answer(orc, torch_cuda, kw):
+    nbr, res, term = S.random_graph(2000, 77, cap_max=30)
+    a = orc.bk_solve(nbr, res, term)
+    assert_same(orc, nbr, res, term, gpu_solve(torch_cuda, nbr, res, term, **kw), a.raw_flow, a.partition)
+
+
+def test_deterministic_labels_and_flow(torch_cuda):
+    _, _, g = _c0_soft(16, 5)
+    p1, s1, r1, t1 = gpu_solve(torch_cuda, g.nbr, g.res, g.term)
+    p2, s2, r2, t2 = gpu_solve(torch_cuda, g.nbr, g.res, g.term)
+    assert s1["raw_flow"] == s2["raw_flow"] and np.array_equal(p1, p2)
+
+
+def test_max_rounds_limit(torch_cuda):
+    _, _, g = _c0_soft(16, 6)
+    with pytest.raises(W.WtvError, match="max macro rounds exceeded") as e:
+        gpu_solve(torch_cuda, g.nbr, g.res, g.term, max_rounds=1, local_steps=1)
+    assert e.value.code == W.WTV_ELIMIT
+
+
+def test_asymmetric_neighbours_report_invariant(torch_cuda):
+    nbr, res, term = S.random_graph(50, 3)
+    v = int(np.argwhere(nbr >= 0)[0][0])
+    k = int(np.argwhere(nbr[v] >= 0)[0][0])
+    u = int(nbr[v, k])
+    nbr[u, nbr[u] == v] = -1  # drop the reverse arc
+    with pytest.raises(W.WtvError, match="code=2") as e:
+        gpu_solve(torch_cuda, nbr, res, term)
+    assert e.value.code == W.WTV_EINVARIANT
+
+
+def test_graph_cut_fast_error_paths(torch_cuda):
+    import wtivo_gpupr as G
+    m = S.kuhn_mesh(6, seed=1)
+    lab = np.ones(m.N, np.uint8)
+    lab[: m.N // 3] = 0
+    T = torch_cuda.from_numpy(m.tets).cuda()
+    bad = m.nbrs.copy()
+    bad[10, 1] = m.N + 5
+    with pytest.raises(RuntimeError, match="neighbor out of range"):
+        G.graph_cut_fast(m.V, T, torch_cuda.from_numpy(bad).cuda(), lab, 10.0)
+    odd = m.nbrs.copy()
+    i = int(np.flatnonzero((odd >= 0).all(1))[0])
+    odd[i, 2] = -1
+    with pytest.raises(RuntimeError, match="neighbor reference count not even"):
+        G.graph_cut_fast(m.V, T, torch_cuda.from_numpy(odd).cuda(), lab, 10.0)
+    Nb = torch_cuda.from_numpy(m.nbrs).cuda()
+    with pytest.raises(RuntimeError, match="invalid graph reduction/capacity state"):
+        G.graph_cut_fast(m.V, T, Nb, lab, 1e15)  # capacities beyond 2^30
+    with pytest.raises(RuntimeError, match="tets/neighbors must be int32"):
+        G.graph_cut_fast(m.V, T.long(), Nb, lab, 10.0)
+    with pytest.raises(RuntimeError, match="labels size mismatch"):
+        G.graph_cut_fast(m.V, T, Nb, lab[:-1], 10.0)
+    # all label 0 -> no free nodes: labels returned unchanged, full 18-tuple
+    z = np.zeros(m.N, np.uint8)
+    out = G.graph_cut_fast(m.V, T, Nb, z, 10.0)
+    assert len(out) == 18 and np.array_equal(out[0], z) and out[1] == 0 and out[7] == m.N
+
+
+def test_larger_graph_certificate_and_bk(orc, torch_cuda):
+    """1.57M tets (64^3): GPU flow == integer capacity of the GPU cut == BK flow."""
+    k = 64
+    m = S.kuhn_mesh(k, seed=9)
+    shape = S.genus3(9)
+    lab = S.labels_from_sdf(S.tet_centroids_sdf(m, shape), 0.02, seed=10)
+    g = S.soft_graph(m, lab, shape, 10.0, 1.0 / k, rev_slots=S.KUHN_REV_SLOT)
+    part, st, r_out, t_out = gpu_solve(torch_cuda, g.nbr, g.res, g.term, device_partition=True)
+    assert S.quantised_cut(g.nbr, g.res, g.term, part) == st["raw_flow"]
+    b = orc.bk_solve(g.nbr, g.res, g.term)
+    assert b.raw_flow == st["raw_flow"] and np.array_equal(b.partition, part)

@@ -122,6 +122,12 @@ py::dict stats_dict(const wtv_stats& s) {
     d["bfs_levels"] = s.bfs_levels;
     d["t_solve_ms"] = s.t_solve_ms;
     d["t_bfs_ms"] = s.t_bfs_ms;
+    d["t_push_ms"] = s.t_push_ms;
+    d["t_relabel_ms"] = s.t_relabel_ms;
+    d["t_level_ms"] = s.t_level_ms;
+    d["n_push"] = s.n_push;
+    d["n_relabel"] = s.n_relabel;
+    d["n_level"] = s.n_level;
     return d;
 }
 
@@ -129,7 +135,7 @@ py::dict stats_dict(const wtv_stats& s) {
 // (cellocut_gpu_push_relabel_fast, gpu_push_relabel_fast.h:20-28).  res and
 // term are updated in place; returns (partition uint8[n] numpy, stats dict).
 py::tuple pr_solve(py::object nbr_cuda, py::object res_cuda, py::object term_cuda, int global_relabel_period,
-                   std::uint64_t max_rounds, int local_steps, int grid_wgs) {
+                   std::uint64_t max_rounds, int local_steps, int grid_wgs, int profile) {
     for (auto* t : {&nbr_cuda, &res_cuda, &term_cuda})
         if (!t->attr("is_cuda").cast<bool>() || !t->attr("is_contiguous")().cast<bool>())
             throw std::runtime_error("nbr/res/term must be contiguous CUDA tensors");
@@ -149,6 +155,7 @@ py::tuple pr_solve(py::object nbr_cuda, py::object res_cuda, py::object term_cud
     p.max_rounds = max_rounds;
     p.local_steps = local_steps;
     p.grid_wgs = grid_wgs;
+    p.profile = profile;
     py::array_t<std::uint8_t> part(static_cast<py::ssize_t>(n));
     wtv_stats st{};
     int rc;
@@ -179,7 +186,7 @@ PYBIND11_MODULE(wtivo_gpupr, m) {
           py::arg("global_relabel_period") = 1024, py::arg("max_rounds") = 2000000ULL, py::arg("local_steps") = 8);
     m.def("pr_solve", &pr_solve, py::arg("nbr_cuda"), py::arg("res_cuda"), py::arg("term_cuda"),
           py::arg("global_relabel_period") = 1024, py::arg("max_rounds") = 2000000ULL, py::arg("local_steps") = 8,
-          py::arg("grid_wgs") = 0);
+          py::arg("grid_wgs") = 0, py::arg("profile") = 0);
     m.def("last_stats", &last_stats);
     m.def("version", []() { return std::string(wtv_version()); });
 }

@@ -371,6 +371,49 @@ struct Event {
     ~Event() { if (e) (void)hipEventDestroy(e); }
 };
 
+// Per-launch HIP event pairs on the solver's stream (profile mode only).
+// Pairs are harvested after each host synchronisation and recycled.
+struct KernelTimer {
+    enum Kind { kPush = 0, kRelabel = 1, kLevel = 2 };
+    bool on = false;
+    hipStream_t s = nullptr;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<int, size_t>> pending;  // (kind, index of start event)
+    size_t used = 0;
+    double ms[3] = {0, 0, 0};
+    uint64_t count[3] = {0, 0, 0};
+    ~KernelTimer() { for (auto e : pool) (void)hipEventDestroy(e); }
+    hipEvent_t next() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            ck(hipEventCreate(&e), "event create");
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void begin(int kind) {
+        if (!on) return;
+        pending.emplace_back(kind, used);
+        ck(hipEventRecord(next(), s), "event");
+    }
+    void end() {
+        if (!on) return;
+        ck(hipEventRecord(next(), s), "event");
+    }
+    // call only after the stream has been synchronised
+    void harvest() {
+        if (!on) return;
+        for (auto& pr : pending) {
+            float t = 0.f;
+            ck(hipEventElapsedTime(&t, pool[pr.second], pool[pr.second + 1]), "event time");
+            ms[pr.first] += t;
+            count[pr.first] += 1;
+        }
+        pending.clear();
+        used = 0;
+    }
+};
+
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 struct Layout {
@@ -432,9 +475,11 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
     ck(hipHostMalloc(&hctl_buf.p, sizeof(Ctl), hipHostMallocDefault), "pinned ctl");
     Ctl* hctl = static_cast<Ctl*>(hctl_buf.p);
 
+    KernelTimer* ktp = nullptr;
     auto read_ctl = [&](const char* where) {
         ck(hipMemcpyAsync(hctl, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s), where);
         ck(hipStreamSynchronize(s), where);
+        if (ktp) ktp->harvest();
     };
     auto check_err = [&](const char* stage) {
         read_ctl(stage);
@@ -447,6 +492,10 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
 
     Event ev_t0, ev_t1, ev_b0, ev_b1;
     double bfs_ms = 0.0;
+    KernelTimer kt;
+    kt.on = prm.profile != 0;
+    kt.s = s;
+    ktp = &kt;
 
     ck(hipMemsetAsync(ctl, 0, sizeof(Ctl), s), "ctl clear");
     ck(hipEventRecord(ev_t0.e, s), "event");
@@ -464,8 +513,11 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
         uint32_t d = 1;
         const uint32_t batch = 8;
         for (;;) {
-            for (uint32_t b = 0; b < batch; ++b, ++d)
+            for (uint32_t b = 0; b < batch; ++b, ++d) {
+                kt.begin(KernelTimer::kLevel);
                 k_bfs_level<<<grid, tpb, 0, s>>>(d, wl[d & 1u], wl[(d + 1u) & 1u], nbr4, rslot, d_res, height, ctl);
+                kt.end();
+            }
             ck(hipGetLastError(), "bfs expand");
             read_ctl("bfs count");
             if (hctl->bfs_cnt[d % 3u] == 0u) break;
@@ -500,8 +552,12 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
         if (st.rounds >= max_rounds) fail(WTV_ELIMIT, std::string(kPrefix) + "max macro rounds exceeded");
         for (int j = 0; j < local_steps; ++j, ++sub) {
             const uint32_t p = static_cast<uint32_t>(sub & 1u);
+            kt.begin(KernelTimer::kPush);
             k_push<<<grid, tpb, 0, s>>>(n, p, wl[p], rl, nbr4, rslot, d_res, d_term, excess, height, mark, ctl);
+            kt.end();
+            kt.begin(KernelTimer::kRelabel);
             k_relabel<<<grid, tpb, 0, s>>>(n, p, rl, wl[p ^ 1u], nbr4, d_res, d_term, excess, height, mark, ctl);
+            kt.end();
         }
         ck(hipGetLastError(), "discharge");
         ++st.rounds;
@@ -559,6 +615,13 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
     ck(hipEventElapsedTime(&ms, ev_t0.e, ev_t1.e), "event time");
     st.t_solve_ms = ms;
     st.t_bfs_ms = bfs_ms;
+    kt.harvest();
+    st.t_push_ms = kt.ms[KernelTimer::kPush];
+    st.t_relabel_ms = kt.ms[KernelTimer::kRelabel];
+    st.t_level_ms = kt.ms[KernelTimer::kLevel];
+    st.n_push = kt.count[KernelTimer::kPush];
+    st.n_relabel = kt.count[KernelTimer::kRelabel];
+    st.n_level = kt.count[KernelTimer::kLevel];
 }
 
 }  // namespace wtv

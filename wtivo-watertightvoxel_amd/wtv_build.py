@@ -1,6 +1,6 @@
 """Build recipe for libwtv_gpupr.so (HIP, gfx950) and the wtivo_gpupr Python module.
 
-    python wtivo-watertightvoxel_amd/build.py        # builds in-tree
+    python wtivo-watertightvoxel_amd/wtv_build.py    # builds in-tree
 
 Outputs (git-ignored, shipped with the tree):
     wtivo-watertightvoxel_amd/libwtv_gpupr.so

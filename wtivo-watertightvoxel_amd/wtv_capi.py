@@ -1,0 +1,108 @@
+"""ctypes binding of libwtv_gpupr.so's C-ABI (include/wtv_gpupr.h).
+
+This is the same boundary a ctypes/cgo/JNI host would bind.  Loading fails
+loudly when the HIP library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwtv_gpupr.so")
+
+WTV_OK, WTV_EINVAL, WTV_EOVERFLOW, WTV_EINVARIANT, WTV_ENOMEM, WTV_ELIMIT, WTV_EDEVICE = range(7)
+
+# every extern "C" symbol declared in include/wtv_gpupr.h
+EXPORTS = ["wtv_params_default", "wtv_pr_solve", "wtv_graph_cut_fast", "wtv_last_stats",
+           "wtv_last_error", "wtv_version"]
+
+
+class WtvStats(C.Structure):
+    _fields_ = [
+        ("solve_seconds", C.c_double), ("final_relabel_seconds", C.c_double),
+        ("solver_workspace_bytes", C.c_uint64), ("rounds", C.c_uint64),
+        ("global_relabels", C.c_uint64), ("raw_flow", C.c_int64),
+        ("visits", C.c_uint64), ("pushes", C.c_uint64), ("relabels", C.c_uint64),
+        ("bfs_pops", C.c_uint64), ("sub_rounds", C.c_uint64), ("bfs_levels", C.c_uint64),
+        ("t_solve_ms", C.c_double), ("t_bfs_ms", C.c_double),
+        ("t_push_ms", C.c_double), ("t_relabel_ms", C.c_double), ("t_level_ms", C.c_double),
+        ("n_push", C.c_uint64), ("n_relabel", C.c_uint64), ("n_level", C.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class WtvParams(C.Structure):
+    _fields_ = [
+        ("global_relabel_period", C.c_int32), ("local_steps", C.c_int32),
+        ("max_rounds", C.c_uint64), ("wg_size", C.c_int32), ("grid_wgs", C.c_int32),
+        ("time_limit_s", C.c_double), ("profile", C.c_int32),
+    ]
+
+
+class WtvCutResult(C.Structure):
+    _fields_ = [
+        ("adjusted_flow", C.c_int64), ("raw_flow", C.c_int64), ("graph_bytes", C.c_uint64),
+        ("workspace_bytes", C.c_uint64), ("full_edges", C.c_uint64), ("free_edges", C.c_uint64),
+        ("fixed_source", C.c_uint64), ("fixed_sink", C.c_uint64), ("terminal_edges", C.c_uint64),
+        ("threads", C.c_int64), ("macro_rounds", C.c_uint64), ("global_relabels", C.c_uint64),
+        ("map_bytes", C.c_uint64), ("build_upload_seconds", C.c_double), ("local_steps", C.c_int64),
+        ("solve_seconds", C.c_double), ("native_total_seconds", C.c_double),
+    ]
+
+
+class WtvError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libwtv_gpupr.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.wtv_params_default.restype = None
+        L.wtv_params_default.argtypes = [C.POINTER(WtvParams)]
+        L.wtv_pr_solve.restype = C.c_int
+        L.wtv_pr_solve.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                   C.POINTER(WtvParams), C.POINTER(WtvStats), C.c_void_p]
+        L.wtv_graph_cut_fast.restype = C.c_int
+        L.wtv_graph_cut_fast.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_size_t,
+                                         C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_uint64, C.c_int,
+                                         C.c_void_p, C.POINTER(WtvCutResult), C.c_void_p]
+        L.wtv_last_stats.restype = None
+        L.wtv_last_stats.argtypes = [C.POINTER(WtvStats)]
+        L.wtv_last_error.restype = C.c_char_p
+        L.wtv_version.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def params(**kw) -> WtvParams:
+    p = WtvParams()
+    lib().wtv_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def check(rc):
+    if rc != WTV_OK:
+        raise WtvError(rc, lib().wtv_last_error().decode())
+
+
+def pr_solve(n, d_nbr, d_res, d_term, d_or_h_partition, partition_on_device, prm=None, stream=0):
+    """Call wtv_pr_solve with raw device pointers (ints); returns WtvStats."""
+    st = WtvStats()
+    prm = prm or params()
+    rc = lib().wtv_pr_solve(n, d_nbr, d_res, d_term, d_or_h_partition, int(partition_on_device),
+                            C.byref(prm), C.byref(st), stream)
+    check(rc)
+    return st

@@ -68,8 +68,15 @@ class TetMesh:
         return int(self.tets.shape[0])
 
 
-def kuhn_mesh(k: int, jitter: float = 0.2, seed: int = 0, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)) -> TetMesh:
-    """Jittered Kuhn 6-tet subdivision of a k^3 grid: 6k^3 tets, (k+1)^3 vertices."""
+def kuhn_mesh(k: int, jitter: float = 0.2, seed: int = 0, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
+              device=None) -> TetMesh:
+    """Jittered Kuhn 6-tet subdivision of a k^3 grid: 6k^3 tets, (k+1)^3 vertices (numpy arrays)."""
+    V, tets, nbrs = kuhn_mesh_t(k, jitter, seed, lo, hi, device=device)
+    return TetMesh(k, V.cpu().numpy(), tets.cpu().numpy(), nbrs.cpu().numpy())
+
+
+def kuhn_vertices(k: int, jitter: float, seed: int, lo, hi) -> np.ndarray:
+    """(k+1)^3 lattice points, x fastest, each jittered by U(-jitter, jitter) cells per axis."""
     rng = np.random.Generator(np.random.PCG64(seed))
     k1 = k + 1
     lo = np.asarray(lo, np.float64)
@@ -79,47 +86,58 @@ def kuhn_mesh(k: int, jitter: float = 0.2, seed: int = 0, lo=(0.0, 0.0, 0.0), hi
     z, y, x = np.meshgrid(g, g, g, indexing="ij")
     V = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
     V += rng.uniform(-jitter, jitter, size=V.shape)
-    V = lo + V * cell
-    tets, nbrs = kuhn_topology(k)
-    return TetMesh(k, V, tets, nbrs)
+    return lo + V * cell
 
 
-def kuhn_topology(k: int, chunk_cells: int = 1 << 21):
-    """Tets and 4-slot neighbours of the Kuhn subdivision, tet id = cell*6 + perm."""
+def kuhn_mesh_t(k: int, jitter: float = 0.2, seed: int = 0, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
+                device=None):
+    """Torch version of kuhn_mesh: (V float64 [nv,3], tets int32 [N,4], nbrs int32 [N,4]) on `device`."""
+    dev = _dev(device)
+    V = torch.from_numpy(kuhn_vertices(k, jitter, seed, lo, hi)).to(dev)
+    tets, nbrs = kuhn_topology_t(k, dev)
+    return V, tets, nbrs
+
+
+def kuhn_topology(k: int):
+    """Tets and 4-slot neighbours of the Kuhn subdivision (numpy), tet id = cell*6 + perm."""
+    t, n = kuhn_topology_t(k, "cpu")
+    return t.numpy(), n.numpy()
+
+
+def kuhn_topology_t(k: int, device=None, chunk_cells: int = 1 << 22):
+    """Kuhn topology built with torch on `device` (int32 [6k^3,4] tets and neighbours)."""
+    dev = _dev(device)
     k1 = k + 1
     ncell = k * k * k
-    tets = np.empty((ncell * 6, 4), np.int32)
-    nbrs = np.empty((ncell * 6, 4), np.int32)
-    e = np.eye(3, dtype=np.int64)
-    vstride = np.array([1, k1, k1 * k1], np.int64)
-    cstride = np.array([1, k, k * k], np.int64)
-    perms = np.array(PERMS, np.int64)
+    tets = torch.empty((ncell * 6, 4), dtype=torch.int32, device=dev)
+    nbrs = torch.empty((ncell * 6, 4), dtype=torch.int32, device=dev)
+    vstride = [1, k1, k1 * k1]
     for c0 in range(0, ncell, chunk_cells):
         c1 = min(ncell, c0 + chunk_cells)
-        c = np.arange(c0, c1, dtype=np.int64)
-        cx = c % k
-        cy = (c // k) % k
-        cz = c // (k * k)
-        cc = np.stack([cx, cy, cz], 1)  # [m,3]
-        base_v = cc @ vstride
-        m = c.shape[0]
-        T = np.empty((m, 6, 4), np.int64)
-        Nb = np.empty((m, 6, 4), np.int64)
-        for pi in range(6):
-            p = perms[pi]
+        c = torch.arange(c0, c1, dtype=torch.int64, device=dev)
+        cc = [c % k, (c // k) % k, c // (k * k)]
+        base_v = cc[0] + cc[1] * k1 + cc[2] * (k1 * k1)
+        m = c1 - c0
+        T = torch.empty((m, 6, 4), dtype=torch.int32, device=dev)
+        Nb = torch.empty((m, 6, 4), dtype=torch.int32, device=dev)
+        for pi, p in enumerate(PERMS):
             v1 = base_v + vstride[p[0]]
             v2 = v1 + vstride[p[1]]
-            v3 = base_v + vstride.sum()
-            T[:, pi, 0] = base_v
-            T[:, pi, 1] = v1
-            T[:, pi, 2] = v2
-            T[:, pi, 3] = v3
+            v3 = base_v + sum(vstride)
+            T[:, pi, 0] = base_v.to(torch.int32)
+            T[:, pi, 1] = v1.to(torch.int32)
+            T[:, pi, 2] = v2.to(torch.int32)
+            T[:, pi, 3] = v3.to(torch.int32)
             for s in range(4):
                 off = _NB_OFF[s, pi]
-                nc = cc + off
-                ok = np.all((nc >= 0) & (nc < k), axis=1)
-                nid = (nc @ cstride) * 6 + _NB_PERM[s, pi]
-                Nb[:, pi, s] = np.where(ok, nid, -1)
+                ok = torch.ones(m, dtype=torch.bool, device=dev)
+                nc = []
+                for ax in range(3):
+                    q = cc[ax] + int(off[ax])
+                    ok &= (q >= 0) & (q < k)
+                    nc.append(q)
+                nid = (nc[0] + nc[1] * k + nc[2] * (k * k)) * 6 + int(_NB_PERM[s, pi])
+                Nb[:, pi, s] = torch.where(ok, nid, torch.full_like(nid, -1)).to(torch.int32)
         tets[c0 * 6:c1 * 6] = T.reshape(-1, 4)
         nbrs[c0 * 6:c1 * 6] = Nb.reshape(-1, 4)
     return tets, nbrs
@@ -242,12 +260,12 @@ def sphere_with_holes(seed: int, r=0.35, hole_r=0.05, holes=3) -> Shape:
 
 
 def genus3(seed: int = 0) -> Shape:
-    """Three fused tori in a row (a genus-3 solid)."""
+    """Three fused tori in a row along x spanning most of the unit box (a genus-3 solid)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     parts = []
     for i in range(3):
-        cx = 0.5 + (i - 1) * 0.27 + rng.uniform(-0.01, 0.01)
-        parts.append(Torus((cx, 0.5, 0.5), (0.0, 0.0, 1.0), 0.14, 0.05))
+        cx = 0.5 + (i - 1) * 0.26 + rng.uniform(-0.01, 0.01)
+        parts.append(Torus((cx, 0.5, 0.5), (0.0, 0.0, 1.0), 0.135, 0.065))
     return Union(parts)
 
 
@@ -324,24 +342,61 @@ class SoftGraph:
     meta: dict = field(default_factory=dict)
 
 
-def soft_graph(mesh: TetMesh, labels: np.ndarray, shape: Shape, lam: float, h: float,
-               device=None, chunk=1 << 21, max_cap=(1 << 30) - 1, max_term=(1 << 31) - 1) -> SoftGraph:
-    """Soft-seed capacity model: terminal = lam * |vol| on the labelled side, facet =
-    area * exp(-|UDF(facet centroid)|/h), fp32, quantised by one power-of-two scale."""
-    dev = _dev(device)
-    N = mesh.N
-    Vt = torch.from_numpy(mesh.V).to(dev)
-    cap = np.empty((N, 4), np.float32)
-    vol = np.empty(N, np.float32)
+def reverse_slots_t(nbrs):
+    """Torch reverse slots: rs[i,k] = slot of i in nbrs[nbrs[i,k]], -1 for hull/missing (int64)."""
+    N = nbrs.shape[0]
+    dev = nbrs.device
+    idx = torch.arange(N, dtype=torch.int64, device=dev)
+    rs = torch.full((N, 4), -1, dtype=torch.int64, device=dev)
+    nb64 = nbrs.to(torch.int64)
+    for k in range(4):
+        j = nb64[:, k]
+        ok = j >= 0
+        row = nb64[torch.where(ok, j, torch.zeros_like(j))]
+        for r in range(3, -1, -1):  # first match wins
+            hit = ok & (row[:, r] == idx)
+            rs[:, k] = torch.where(hit, torch.full_like(j, r), rs[:, k])
+    return rs
+
+
+def tet_sdf_t(V, tets, shape: Shape, chunk=1 << 22):
+    """Shape SDF at tet centroids (torch, on the tensors' device)."""
+    N = tets.shape[0]
+    out = torch.empty(N, dtype=torch.float64, device=V.device)
     for a in range(0, N, chunk):
         b = min(N, a + chunk)
-        t = torch.from_numpy(mesh.tets[a:b].astype(np.int64)).to(dev)
-        P = Vt[t]  # [m,4,3]
+        out[a:b] = shape.sdf(V[tets[a:b].long()].mean(dim=1))
+    return out
+
+
+def flip_mask(N: int, p: float, seed: int) -> np.ndarray:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return rng.uniform(size=N) < p
+
+
+def soft_graph_t(V, tets, nbrs, labels, shape: Shape, lam: float, h: float, rev_slots=None,
+                 chunk=1 << 21, max_cap=(1 << 30) - 1, max_term=(1 << 31) - 1):
+    """Soft-seed capacity model on torch tensors (any device).
+
+    terminal = lam * |vol| (fp32) on the labelled side (+ source for label 0,
+    - sink otherwise); facet = area * exp(-|UDF(facet centroid)|/h) (fp32),
+    one value per facet (the lower tet id's evaluation); all quantised with one
+    power-of-two scale S, the largest with every arc < 2^30 and |term| < 2^31,
+    cap_int = floor(S * cap_f32).  Returns a dict of tensors on V.device:
+    nbr int32 [N,4], res int32 [N,4] (uint32 bit pattern), term int32 [N],
+    cap_f32 [N,4], term_f32 [N], and the python float `scale`.
+    """
+    dev = V.device
+    N = tets.shape[0]
+    cap = torch.empty((N, 4), dtype=torch.float32, device=dev)
+    vol = torch.empty(N, dtype=torch.float32, device=dev)
+    for a in range(0, N, chunk):
+        b = min(N, a + chunk)
+        P = V[tets[a:b].long()]  # [m,4,3]
         d1 = P[:, 1] - P[:, 0]
         d2 = P[:, 2] - P[:, 0]
         d3 = P[:, 3] - P[:, 0]
-        vol[a:b] = (torch.abs((torch.cross(d1, d2, dim=1) * d3).sum(1)) / 6.0).float().cpu().numpy()
-        cols = []
+        vol[a:b] = (torch.abs((torch.cross(d1, d2, dim=1) * d3).sum(1)) / 6.0).float()
         for fi in range(4):
             i0, i1, i2 = FACE_LUT[fi]
             e1 = P[:, i1] - P[:, i0]
@@ -349,33 +404,52 @@ def soft_graph(mesh: TetMesh, labels: np.ndarray, shape: Shape, lam: float, h: f
             area = 0.5 * torch.linalg.vector_norm(torch.cross(e1, e2, dim=1), dim=1)
             cen = (P[:, i0] + P[:, i1] + P[:, i2]) / 3.0
             udf = torch.abs(shape.sdf(cen))
-            cols.append((area * torch.exp(-udf / h)).float())
-        cap[a:b] = torch.stack(cols, 1).cpu().numpy()
-    nb = mesh.nbrs
-    # one value per facet: take the lower tet id's evaluation for both arcs
-    rs = reverse_slots(nb)
-    idx = np.arange(N, dtype=np.int64)[:, None]
-    j = nb.astype(np.int64)
-    take = (j >= 0) & (j < idx)
-    jj = np.where(take, j, 0)
-    rr = np.where(take, rs, 0).astype(np.int64)
-    cap = np.where(take, cap[jj, rr], cap).astype(np.float32)
-    cap[nb < 0] = 0.0
-    tf = (lam * vol).astype(np.float32)
-    term_f32 = np.where(labels == 0, tf, -tf).astype(np.float32)
-    mc = float(cap.max()) if cap.size else 0.0
-    mt = float(np.abs(term_f32).max()) if term_f32.size else 0.0
-    s = math.inf
+            cap[a:b, fi] = (area * torch.exp(-udf / h)).float()
+        del P
+    nb64 = nbrs.to(torch.int64)
+    if rev_slots is None:
+        rs = reverse_slots_t(nbrs)
+    else:
+        rs = torch.tensor(rev_slots, dtype=torch.int64, device=dev).expand(N, 4)
+    for a in range(0, N, chunk):  # symmetrise: the lower id's value wins
+        b = min(N, a + chunk)
+        j = nb64[a:b]
+        idx = torch.arange(a, b, dtype=torch.int64, device=dev)[:, None]
+        take = (j >= 0) & (j < idx)
+        jj = torch.where(take, j, torch.zeros_like(j))
+        rr = torch.where(take, rs[a:b], torch.zeros_like(j))
+        cap[a:b] = torch.where(take, cap[jj, rr], cap[a:b])
+    cap[nbrs < 0] = 0.0
+    tf = (lam * vol).float()
+    term_f32 = torch.where(labels == 0, tf, -tf)
+    mc = float(cap.max()) if N else 0.0
+    mt = float(term_f32.abs().max()) if N else 0.0
+    sc = math.inf
     if mc > 0:
-        s = min(s, max_cap / mc)
+        sc = min(sc, max_cap / mc)
     if mt > 0:
-        s = min(s, max_term / mt)
-    scale = 2.0 ** math.floor(math.log2(s)) if math.isfinite(s) else 1.0
-    res = np.floor(cap.astype(np.float64) * scale).astype(np.int64)
-    tq = np.trunc(term_f32.astype(np.float64) * scale).astype(np.int64)
-    assert res.max(initial=0) <= max_cap and np.abs(tq).max(initial=0) <= max_term
-    return SoftGraph(nb.copy(), res.astype(np.uint32), tq.astype(np.int32), cap, term_f32, scale,
-                     {"lam": lam, "h": h, "N": N})
+        sc = min(sc, max_term / mt)
+    scale = 2.0 ** math.floor(math.log2(sc)) if math.isfinite(sc) else 1.0
+    res = torch.floor(cap.double() * scale).to(torch.int64)
+    term = torch.trunc(term_f32.double() * scale).to(torch.int64)
+    assert int(res.max()) <= max_cap and int(term.abs().max()) <= max_term
+    return {"nbr": nbrs.to(torch.int32).contiguous(), "res": res.to(torch.int32).contiguous(),
+            "term": term.to(torch.int32).contiguous(), "cap_f32": cap, "term_f32": term_f32,
+            "scale": scale}
+
+
+def soft_graph(mesh: TetMesh, labels: np.ndarray, shape: Shape, lam: float, h: float,
+               device=None, rev_slots=None) -> "SoftGraph":
+    """numpy wrapper of soft_graph_t."""
+    dev = _dev(device)
+    V = torch.from_numpy(mesh.V).to(dev)
+    T = torch.from_numpy(mesh.tets).to(dev)
+    Nb = torch.from_numpy(mesh.nbrs).to(dev)
+    L = torch.from_numpy(np.ascontiguousarray(labels, dtype=np.uint8)).to(dev)
+    g = soft_graph_t(V, T, Nb, L, shape, lam, h, rev_slots=rev_slots)
+    return SoftGraph(g["nbr"].cpu().numpy(), g["res"].cpu().numpy().view(np.uint32),
+                     g["term"].cpu().numpy(), g["cap_f32"].cpu().numpy(), g["term_f32"].cpu().numpy(),
+                     g["scale"], {"lam": lam, "h": h, "N": mesh.N})
 
 
 def cut_energy(g: SoftGraph, partition: np.ndarray) -> float:
