@@ -50,6 +50,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20260001)
     ap.add_argument("--period", type=int, default=1024)
     ap.add_argument("--local-steps", type=int, default=8)
+    ap.add_argument("--alpha", type=float, default=0.0, help="adaptive global-relabel factor (0 = default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle check")
     ap.add_argument("--no-profile", action="store_true")
     a = ap.parse_args()
@@ -91,7 +92,7 @@ def main():
         f"scale=2^{int(np.log2(g['scale']))} gen={gen_s:.1f}s")
 
     stream = torch.cuda.current_stream().cuda_stream
-    prm = W.params(global_relabel_period=a.period, local_steps=a.local_steps)
+    prm = W.params(global_relabel_period=a.period, local_steps=a.local_steps, relabel_alpha=a.alpha)
 
     def step(p=prm):
         res.copy_(res0)
@@ -137,12 +138,19 @@ def main():
     # -------------------------------------------------- profiled solve
     prof = None
     if not a.no_profile:
-        pst = step(W.params(global_relabel_period=a.period, local_steps=a.local_steps, profile=1)).as_dict()
+        pst = step(W.params(global_relabel_period=a.period, local_steps=a.local_steps, relabel_alpha=a.alpha,
+                            profile=1)).as_dict()
         assert pst["raw_flow"] == raw_flow
         prof = pst
     s0 = stats[len(stats) // 2]
     n = N
-    bytes_alg = (69 * s0["visits"] + 32 * s0["pushes"] + 4 * s0["relabels"] + 61 * s0["bfs_pops"]
+    # SURVEY byte model; a bottom-up BFS level costs a 4 B height sweep over all
+    # n vertices plus 48 B (nbr 16 + res 16 + 4 neighbour heights 16) per
+    # unreached vertex it examines.
+    def bfs_bytes(x):
+        return 61 * x["bfs_pops"] + 4 * n * x["bfs_bu_levels"] + 48 * x["bfs_bu_checks"]
+
+    bytes_alg = (69 * s0["visits"] + 32 * s0["pushes"] + 4 * s0["relabels"] + bfs_bytes(s0)
                  + 20 * n * s0["global_relabels"] + 2 * n)
     t_solve_s = s0["t_solve_ms"] / 1e3
     roofline = None
@@ -150,9 +158,11 @@ def main():
     if prof:
         kernels = {
             "push": (prof["t_push_ms"], prof["n_push"], 69 * prof["visits"] + 32 * prof["pushes"]),
-            "bfs_level": (prof["t_level_ms"], prof["n_level"], 61 * prof["bfs_pops"]),
+            "bfs_level": (prof["t_level_ms"], prof["n_level"], bfs_bytes(prof)),
+            "relabel": (prof["t_relabel_ms"], prof["n_relabel"], None),
         }
-        name, (tms, cnt, byts) = max(kernels.items(), key=lambda kv: kv[1][0])
+        name, (tms, cnt, byts) = max(((k2, v2) for k2, v2 in kernels.items() if v2[2] is not None),
+                                     key=lambda kv: kv[1][0])
         avg_ms = tms / max(cnt, 1)
         per_launch = byts / max(cnt, 1)
         achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
@@ -202,7 +212,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"kuhn{k}_genus3_softseed_lambda10", "tets": N, "free_nodes": n,
                    "lambda_fill": 10.0, "h_cells": 1.0, "label_flips": 0.02, "seed": a.seed,
-                   "global_relabel_period": a.period, "local_steps": a.local_steps,
+                   "global_relabel_period": a.period, "local_steps": a.local_steps, "relabel_alpha": a.alpha,
                    "stand_in": "Kuhn grid replaces the 12M-point Delaunay mesh"},
         "tets_per_s": round(N / value, 1),
         "t_solve_ms_device": round(s0["t_solve_ms"], 3),
@@ -210,7 +220,8 @@ def main():
         "cut_energy_fp64": energy,
         "certificate": "flow == integer cut capacity",
         "work": {k2: s0[k2] for k2 in ("rounds", "sub_rounds", "global_relabels", "visits", "pushes",
-                                        "relabels", "bfs_pops", "bfs_levels")},
+                                        "relabels", "bfs_pops", "bfs_levels", "bfs_bu_levels",
+                                        "bfs_bu_checks")},
         "solve_roofline": {"bytes_alg": bytes_alg, "achieved_GBs": round(bytes_alg / t_solve_s / 1e9, 2),
                            "frac": round(bytes_alg / t_solve_s / 1e9 / HBM_PEAK_GBS, 5)},
         "roofline": roofline,

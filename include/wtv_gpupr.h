@@ -64,6 +64,8 @@ typedef struct wtv_stats {
     uint64_t n_push;     /* launches of each kind                             */
     uint64_t n_relabel;
     uint64_t n_level;
+    uint64_t bfs_bu_levels;  /* BFS levels expanded bottom-up                */
+    uint64_t bfs_bu_checks;  /* unreached vertices examined bottom-up        */
 } wtv_stats;
 
 /* Solver knobs.  global_relabel_period, max_rounds and local_steps keep the
@@ -78,6 +80,11 @@ typedef struct wtv_params {
     int32_t grid_wgs;              /* 0 = default            */
     double time_limit_s;           /* 0 = 240 s (reference)  */
     int32_t profile;               /* 1 = time every kernel with HIP events */
+    double relabel_alpha;          /* also relabel globally once the height
+                                      increases since the last global relabel
+                                      exceed alpha * (active vertices after it)
+                                      + 1024; 0 = default 4.0, < 0 = only the
+                                      reference period                        */
 } wtv_params;
 
 /* Fill *p with the reference defaults. */

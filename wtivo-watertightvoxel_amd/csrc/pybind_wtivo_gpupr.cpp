@@ -128,6 +128,8 @@ py::dict stats_dict(const wtv_stats& s) {
     d["n_push"] = s.n_push;
     d["n_relabel"] = s.n_relabel;
     d["n_level"] = s.n_level;
+    d["bfs_bu_levels"] = s.bfs_bu_levels;
+    d["bfs_bu_checks"] = s.bfs_bu_checks;
     return d;
 }
 

@@ -57,6 +57,7 @@ struct Ctl {
     uint32_t bfs_levels;
     unsigned long long sink_flow;
     unsigned long long visits, pushes, relabels, bfs_pops;
+    unsigned long long bu_levels, bu_checks;
     int32_t error;
     uint32_t violations;
 };
@@ -91,19 +92,6 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
 }
 
-// Append val to list for every active lane with pred set: one atomic per wave.
-// All active lanes of the wave must reach this call.
-__device__ __forceinline__ void wave_append(bool pred, uint32_t val, uint32_t* list, uint32_t* cnt) {
-    const uint64_t m = __ballot(pred);
-    if (m == 0) return;
-    const uint32_t rank = lane_rank(m);
-    const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
-    uint32_t base = 0;
-    if (pred && rank == 0) base = atom_add(cnt, static_cast<uint32_t>(__popcll(m)));
-    base = __shfl(base, leader);
-    if (pred) list[base + rank] = val;
-}
-
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
@@ -114,6 +102,51 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
 __device__ __forceinline__ void flush_counter(unsigned long long* dst, unsigned long long x) {
     x = wave_sum(x);
     if (__lane_id() == 0 && x) atom_add(dst, x);
+}
+
+// ------------------------------------------- LDS-staged workgroup queue
+// Items are appended to an LDS buffer (one LDS atomic per wave, ranks from
+// ballot + mbcnt), then the workgroup reserves space in the global list with
+// a single device atomic and copies the items out coalesced.  A single global
+// counter sustains only ~1e8 atomics/s, so this is what keeps frontier and
+// worklist appends off the critical path.
+constexpr int kQCap = kTPB * 5;  // max appends per thread per iteration: 5
+struct WgQueue {
+    uint32_t cnt;
+    uint32_t base;
+    uint32_t buf[kQCap];
+};
+
+// All active lanes of the wave must call this (pred may be false).
+__device__ __forceinline__ void q_push(WgQueue& q, bool pred, uint32_t val) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const uint32_t rank = lane_rank(m);
+    const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+    uint32_t off = 0;
+    if (pred && rank == 0) off = atomicAdd(&q.cnt, static_cast<uint32_t>(__popcll(m)));
+    off = __shfl(off, leader);
+    if (pred) q.buf[off + rank] = val;
+}
+
+// Every thread of the workgroup must call this the same number of times.
+__device__ __forceinline__ void q_flush(WgQueue& q, uint32_t* list, uint32_t* gcnt) {
+    __syncthreads();
+    const uint32_t c = q.cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        q.base = c ? atom_add(gcnt, c) : 0u;
+        q.cnt = 0u;
+    }
+    __syncthreads();
+    const uint32_t b = q.base;
+    for (uint32_t i = threadIdx.x; i < c; i += kTPB) list[b + i] = q.buf[i];
+    __syncthreads();
+}
+
+__device__ __forceinline__ void q_init(WgQueue& q) {
+    if (threadIdx.x == 0) q.cnt = 0u;
+    __syncthreads();
 }
 
 __device__ __forceinline__ int slot_of(const int4& v, int k) {
@@ -127,11 +160,12 @@ __device__ __forceinline__ uint32_t slot_of(const uint4& v, int k) {
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
 // Reverse slots, preflow init (gpu_push_relabel_fast.cu:69-83) and mark reset.
-__global__ void k_prepare(uint32_t n, const int4* __restrict__ nbr4, const int32_t* __restrict__ term,
-                          uint8_t* __restrict__ rslot, long long* __restrict__ excess,
-                          uint32_t* __restrict__ mark, Ctl* ctl) {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+__global__ void __launch_bounds__(kTPB) k_prepare(uint32_t n, const int4* __restrict__ nbr4,
+                                                  const int32_t* __restrict__ term, uint8_t* __restrict__ rslot,
+                                                  long long* __restrict__ excess, uint32_t* __restrict__ mark,
+                                                  Ctl* ctl) {
+    const uint32_t stride = gridDim.x * kTPB;
+    for (uint32_t v = blockIdx.x * kTPB + threadIdx.x; v < n; v += stride) {
         const int4 nb = nbr4[v];
         uint32_t code = 0;
         int err = 0;
@@ -156,85 +190,135 @@ __global__ void k_prepare(uint32_t n, const int4* __restrict__ nbr4, const int32
 
 // BFS seeds: vertices with remaining sink capacity get height 1
 // (gpu_push_relabel_fast.cu:85-101); every other vertex starts unreached.
-__global__ void k_bfs_seed(uint32_t n, const int32_t* __restrict__ term, uint32_t* __restrict__ height,
-                           uint32_t* __restrict__ frontier, Ctl* ctl) {
+// No seed list is built: level 1 always expands bottom-up.
+__global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __restrict__ term,
+                                                   uint32_t* __restrict__ height, Ctl* ctl) {
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->bfs_cnt[2] = 0u;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
-        const bool s = term[v] < 0;
-        height[v] = s ? 1u : kSent;
-        wave_append(s, v, frontier, &ctl->bfs_cnt[1]);
-    }
+    const uint32_t stride = gridDim.x * kTPB;
+    for (uint32_t v = blockIdx.x * kTPB + threadIdx.x; v < n; v += stride)
+        height[v] = term[v] < 0 ? 1u : kSent;
 }
 
-// One BFS level: frontier at height d -> unreached v with residual v->u > 0
-// gets height d+1 (gpu_push_relabel_fast.cu:103-139).
-__global__ void k_bfs_level(uint32_t d, const uint32_t* __restrict__ fin, uint32_t* __restrict__ fout,
-                            const int4* __restrict__ nbr4, const uint8_t* __restrict__ rslot,
-                            const uint32_t* __restrict__ res, uint32_t* height, Ctl* ctl) {
+// One BFS level from height d to d+1 (gpu_push_relabel_fast.cu:103-139).
+// Top-down over the frontier list when it is small: frontier u claims each
+// unreached neighbour v with residual v->u > 0 by CAS.  Bottom-up over all
+// vertices when the frontier is large (or for level 1, which has no list):
+// each unreached v checks its own arcs v->u for a residual arc into level d,
+// needing neither the reverse slot nor a CAS.
+__global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint32_t bu_threshold,
+                                                    const uint32_t* __restrict__ fin, uint32_t* __restrict__ fout,
+                                                    const int4* __restrict__ nbr4, const uint8_t* __restrict__ rslot,
+                                                    const uint32_t* __restrict__ res, uint32_t* height, Ctl* ctl) {
+    __shared__ WgQueue q;
+    q_init(q);
     const uint32_t cnt = ctl->bfs_cnt[d % 3u];
     uint32_t* out_cnt = &ctl->bfs_cnt[(d + 1u) % 3u];
+    const bool bottom_up = d == 1u || cnt > bu_threshold;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->bfs_cnt[(d + 2u) % 3u] = 0u;
-        if (cnt) ctl->bfs_levels += 1u;
+        if (cnt || d == 1u) ctl->bfs_levels += 1u;
+        if (bottom_up) ctl->bu_levels += 1u;
     }
-    const uint32_t stride = gridDim.x * blockDim.x;
-    unsigned long long pops = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += stride) {
-        const uint32_t u = fin[i];
-        const int4 nb = nbr4[u];
-        const uint32_t rs = rslot[u];
-        ++pops;
+    const uint32_t stride = gridDim.x * kTPB;
+    unsigned long long work = 0;
+    if (bottom_up) {
+        if (d != 1u && cnt == 0u) return;
+        for (uint32_t base = blockIdx.x * kTPB; base < n; base += stride) {
+            const uint32_t v = base + threadIdx.x;
+            bool found = false;
+            if (v < n && height[v] == kSent) {
+                ++work;
+                const int4 nb = nbr4[v];
+                const uint4 r = reinterpret_cast<const uint4*>(res)[v];
+                uint32_t hu[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int v = slot_of(nb, k);
-            bool claim = false;
-            if (v >= 0 && height[v] == kSent) {
-                const uint32_t kv = (rs >> (2 * k)) & 3u;
-                if (res[static_cast<size_t>(v) * 4u + kv] != 0u)
-                    claim = atom_cas(&height[v], kSent, d + 1u);
+                for (int k = 0; k < 4; ++k) {
+                    const int u = slot_of(nb, k);
+                    hu[k] = (u >= 0 && slot_of(r, k) != 0u) ? height[u] : kSent;
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) found |= hu[k] == d;
+                if (found) height[v] = d + 1u;
             }
-            wave_append(claim, static_cast<uint32_t>(v), fout, out_cnt);
+            q_push(q, found, v);
+            q_flush(q, fout, out_cnt);
         }
+        flush_counter(&ctl->bu_checks, work);
+    } else {
+        for (uint32_t base = blockIdx.x * kTPB; base < cnt; base += stride) {
+            const uint32_t i = base + threadIdx.x;
+            int4 nb = make_int4(-1, -1, -1, -1);
+            uint32_t rs = 0;
+            if (i < cnt) {
+                const uint32_t u = fin[i];
+                nb = nbr4[u];
+                rs = rslot[u];
+                ++work;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int v = slot_of(nb, k);
+                bool claim = false;
+                if (v >= 0 && height[v] == kSent) {
+                    const uint32_t kv = (rs >> (2 * k)) & 3u;
+                    if (res[static_cast<size_t>(v) * 4u + kv] != 0u) claim = atom_cas(&height[v], kSent, d + 1u);
+                }
+                q_push(q, claim, static_cast<uint32_t>(v));
+            }
+            q_flush(q, fout, out_cnt);
+        }
+        flush_counter(&ctl->bfs_pops, work);
     }
-    flush_counter(&ctl->bfs_pops, pops);
 }
 
 // Active set {excess > 0, height < n} (gpu_push_relabel_fast.cu:149-166);
 // unreached vertices are normalised to n+1 (gpu_push_relabel_fast.cu:141-147).
-__global__ void k_build_active(uint32_t n, const long long* __restrict__ excess, uint32_t* __restrict__ height,
-                               uint32_t* __restrict__ wl, uint32_t p, Ctl* ctl) {
+__global__ void __launch_bounds__(kTPB) k_build_active(uint32_t n, const long long* __restrict__ excess,
+                                                       uint32_t* __restrict__ height, uint32_t* __restrict__ wl,
+                                                       uint32_t p, Ctl* ctl) {
+    __shared__ WgQueue q;
+    q_init(q);
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rl_cnt[p] = 0u;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
-        uint32_t h = height[v];
-        if (h == kSent) {
-            h = n + 1u;
-            height[v] = h;
+    const uint32_t stride = gridDim.x * kTPB;
+    for (uint32_t base = blockIdx.x * kTPB; base < n; base += stride) {
+        const uint32_t v = base + threadIdx.x;
+        bool a = false;
+        if (v < n) {
+            uint32_t h = height[v];
+            if (h == kSent) {
+                h = n + 1u;
+                height[v] = h;
+            }
+            a = h < n && excess[v] > 0;
         }
-        const bool a = excess[v] > 0 && h < n;
-        wave_append(a, v, wl, &ctl->wl_cnt[p]);
+        q_push(q, a, v);
+        q_flush(q, wl, &ctl->wl_cnt[p]);
     }
 }
 
 // Push phase of sub-round with parity p (heights read-only).
-__global__ void k_push(uint32_t n, uint32_t p, const uint32_t* __restrict__ wl, uint32_t* __restrict__ rl,
-                       const int4* __restrict__ nbr4, const uint8_t* __restrict__ rslot, uint32_t* res,
-                       int32_t* __restrict__ term, long long* excess, const uint32_t* __restrict__ height,
-                       uint32_t* mark, Ctl* ctl) {
+__global__ void __launch_bounds__(kTPB) k_push(uint32_t n, uint32_t p, const uint32_t* __restrict__ wl,
+                                               uint32_t* __restrict__ rl, const int4* __restrict__ nbr4,
+                                               const uint8_t* __restrict__ rslot, uint32_t* res,
+                                               int32_t* __restrict__ term, long long* excess,
+                                               const uint32_t* __restrict__ height, uint32_t* mark, Ctl* ctl) {
+    __shared__ WgQueue q;
+    q_init(q);
     const uint32_t cnt = ctl->wl_cnt[p];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->wl_cnt[p ^ 1u] = 0u;
     uint32_t* rl_cnt = &ctl->rl_cnt[p];
-    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t stride = gridDim.x * kTPB;
     unsigned long long visits = 0, pushes = 0, sinkf = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += stride) {
-        const uint32_t v = wl[i];
+    for (uint32_t base = blockIdx.x * kTPB; base < cnt; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < cnt;
+        const uint32_t v = valid ? wl[i] : 0u;
         // every worklist vertex is relabel-checked exactly once this sub-round
-        const bool first = atom_xchg(&mark[v], 1u) == 0u;
-        wave_append(first, v, rl, rl_cnt);
+        const bool first = valid && atom_xchg(&mark[v], 1u) == 0u;
+        q_push(q, first, v);
 
-        long long e = atom_load(&excess[v]);
-        const uint32_t h = height[v];
+        long long e = valid ? atom_load(&excess[v]) : 0LL;
+        const uint32_t h = valid ? height[v] : kSent;
         const bool act = e > 0 && h < n;
         int4 nb = make_int4(-1, -1, -1, -1);
         uint4 r = make_uint4(0u, 0u, 0u, 0u);
@@ -279,9 +363,10 @@ __global__ void k_push(uint32_t n, uint32_t p, const uint32_t* __restrict__ wl, 
                 ++pushes;
                 if (old == 0) cand = atom_xchg(&mark[u], 1u) == 0u;
             }
-            wave_append(cand, static_cast<uint32_t>(u), rl, rl_cnt);
+            q_push(q, cand, static_cast<uint32_t>(u));
         }
         if (pushed) atom_add(&excess[v], -pushed);
+        q_flush(q, rl, rl_cnt);
     }
     flush_counter(&ctl->visits, visits);
     flush_counter(&ctl->pushes, pushes);
@@ -290,57 +375,66 @@ __global__ void k_push(uint32_t n, uint32_t p, const uint32_t* __restrict__ wl, 
 
 // Relabel phase of sub-round with parity p: recompute heights of every vertex
 // that was active or received flow, and rebuild the active list.
-__global__ void k_relabel(uint32_t n, uint32_t p, const uint32_t* __restrict__ rl, uint32_t* __restrict__ wl_next,
-                          const int4* __restrict__ nbr4, const uint32_t* __restrict__ res,
-                          const int32_t* __restrict__ term, const long long* __restrict__ excess,
-                          uint32_t* height, uint32_t* __restrict__ mark, Ctl* ctl) {
+__global__ void __launch_bounds__(kTPB) k_relabel(uint32_t n, uint32_t p, const uint32_t* __restrict__ rl,
+                                                  uint32_t* __restrict__ wl_next, const int4* __restrict__ nbr4,
+                                                  const uint32_t* __restrict__ res, const int32_t* __restrict__ term,
+                                                  const long long* __restrict__ excess, uint32_t* height,
+                                                  uint32_t* __restrict__ mark, Ctl* ctl) {
+    __shared__ WgQueue q;
+    q_init(q);
     const uint32_t cnt = ctl->rl_cnt[p];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rl_cnt[p ^ 1u] = 0u;
     uint32_t* wl_cnt = &ctl->wl_cnt[p ^ 1u];
-    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t stride = gridDim.x * kTPB;
     unsigned long long relabels = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += stride) {
-        const uint32_t v = rl[i];
-        mark[v] = 0u;
-        const long long e = excess[v];
-        uint32_t h = height[v];
+    for (uint32_t base = blockIdx.x * kTPB; base < cnt; base += stride) {
+        const uint32_t i = base + threadIdx.x;
         bool act = false;
-        if (e > 0 && h < n) {
-            const int32_t t = term[v];
-            uint32_t mh = kSent;
-            if (t < 0) mh = 0u;
-            else if (t > 0) mh = n;
-            const int4 nb = nbr4[v];
-            const uint4 r = reinterpret_cast<const uint4*>(res)[v];
+        uint32_t v = 0;
+        if (i < cnt) {
+            v = rl[i];
+            mark[v] = 0u;
+            const long long e = excess[v];
+            uint32_t h = height[v];
+            if (e > 0 && h < n) {
+                const int32_t t = term[v];
+                uint32_t mh = kSent;
+                if (t < 0) mh = 0u;
+                else if (t > 0) mh = n;
+                const int4 nb = nbr4[v];
+                const uint4 r = reinterpret_cast<const uint4*>(res)[v];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int u = slot_of(nb, k);
-                if (u >= 0 && slot_of(r, k) != 0u) {
-                    const uint32_t x = height[u];
-                    mh = x < mh ? x : mh;
+                for (int k = 0; k < 4; ++k) {
+                    const int u = slot_of(nb, k);
+                    if (u >= 0 && slot_of(r, k) != 0u) {
+                        const uint32_t x = height[u];
+                        mh = x < mh ? x : mh;
+                    }
                 }
+                if (mh == kSent) {
+                    set_error(ctl, 5);
+                } else if (mh + 1u > h) {
+                    h = mh + 1u;
+                    height[v] = h;
+                    ++relabels;
+                }
+                act = h < n;
             }
-            if (mh == kSent) {
-                set_error(ctl, 5);
-            } else if (mh + 1u > h) {
-                h = mh + 1u;
-                height[v] = h;
-                ++relabels;
-            }
-            act = h < n;
         }
-        wave_append(act, v, wl_next, wl_cnt);
+        q_push(q, act, v);
+        q_flush(q, wl_next, wl_cnt);
     }
     flush_counter(&ctl->relabels, relabels);
 }
 
 // partition = reaches the sink (gpu_push_relabel_fast.cu:315-324); counts any
 // vertex still active after the final BFS (an invariant violation).
-__global__ void k_partition(uint32_t n, const uint32_t* __restrict__ height, const long long* __restrict__ excess,
-                            uint8_t* __restrict__ part, Ctl* ctl) {
-    const uint32_t stride = gridDim.x * blockDim.x;
+__global__ void __launch_bounds__(kTPB) k_partition(uint32_t n, const uint32_t* __restrict__ height,
+                                                    const long long* __restrict__ excess,
+                                                    uint8_t* __restrict__ part, Ctl* ctl) {
+    const uint32_t stride = gridDim.x * kTPB;
     unsigned long long bad = 0;
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    for (uint32_t v = blockIdx.x * kTPB + threadIdx.x; v < n; v += stride) {
         const uint32_t h = height[v];
         part[v] = h < n + 1u ? 1u : 0u;
         if (excess[v] > 0 && h < n) ++bad;
@@ -455,6 +549,8 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
     const int tpb = kTPB;
     int grid = prm.grid_wgs > 0 ? prm.grid_wgs : 1024;
     const int sweep_grid = static_cast<int>(std::min<uint64_t>((static_cast<uint64_t>(n) + tpb - 1) / tpb, 8192));
+    // a level whose frontier holds more than n/32 vertices expands bottom-up
+    const uint32_t bu_threshold = std::max<uint32_t>(n / 32u, 4096u);
     hipStream_t s = static_cast<hipStream_t>(stream_v);
 
     const Layout L = layout_for(n);
@@ -508,14 +604,15 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
     auto global_relabel = [&]() {
         ck(hipEventRecord(ev_b0.e, s), "event");
         k_set_u32<<<1, 1, 0, s>>>(&ctl->bfs_cnt[1], 0u);
-        k_bfs_seed<<<sweep_grid, tpb, 0, s>>>(n, d_term, height, wl[1], ctl);
+        k_bfs_seed<<<sweep_grid, tpb, 0, s>>>(n, d_term, height, ctl);
         ck(hipGetLastError(), "bfs seed");
         uint32_t d = 1;
         const uint32_t batch = 8;
         for (;;) {
             for (uint32_t b = 0; b < batch; ++b, ++d) {
                 kt.begin(KernelTimer::kLevel);
-                k_bfs_level<<<grid, tpb, 0, s>>>(d, wl[d & 1u], wl[(d + 1u) & 1u], nbr4, rslot, d_res, height, ctl);
+                k_bfs_level<<<grid, tpb, 0, s>>>(n, d, bu_threshold, wl[d & 1u], wl[(d + 1u) & 1u], nbr4, rslot,
+                                                 d_res, height, ctl);
                 kt.end();
             }
             ck(hipGetLastError(), "bfs expand");
@@ -546,6 +643,11 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
 
     int macro_period = 0;
     if (period > 0) macro_period = std::max(1, (period + local_steps - 1) / local_steps);
+    // Adaptive cadence: a global relabel also runs once the relabel work since
+    // the last one exceeds alpha * (active vertices right after it) + 1024.
+    const double alpha = prm.relabel_alpha == 0.0 ? 4.0 : prm.relabel_alpha;
+    uint64_t relabels_at_gr = hctl->relabels;
+    uint64_t active_at_gr = active;
 
     const auto solve0 = Clock::now();
     while (active) {
@@ -568,9 +670,14 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
             oss << kPrefix << "device invariant failed at discharge code=" << hctl->error;
             fail(WTV_EINVARIANT, oss.str());
         }
-        if (macro_period > 0 && active && (st.rounds % static_cast<uint64_t>(macro_period)) == 0ull) {
+        const bool by_period = macro_period > 0 && (st.rounds % static_cast<uint64_t>(macro_period)) == 0ull;
+        const bool by_work = alpha > 0.0 && static_cast<double>(hctl->relabels - relabels_at_gr) >
+                                                alpha * static_cast<double>(active_at_gr) + 1024.0;
+        if (active && (by_period || by_work)) {
             global_relabel();
             active = build_active();
+            relabels_at_gr = hctl->relabels;
+            active_at_gr = active;
         }
         if ((st.rounds & 255ull) == 0ull) {
             const double el = std::chrono::duration<double>(Clock::now() - solve0).count();
@@ -610,6 +717,8 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
     st.relabels = hctl->relabels;
     st.bfs_pops = hctl->bfs_pops;
     st.bfs_levels = hctl->bfs_levels;
+    st.bfs_bu_levels = hctl->bu_levels;
+    st.bfs_bu_checks = hctl->bu_checks;
     st.sub_rounds = sub;
     float ms = 0.f;
     ck(hipEventElapsedTime(&ms, ev_t0.e, ev_t1.e), "event time");
