@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--alpha", type=float, default=0.0, help="adaptive global-relabel factor (0 = default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle check")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--batch-graphs", type=int, default=256, help="batched leg: number of graphs (0 = skip)")
+    ap.add_argument("--batch-max", type=int, default=4_000_000, help="batched leg: largest graph (tets)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); "
@@ -196,6 +198,11 @@ def main():
                          f"1 thread; flow and labels equal to the GPU's",
                "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
 
+    batched = None
+    if a.batch_graphs > 0:
+        del res, term, part
+        batched = batch_leg(a, torch, np, W, S, dev, stream)
+
     value = ms_per_step / 1e3
     line = {
         "metric": "cut solve s @78M tets",
@@ -227,9 +234,72 @@ def main():
         "roofline": roofline,
         "kernels_ms": {kk: round(vv[0], 3) for kk, vv in kernels.items()} if kernels else None,
         "cpu_baseline": cpu,
+        "batched": batched,
         "gen_seconds": round(gen_s, 2),
     }
     print(json.dumps(line), flush=True)
+
+
+def batch_leg(a, torch, np, W, S, dev, stream):
+    """Config-2 style batch: G uneven soft-seed graphs, looped plain solves vs one batched solve."""
+    t = time.time()
+    graphs = S.batch_workload(a.batch_graphs, a.seed + 2, 50_000, a.batch_max, device=dev)
+    sizes = [g["N"] for g in graphs]
+    off = np.zeros(len(graphs) + 1, np.uint64)
+    off[1:] = np.cumsum(sizes)
+    nbr = torch.cat([g["nbr"] for g in graphs])
+    res0 = torch.cat([g["res"] for g in graphs])
+    term0 = torch.cat([g["term"] for g in graphs])
+    del graphs
+    res, term = res0.clone(), term0.clone()
+    total = int(off[-1])
+    part_loop = torch.empty(total, dtype=torch.uint8, device=dev)
+    part_b = torch.empty(total, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    gen_s = time.time() - t
+    log(f"[bench] batch: {len(sizes)} graphs, {total} tets (min {min(sizes)}, max {max(sizes)}), gen {gen_s:.1f}s")
+    prm = W.params(global_relabel_period=a.period, local_steps=a.local_steps, relabel_alpha=a.alpha)
+
+    def looped():
+        res.copy_(res0)
+        term.copy_(term0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        flows = []
+        for g in range(len(sizes)):
+            o = int(off[g])
+            st = W.pr_solve(sizes[g], nbr.data_ptr() + 16 * o, res.data_ptr() + 16 * o, term.data_ptr() + 4 * o,
+                            part_loop.data_ptr() + o, 1, prm, stream)
+            flows.append(st.raw_flow)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, flows
+
+    def batched():
+        res.copy_(res0)
+        term.copy_(term0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        flows, st = W.batch_solve(off, nbr.data_ptr(), res.data_ptr(), term.data_ptr(), part_b.data_ptr(), 1,
+                                  prm, stream)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, flows, st
+
+    looped()  # warm-up
+    t_loop, f_loop = looped()
+    batched()
+    t_b, f_b, st = batched()
+    assert list(map(int, f_b)) == list(map(int, f_loop)), "batched flows differ from looped solves"
+    assert torch.equal(part_loop, part_b), "batched labels differ from looped solves"
+    log(f"[bench] batch: looped {t_loop*1e3:.1f} ms, batched {t_b*1e3:.1f} ms, x{t_loop / t_b:.2f}")
+    return {"workload": f"{len(sizes)} uneven soft-seed Kuhn graphs, tets log-uniform [50k, {a.batch_max}], "
+                        "random shapes, lambda U[1,30]",
+            "graphs": len(sizes), "tets": total, "min_tets": min(sizes), "max_tets": max(sizes),
+            "looped_s": round(t_loop, 4), "batched_s": round(t_b, 4),
+            "graphs_per_s_looped": round(len(sizes) / t_loop, 2),
+            "graphs_per_s_batched": round(len(sizes) / t_b, 2),
+            "speedup": round(t_loop / t_b, 2), "identical_to_looped": True,
+            "batched_work": {"sub_rounds": st.sub_rounds, "global_relabels": st.global_relabels,
+                             "bfs_levels": st.bfs_levels}}
 
 
 def _cpu_model():

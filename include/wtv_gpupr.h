@@ -14,6 +14,8 @@
  *                          wtivo_gpupr.graph_cut_fast and
  *                          wtivo_gpupr.graph_cut_gpupr_fast_v630
  *                          (native/gpupr/wtivo_gpupr_bindings.cpp:100-397,581-596)
+ *   wtv_batch_solve     <- no single reference function: G independent calls
+ *                          of cellocut_gpu_push_relabel_fast solved together
  */
 #ifndef WTV_GPUPR_H
 #define WTV_GPUPR_H
@@ -104,6 +106,23 @@ void wtv_params_default(wtv_params* p);
 int wtv_pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
                  uint8_t* partition, int partition_on_device,
                  const wtv_params* params, wtv_stats* stats, void* stream);
+
+/*
+ * Batched solve: G independent reduced graphs in one launch sequence.
+ *   offsets     uint64[G+1] host: graph g owns nodes [offsets[g], offsets[g+1])
+ *               of the concatenated arrays; offsets[0] = 0, total < 2^31
+ *   d_nbr       int32[total*4] concatenated neighbour ids, LOCAL to each graph
+ *   d_res       uint32[total*4] concatenated residuals (in/out)
+ *   d_term      int32[total] concatenated terminals (in/out)
+ *   partition   uint8[total] (host or device, as in wtv_pr_solve)
+ *   flows       int64[G] host: raw flow of each graph
+ * Each graph keeps its own source height (its node count), so every graph's
+ * labels and flow equal those of a separate wtv_pr_solve call.
+ */
+int wtv_batch_solve(uint32_t num_graphs, const uint64_t* offsets, const int32_t* d_nbr,
+                    uint32_t* d_res, int32_t* d_term, uint8_t* partition,
+                    int partition_on_device, int64_t* flows, const wtv_params* params,
+                    wtv_stats* stats, void* stream);
 
 /* The 17 scalars of graph_cut_fast's 18-tuple, in tuple order
  * (wtivo_gpupr_bindings.cpp:377-396); labels are written to labels_out. */

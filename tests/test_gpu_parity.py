@@ -219,3 +219,89 @@ def test_larger_graph_certificate_and_bk(orc, torch_cuda):
     assert S.quantised_cut(g.nbr, g.res, g.term, part) == st["raw_flow"]
     b = orc.bk_solve(g.nbr, g.res, g.term)
     assert b.raw_flow == st["raw_flow"] and np.array_equal(b.partition, part)
+
+
+# ------------------------------------------------------------ batched mode
+def _batch(graphs):
+    off = np.zeros(len(graphs) + 1, np.uint64)
+    for i, (nbr, res, term) in enumerate(graphs):
+        off[i + 1] = off[i] + term.shape[0]
+    nbr = np.concatenate([g[0] for g in graphs]).astype(np.int32)
+    res = np.concatenate([g[1] for g in graphs]).astype(np.uint32)
+    term = np.concatenate([g[2] for g in graphs]).astype(np.int32)
+    return off, nbr, res, term
+
+
+def gpu_batch(torch, graphs, device_partition=True, **kw):
+    off, nbr, res, term = _batch(graphs)
+    dn = torch.from_numpy(nbr).cuda()
+    dr = torch.from_numpy(res.view(np.int32)).cuda()
+    dt = torch.from_numpy(term).cuda()
+    n = int(off[-1])
+    stream = torch.cuda.current_stream().cuda_stream
+    if device_partition:
+        part_d = torch.empty(max(n, 1), dtype=torch.uint8, device="cuda")
+        flows, st = W.batch_solve(off, dn.data_ptr(), dr.data_ptr(), dt.data_ptr(), part_d.data_ptr(), 1,
+                                  W.params(**kw), stream)
+        part = part_d.cpu().numpy()[:n]
+    else:
+        part = np.empty(max(n, 1), np.uint8)
+        flows, st = W.batch_solve(off, dn.data_ptr(), dr.data_ptr(), dt.data_ptr(), part.ctypes.data, 0,
+                                  W.params(**kw), stream)
+    return off, flows, part, dr.cpu().numpy().view(np.uint32), dt.cpu().numpy(), st
+
+
+def test_batch_matches_individual_solves(orc, torch_cuda):
+    graphs = []
+    for seed in range(12):
+        graphs.append(S.random_graph(int(50 + seed * 37), 500 + seed, cap_max=25))
+    # the quirk graph: its answer depends on its own source height (2), not the union's
+    graphs.append((np.array([[1, -1, -1, -1], [0, -1, -1, -1]], np.int32),
+                   np.array([[5, 0, 0, 0], [0, 0, 0, 0]], np.uint32), np.array([5, -5], np.int32)))
+    graphs.append(tuple(np.zeros((0, 4), np.int32) if i < 2 else np.zeros(0, np.int32) for i in range(3)))
+    _, _, g = _c0_soft(16, 8)
+    graphs.append((g.nbr, g.res, g.term))
+    off, flows, part, r_out, t_out, st = gpu_batch(torch_cuda, graphs)
+    for i, (nbr, res, term) in enumerate(graphs):
+        a, b = int(off[i]), int(off[i + 1])
+        if a == b:
+            assert flows[i] == 0
+            continue
+        o = orc.pr_solve(nbr, res, term)
+        assert flows[i] == o.raw_flow, i
+        assert np.array_equal(part[a:b], o.partition), i
+        if orc.certificate(nbr, res, term, o.res, o.term, o.partition, o.raw_flow) == 0:
+            assert orc.certificate(nbr, res, term, r_out[a:b], t_out[a:b], part[a:b], int(flows[i])) == 0
+    assert int(flows.sum()) == st.raw_flow
+
+
+def test_batch_soft_graphs_vs_bk(orc, torch_cuda):
+    rng = np.random.default_rng(3)
+    graphs, metas = [], []
+    for i in range(6):
+        k = int(rng.integers(10, 28))
+        m = S.kuhn_mesh(k, seed=100 + i)
+        shape = S.random_shape(rng)
+        lab = S.labels_from_sdf(S.tet_centroids_sdf(m, shape), 0.02, seed=200 + i)
+        g = S.soft_graph(m, lab, shape, float(rng.uniform(1, 30)), 1.0 / k, rev_slots=S.KUHN_REV_SLOT)
+        graphs.append((g.nbr, g.res, g.term))
+    off, flows, part, _, _, st = gpu_batch(torch_cuda, graphs, device_partition=False, local_steps=4)
+    for i, (nbr, res, term) in enumerate(graphs):
+        b = orc.bk_solve(nbr, res, term)
+        assert flows[i] == b.raw_flow
+        assert np.array_equal(part[int(off[i]):int(off[i + 1])], b.partition)
+
+
+def test_batch_rejects_bad_offsets_and_ids(torch_cuda):
+    nbr, res, term = S.random_graph(40, 9)
+    dn = torch_cuda.from_numpy(nbr).cuda()
+    dr = torch_cuda.from_numpy(res.view(np.int32)).cuda()
+    dt = torch_cuda.from_numpy(term).cuda()
+    part = torch_cuda.empty(40, dtype=torch_cuda.uint8, device="cuda")
+    with pytest.raises(W.WtvError, match="offsets"):
+        W.batch_solve(np.array([0, 30, 20, 40], np.uint64), dn.data_ptr(), dr.data_ptr(), dt.data_ptr(),
+                      part.data_ptr(), 1)
+    # local ids of the second graph exceed its size -> invariant code 3
+    with pytest.raises(W.WtvError, match="code=3"):
+        W.batch_solve(np.array([0, 20, 40], np.uint64), dn.data_ptr(), dr.data_ptr(), dt.data_ptr(),
+                      part.data_ptr(), 1)

@@ -70,6 +70,21 @@ int wtv_pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, u
     });
 }
 
+int wtv_batch_solve(uint32_t num_graphs, const uint64_t* offsets, const int32_t* d_nbr, uint32_t* d_res,
+                    int32_t* d_term, uint8_t* partition, int partition_on_device, int64_t* flows,
+                    const wtv_params* params, wtv_stats* stats, void* stream) {
+    return guarded([&]() {
+        wtv_params p;
+        wtv_params_default(&p);
+        if (params) p = *params;
+        wtv_stats st{};
+        wtv::batch_solve(num_graphs, offsets, d_nbr, d_res, d_term, partition, partition_on_device != 0, flows, p,
+                         st, stream);
+        wtv::set_last_stats(st);
+        if (stats) *stats = st;
+    });
+}
+
 int wtv_graph_cut_fast(const double* V, size_t nv, const int32_t* d_tets, const int32_t* d_nbr, size_t N,
                        const uint8_t* labels, double fill_T, int threads, int global_relabel_period,
                        uint64_t max_rounds, int local_steps, uint8_t* labels_out, wtv_cut_result* out,

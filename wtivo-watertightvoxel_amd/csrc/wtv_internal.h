@@ -27,6 +27,11 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
               uint8_t* partition, bool partition_on_device, const wtv_params& p,
               wtv_stats& st, void* stream);
 
+// Solve G independent graphs as one disjoint union (wtv_solver.hip).
+void batch_solve(uint32_t G, const uint64_t* h_off, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
+                 uint8_t* partition, bool partition_on_device, int64_t* h_flows, const wtv_params& p,
+                 wtv_stats& st, void* stream);
+
 // Thread-local stats of the last solve (wtv_capi.cpp).
 void set_last_stats(const wtv_stats& st);
 

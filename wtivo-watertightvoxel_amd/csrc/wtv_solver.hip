@@ -156,8 +156,46 @@ __device__ __forceinline__ uint32_t slot_of(const uint4& v, int k) {
     return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
 }
 
+// Source height of v: n for a single graph; in batched mode the node count
+// of v's own graph, so every graph keeps its own height conventions.
+template <bool kB>
+__device__ __forceinline__ uint32_t src_h(uint32_t v, uint32_t n, const uint32_t* __restrict__ gid,
+                                          const uint32_t* __restrict__ gn) {
+    if constexpr (kB) return gn[gid[v]];
+    else return n;
+}
+
 // ---------------------------------------------------------------- kernels
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+
+// Batched mode: graph g occupies nodes [off[g], off[g+1]) of the union and
+// its neighbour ids are local to g.  Writes union-wide ids and the graph id.
+__global__ void __launch_bounds__(kTPB) k_globalize(uint32_t n, uint32_t G, const unsigned long long* __restrict__ off,
+                                                    const int4* __restrict__ nbr_local, int4* __restrict__ nbr_global,
+                                                    uint32_t* __restrict__ gid, Ctl* ctl) {
+    const uint32_t stride = gridDim.x * kTPB;
+    for (uint32_t v = blockIdx.x * kTPB + threadIdx.x; v < n; v += stride) {
+        uint32_t lo = 0, hi = G;  // largest g with off[g] <= v
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (off[mid] <= v) lo = mid; else hi = mid;
+        }
+        const uint32_t base = static_cast<uint32_t>(off[lo]);
+        const uint32_t ng = static_cast<uint32_t>(off[lo + 1] - off[lo]);
+        const int4 a = nbr_local[v];
+        int4 b;
+        int err = 0;
+        auto map = [&](int x) -> int {
+            if (x < 0) return -1;
+            if (static_cast<uint32_t>(x) >= ng) { err = 3; return -1; }
+            return static_cast<int>(base + static_cast<uint32_t>(x));
+        };
+        b.x = map(a.x); b.y = map(a.y); b.z = map(a.z); b.w = map(a.w);
+        if (err) set_error(ctl, err);
+        nbr_global[v] = b;
+        gid[v] = lo;
+    }
+}
 
 // Reverse slots, preflow init (gpu_push_relabel_fast.cu:69-83) and mark reset.
 __global__ void __launch_bounds__(kTPB) k_prepare(uint32_t n, const int4* __restrict__ nbr4,
@@ -273,9 +311,11 @@ __global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint
 
 // Active set {excess > 0, height < n} (gpu_push_relabel_fast.cu:149-166);
 // unreached vertices are normalised to n+1 (gpu_push_relabel_fast.cu:141-147).
+template <bool kB>
 __global__ void __launch_bounds__(kTPB) k_build_active(uint32_t n, const long long* __restrict__ excess,
                                                        uint32_t* __restrict__ height, uint32_t* __restrict__ wl,
-                                                       uint32_t p, Ctl* ctl) {
+                                                       uint32_t p, const uint32_t* __restrict__ gid,
+                                                       const uint32_t* __restrict__ gn, Ctl* ctl) {
     __shared__ WgQueue q;
     q_init(q);
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rl_cnt[p] = 0u;
@@ -284,12 +324,13 @@ __global__ void __launch_bounds__(kTPB) k_build_active(uint32_t n, const long lo
         const uint32_t v = base + threadIdx.x;
         bool a = false;
         if (v < n) {
+            const uint32_t sh = src_h<kB>(v, n, gid, gn);
             uint32_t h = height[v];
             if (h == kSent) {
-                h = n + 1u;
+                h = sh + 1u;
                 height[v] = h;
             }
-            a = h < n && excess[v] > 0;
+            a = h < sh && excess[v] > 0;
         }
         q_push(q, a, v);
         q_flush(q, wl, &ctl->wl_cnt[p]);
@@ -297,11 +338,14 @@ __global__ void __launch_bounds__(kTPB) k_build_active(uint32_t n, const long lo
 }
 
 // Push phase of sub-round with parity p (heights read-only).
+template <bool kB>
 __global__ void __launch_bounds__(kTPB) k_push(uint32_t n, uint32_t p, const uint32_t* __restrict__ wl,
                                                uint32_t* __restrict__ rl, const int4* __restrict__ nbr4,
                                                const uint8_t* __restrict__ rslot, uint32_t* res,
                                                int32_t* __restrict__ term, long long* excess,
-                                               const uint32_t* __restrict__ height, uint32_t* mark, Ctl* ctl) {
+                                               const uint32_t* __restrict__ height, uint32_t* mark,
+                                               const uint32_t* __restrict__ gid, const uint32_t* __restrict__ gn,
+                                               unsigned long long* __restrict__ gflow, Ctl* ctl) {
     __shared__ WgQueue q;
     q_init(q);
     const uint32_t cnt = ctl->wl_cnt[p];
@@ -319,7 +363,7 @@ __global__ void __launch_bounds__(kTPB) k_push(uint32_t n, uint32_t p, const uin
 
         long long e = valid ? atom_load(&excess[v]) : 0LL;
         const uint32_t h = valid ? height[v] : kSent;
-        const bool act = e > 0 && h < n;
+        const bool act = e > 0 && h < (valid ? src_h<kB>(v, n, gid, gn) : 0u);
         int4 nb = make_int4(-1, -1, -1, -1);
         uint4 r = make_uint4(0u, 0u, 0u, 0u);
         uint32_t rs = 0;
@@ -345,6 +389,7 @@ __global__ void __launch_bounds__(kTPB) k_push(uint32_t n, uint32_t p, const uin
                 pushed += dlt;
                 sinkf += static_cast<unsigned long long>(dlt);
                 ++pushes;
+                if constexpr (kB) atom_add(&gflow[gid[v]], static_cast<unsigned long long>(dlt));
             }
         }
 #pragma unroll
@@ -375,11 +420,13 @@ __global__ void __launch_bounds__(kTPB) k_push(uint32_t n, uint32_t p, const uin
 
 // Relabel phase of sub-round with parity p: recompute heights of every vertex
 // that was active or received flow, and rebuild the active list.
+template <bool kB>
 __global__ void __launch_bounds__(kTPB) k_relabel(uint32_t n, uint32_t p, const uint32_t* __restrict__ rl,
                                                   uint32_t* __restrict__ wl_next, const int4* __restrict__ nbr4,
                                                   const uint32_t* __restrict__ res, const int32_t* __restrict__ term,
                                                   const long long* __restrict__ excess, uint32_t* height,
-                                                  uint32_t* __restrict__ mark, Ctl* ctl) {
+                                                  uint32_t* __restrict__ mark, const uint32_t* __restrict__ gid,
+                                                  const uint32_t* __restrict__ gn, Ctl* ctl) {
     __shared__ WgQueue q;
     q_init(q);
     const uint32_t cnt = ctl->rl_cnt[p];
@@ -396,11 +443,12 @@ __global__ void __launch_bounds__(kTPB) k_relabel(uint32_t n, uint32_t p, const 
             mark[v] = 0u;
             const long long e = excess[v];
             uint32_t h = height[v];
-            if (e > 0 && h < n) {
+            const uint32_t sh = src_h<kB>(v, n, gid, gn);
+            if (e > 0 && h < sh) {
                 const int32_t t = term[v];
                 uint32_t mh = kSent;
                 if (t < 0) mh = 0u;
-                else if (t > 0) mh = n;
+                else if (t > 0) mh = sh;
                 const int4 nb = nbr4[v];
                 const uint4 r = reinterpret_cast<const uint4*>(res)[v];
 #pragma unroll
@@ -418,7 +466,7 @@ __global__ void __launch_bounds__(kTPB) k_relabel(uint32_t n, uint32_t p, const 
                     height[v] = h;
                     ++relabels;
                 }
-                act = h < n;
+                act = h < sh;
             }
         }
         q_push(q, act, v);
@@ -429,15 +477,18 @@ __global__ void __launch_bounds__(kTPB) k_relabel(uint32_t n, uint32_t p, const 
 
 // partition = reaches the sink (gpu_push_relabel_fast.cu:315-324); counts any
 // vertex still active after the final BFS (an invariant violation).
+template <bool kB>
 __global__ void __launch_bounds__(kTPB) k_partition(uint32_t n, const uint32_t* __restrict__ height,
                                                     const long long* __restrict__ excess,
-                                                    uint8_t* __restrict__ part, Ctl* ctl) {
+                                                    uint8_t* __restrict__ part, const uint32_t* __restrict__ gid,
+                                                    const uint32_t* __restrict__ gn, Ctl* ctl) {
     const uint32_t stride = gridDim.x * kTPB;
     unsigned long long bad = 0;
     for (uint32_t v = blockIdx.x * kTPB + threadIdx.x; v < n; v += stride) {
         const uint32_t h = height[v];
-        part[v] = h < n + 1u ? 1u : 0u;
-        if (excess[v] > 0 && h < n) ++bad;
+        const uint32_t sh = src_h<kB>(v, n, gid, gn);
+        part[v] = h < sh + 1u ? 1u : 0u;
+        if (excess[v] > 0 && h < sh) ++bad;
     }
     bad = wave_sum(bad);
     if (__lane_id() == 0 && bad) atom_add(&ctl->violations, static_cast<uint32_t>(bad));
@@ -511,10 +562,13 @@ struct KernelTimer {
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 struct Layout {
-    size_t height, excess, mark, rslot, wl0, wl1, rl, ctl, total;
+    size_t height, excess, mark, rslot, wl0, wl1, rl, ctl, gid, nbrg, gn, goff, gflow, total;
 };
 
-Layout layout_for(uint32_t n) {
+// One allocation carved into 256-byte aligned arrays.  Batched mode adds the
+// graph id per node, a union-wide copy of the neighbour ids and three
+// per-graph tables (node count, offsets, sink flow).
+Layout layout_for(uint32_t n, uint32_t G) {
     Layout L{};
     size_t o = 0;
     auto take = [&](size_t bytes) { size_t at = o; o = align_up(o + bytes, 256); return at; };
@@ -527,14 +581,28 @@ Layout layout_for(uint32_t n) {
     L.wl1 = take(N * 4u);
     L.rl = take(N * 4u);
     L.rslot = take(N);
+    if (G) {
+        L.gid = take(N * 4u);
+        L.nbrg = take(N * 16u);
+        L.gn = take(static_cast<size_t>(G) * 4u);
+        L.goff = take((static_cast<size_t>(G) + 1u) * 8u);
+        L.gflow = take(static_cast<size_t>(G) * 8u);
+    }
     L.total = o;
     return L;
 }
 
+struct BatchArgs {
+    uint32_t G;
+    const uint64_t* h_off;  // G+1 node offsets (host)
+    int64_t* h_flows;       // G raw flows (host, out)
+};
+
 }  // namespace
 
-void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,
-              bool partition_on_device, const wtv_params& prm, wtv_stats& st, void* stream_v) {
+template <bool kB>
+void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,
+             bool partition_on_device, const wtv_params& prm, wtv_stats& st, void* stream_v, const BatchArgs* B) {
     using Clock = std::chrono::steady_clock;
     std::memset(&st, 0, sizeof(st));
     if (!n) return;
@@ -553,7 +621,8 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
     const uint32_t bu_threshold = std::max<uint32_t>(n / 32u, 4096u);
     hipStream_t s = static_cast<hipStream_t>(stream_v);
 
-    const Layout L = layout_for(n);
+    const uint32_t G = kB ? B->G : 0u;
+    const Layout L = layout_for(n, G);
     DeviceBuf ws;
     ck(hipMalloc(&ws.p, L.total), "workspace malloc");
     st.solver_workspace_bytes = L.total;
@@ -566,6 +635,10 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
     uint32_t* rl = reinterpret_cast<uint32_t*>(base + L.rl);
     uint8_t* rslot = reinterpret_cast<uint8_t*>(base + L.rslot);
     const int4* nbr4 = reinterpret_cast<const int4*>(d_nbr);
+    uint32_t* gid = kB ? reinterpret_cast<uint32_t*>(base + L.gid) : nullptr;
+    uint32_t* gn = kB ? reinterpret_cast<uint32_t*>(base + L.gn) : nullptr;
+    unsigned long long* goff = kB ? reinterpret_cast<unsigned long long*>(base + L.goff) : nullptr;
+    unsigned long long* gflow = kB ? reinterpret_cast<unsigned long long*>(base + L.gflow) : nullptr;
 
     HostBuf hctl_buf;
     ck(hipHostMalloc(&hctl_buf.p, sizeof(Ctl), hipHostMallocDefault), "pinned ctl");
@@ -594,7 +667,21 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
     ktp = &kt;
 
     ck(hipMemsetAsync(ctl, 0, sizeof(Ctl), s), "ctl clear");
+    std::vector<uint32_t> h_gn;
+    if constexpr (kB) {
+        h_gn.resize(G);
+        for (uint32_t g = 0; g < G; ++g) h_gn[g] = static_cast<uint32_t>(B->h_off[g + 1] - B->h_off[g]);
+        ck(hipMemcpyAsync(gn, h_gn.data(), G * 4u, hipMemcpyHostToDevice, s), "batch sizes");
+        ck(hipMemcpyAsync(goff, B->h_off, (G + 1u) * 8u, hipMemcpyHostToDevice, s), "batch offsets");
+        ck(hipMemsetAsync(gflow, 0, G * 8u, s), "batch flows");
+    }
     ck(hipEventRecord(ev_t0.e, s), "event");
+    if constexpr (kB) {
+        int4* nbrg = reinterpret_cast<int4*>(base + L.nbrg);
+        k_globalize<<<sweep_grid, tpb, 0, s>>>(n, G, goff, reinterpret_cast<const int4*>(d_nbr), nbrg, gid, ctl);
+        ck(hipGetLastError(), "globalize");
+        nbr4 = nbrg;
+    }
     k_prepare<<<sweep_grid, tpb, 0, s>>>(n, nbr4, d_term, rslot, excess, mark, ctl);
     ck(hipGetLastError(), "prepare");
     check_err("prepare");
@@ -631,7 +718,7 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
     auto build_active = [&]() -> uint32_t {
         const uint32_t p = static_cast<uint32_t>(sub & 1u);
         k_set_u32<<<1, 1, 0, s>>>(&ctl->wl_cnt[p], 0u);
-        k_build_active<<<sweep_grid, tpb, 0, s>>>(n, excess, height, wl[p], p, ctl);
+        k_build_active<kB><<<sweep_grid, tpb, 0, s>>>(n, excess, height, wl[p], p, gid, gn, ctl);
         ck(hipGetLastError(), "build active");
         read_ctl("active count");
         return hctl->wl_cnt[p];
@@ -655,10 +742,12 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
         for (int j = 0; j < local_steps; ++j, ++sub) {
             const uint32_t p = static_cast<uint32_t>(sub & 1u);
             kt.begin(KernelTimer::kPush);
-            k_push<<<grid, tpb, 0, s>>>(n, p, wl[p], rl, nbr4, rslot, d_res, d_term, excess, height, mark, ctl);
+            k_push<kB><<<grid, tpb, 0, s>>>(n, p, wl[p], rl, nbr4, rslot, d_res, d_term, excess, height, mark, gid,
+                                            gn, gflow, ctl);
             kt.end();
             kt.begin(KernelTimer::kRelabel);
-            k_relabel<<<grid, tpb, 0, s>>>(n, p, rl, wl[p ^ 1u], nbr4, d_res, d_term, excess, height, mark, ctl);
+            k_relabel<kB><<<grid, tpb, 0, s>>>(n, p, rl, wl[p ^ 1u], nbr4, d_res, d_term, excess, height, mark,
+                                               gid, gn, ctl);
             kt.end();
         }
         ck(hipGetLastError(), "discharge");
@@ -692,7 +781,7 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
     st.final_relabel_seconds = std::chrono::duration<double>(Clock::now() - fr0).count();
 
     uint8_t* d_part = partition_on_device ? partition : reinterpret_cast<uint8_t*>(rl);
-    k_partition<<<sweep_grid, tpb, 0, s>>>(n, height, excess, d_part, ctl);
+    k_partition<kB><<<sweep_grid, tpb, 0, s>>>(n, height, excess, d_part, gid, gn, ctl);
     ck(hipGetLastError(), "partition");
     ck(hipEventRecord(ev_t1.e, s), "event");
     if (!partition_on_device)
@@ -712,6 +801,17 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
     if (hctl->sink_flow > static_cast<unsigned long long>(INT64_MAX))
         fail(WTV_EINVARIANT, std::string(kPrefix) + "flow exceeds int64");
     st.raw_flow = static_cast<int64_t>(hctl->sink_flow);
+    if constexpr (kB) {
+        std::vector<unsigned long long> f(G);
+        ck(hipMemcpyAsync(f.data(), gflow, G * 8u, hipMemcpyDeviceToHost, s), "batch flows D2H");
+        ck(hipStreamSynchronize(s), "batch flows");
+        unsigned long long sum = 0;
+        for (uint32_t g = 0; g < G; ++g) {
+            B->h_flows[g] = static_cast<int64_t>(f[g]);
+            sum += f[g];
+        }
+        if (sum != hctl->sink_flow) fail(WTV_EINVARIANT, std::string(kPrefix) + "per-graph flows do not add up");
+    }
     st.visits = hctl->visits;
     st.pushes = hctl->pushes;
     st.relabels = hctl->relabels;
@@ -731,6 +831,29 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
     st.n_push = kt.count[KernelTimer::kPush];
     st.n_relabel = kt.count[KernelTimer::kRelabel];
     st.n_level = kt.count[KernelTimer::kLevel];
+}
+
+void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,
+              bool partition_on_device, const wtv_params& prm, wtv_stats& st, void* stream) {
+    solve_t<false>(n, d_nbr, d_res, d_term, partition, partition_on_device, prm, st, stream, nullptr);
+}
+
+void batch_solve(uint32_t G, const uint64_t* h_off, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
+                 uint8_t* partition, bool partition_on_device, int64_t* h_flows, const wtv_params& prm,
+                 wtv_stats& st, void* stream) {
+    std::memset(&st, 0, sizeof(st));
+    if (!G) return;
+    if (!h_off || !h_flows) fail(WTV_EINVAL, std::string(kPrefix) + "null batch pointer");
+    if (h_off[0] != 0) fail(WTV_EINVAL, std::string(kPrefix) + "batch offsets must start at 0");
+    for (uint32_t g = 0; g < G; ++g) {
+        if (h_off[g + 1] < h_off[g]) fail(WTV_EINVAL, std::string(kPrefix) + "batch offsets must not decrease");
+        h_flows[g] = 0;
+    }
+    const uint64_t total = h_off[G];
+    if (total > 0x7FFFFFFFull) fail(WTV_EINVAL, std::string(kPrefix) + ">INT32_MAX reduced nodes unsupported");
+    BatchArgs B{G, h_off, h_flows};
+    solve_t<true>(static_cast<uint32_t>(total), d_nbr, d_res, d_term, partition, partition_on_device, prm, st,
+                  stream, &B);
 }
 
 }  // namespace wtv

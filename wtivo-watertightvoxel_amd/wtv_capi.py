@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libwtv_gpupr.so")
 WTV_OK, WTV_EINVAL, WTV_EOVERFLOW, WTV_EINVARIANT, WTV_ENOMEM, WTV_ELIMIT, WTV_EDEVICE = range(7)
 
 # every extern "C" symbol declared in include/wtv_gpupr.h
-EXPORTS = ["wtv_params_default", "wtv_pr_solve", "wtv_graph_cut_fast", "wtv_last_stats",
+EXPORTS = ["wtv_params_default", "wtv_pr_solve", "wtv_batch_solve", "wtv_graph_cut_fast", "wtv_last_stats",
            "wtv_last_error", "wtv_version"]
 
 
@@ -74,6 +74,9 @@ def lib():
         L.wtv_pr_solve.restype = C.c_int
         L.wtv_pr_solve.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                    C.POINTER(WtvParams), C.POINTER(WtvStats), C.c_void_p]
+        L.wtv_batch_solve.restype = C.c_int
+        L.wtv_batch_solve.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_int, C.c_void_p, C.POINTER(WtvParams), C.POINTER(WtvStats), C.c_void_p]
         L.wtv_graph_cut_fast.restype = C.c_int
         L.wtv_graph_cut_fast.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_size_t,
                                          C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_uint64, C.c_int,
@@ -107,3 +110,18 @@ def pr_solve(n, d_nbr, d_res, d_term, d_or_h_partition, partition_on_device, prm
                             C.byref(prm), C.byref(st), stream)
     check(rc)
     return st
+
+
+def batch_solve(offsets, d_nbr, d_res, d_term, d_or_h_partition, partition_on_device, prm=None, stream=0):
+    """Call wtv_batch_solve; offsets is a host uint64 numpy array of G+1 entries.
+    Returns (flows int64[G] numpy, WtvStats)."""
+    import numpy as np
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    G = off.shape[0] - 1
+    flows = np.zeros(max(G, 1), np.int64)
+    st = WtvStats()
+    prm = prm or params()
+    rc = lib().wtv_batch_solve(G, off.ctypes.data, d_nbr, d_res, d_term, d_or_h_partition,
+                               int(partition_on_device), flows.ctypes.data, C.byref(prm), C.byref(st), stream)
+    check(rc)
+    return flows[:G], st

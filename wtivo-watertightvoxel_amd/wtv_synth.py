@@ -530,3 +530,29 @@ def random_graph(n: int, seed: int, edges_per_node: float = 1.6, cap_max: int = 
     sign = np.where(rng.uniform(size=n) < 0.5, 1, -1)
     term[has] = (mag * sign)[has]
     return nbr, res, term
+
+
+def batch_workload(count: int, seed: int, n_min: int = 50_000, n_max: int = 4_000_000, device=None,
+                   lam_range=(1.0, 30.0), flip: float = 0.02):
+    """Uneven batch of soft-seed Kuhn graphs: tet counts log-uniform in [n_min, n_max]
+    (k = round((N/6)^(1/3))), each with a random shape (union of 1-4 spheres or tori,
+    0-5 punched holes) and lambda_fill ~ U(lam_range).  Returns a list of dicts with
+    device tensors nbr/res/term (local ids) and metadata."""
+    dev = _dev(device)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(count):
+        target = math.exp(rng.uniform(math.log(n_min), math.log(n_max)))
+        k = max(4, int(round((target / 6.0) ** (1.0 / 3.0))))
+        shape = random_shape(rng)
+        lam = float(rng.uniform(*lam_range))
+        V, T, Nb = kuhn_mesh_t(k, seed=seed * 1000 + i, device=dev)
+        N = T.shape[0]
+        sdf = tet_sdf_t(V, T, shape)
+        lab = (sdf >= 0).to(torch.uint8)
+        if flip > 0:
+            lab ^= torch.from_numpy(flip_mask(N, flip, seed * 1000 + i + 7).astype(np.uint8)).to(dev)
+        g = soft_graph_t(V, T, Nb, lab, shape, lam, 1.0 / k, rev_slots=KUHN_REV_SLOT)
+        out.append({"k": k, "N": N, "lam": lam, "nbr": g["nbr"], "res": g["res"], "term": g["term"]})
+        del V, T, Nb, sdf, lab, g
+    return out
