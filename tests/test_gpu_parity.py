@@ -301,7 +301,8 @@ def test_batch_rejects_bad_offsets_and_ids(torch_cuda):
     with pytest.raises(W.WtvError, match="offsets"):
         W.batch_solve(np.array([0, 30, 20, 40], np.uint64), dn.data_ptr(), dr.data_ptr(), dt.data_ptr(),
                       part.data_ptr(), 1)
-    # local ids of the second graph exceed its size -> invariant code 3
-    with pytest.raises(W.WtvError, match="code=3"):
+    # ids of the first graph exceed its size: rejected as a device invariant
+    # (out-of-range id, and therefore also a missing reverse arc)
+    with pytest.raises(W.WtvError, match="device invariant failed at prepare code=[23]"):
         W.batch_solve(np.array([0, 20, 40], np.uint64), dn.data_ptr(), dr.data_ptr(), dt.data_ptr(),
                       part.data_ptr(), 1)

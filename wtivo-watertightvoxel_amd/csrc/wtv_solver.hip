@@ -53,8 +53,10 @@ constexpr int kTPB = 256;
 struct Ctl {
     uint32_t wl_cnt[2];   // active lists W[0], W[1]
     uint32_t rl_cnt[2];   // relabel list counts (ping-pong)
-    uint32_t bfs_cnt[3];  // BFS frontier counts (ring of 3)
+    uint32_t bfs_cnt[3];  // BFS frontier counts, ring indexed by level % 3
     uint32_t bfs_levels;
+    uint32_t tail_d;      // level reached by the one-workgroup tail kernel
+    uint32_t pad0;
     unsigned long long sink_flow;
     unsigned long long visits, pushes, relabels, bfs_pops;
     unsigned long long bu_levels, bu_checks;
@@ -231,15 +233,48 @@ __global__ void __launch_bounds__(kTPB) k_prepare(uint32_t n, const int4* __rest
 // No seed list is built: level 1 always expands bottom-up.
 __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __restrict__ term,
                                                    uint32_t* __restrict__ height, Ctl* ctl) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->bfs_cnt[2] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->bfs_cnt[1] = 0u;
+        ctl->bfs_cnt[2] = 0u;
+    }
     const uint32_t stride = gridDim.x * kTPB;
     for (uint32_t v = blockIdx.x * kTPB + threadIdx.x; v < n; v += stride)
         height[v] = term[v] < 0 ? 1u : kSent;
 }
 
+// Top-down claims of frontier vertex u (valid lanes only): each unreached
+// neighbour v with residual v->u > 0 gets height d+1 by CAS.  The height of v
+// and the reverse residual are loaded together (both addresses are known once
+// nbr[u] and rslot[u] are in) to keep the dependent-latency chain short.
+__device__ __forceinline__ void td_claims(bool valid, uint32_t u, uint32_t d, const int4* __restrict__ nbr4,
+                                          const uint8_t* __restrict__ rslot, const uint32_t* __restrict__ res,
+                                          uint32_t* height, int4& nb, bool claim[4]) {
+    nb = make_int4(-1, -1, -1, -1);
+    uint32_t rs = 0;
+    if (valid) {
+        nb = nbr4[u];
+        rs = rslot[u];
+    }
+    uint32_t hv[4], rv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int v = slot_of(nb, k);
+        hv[k] = 0u;
+        rv[k] = 0u;
+        if (v >= 0) {
+            hv[k] = height[v];
+            rv[k] = res[static_cast<size_t>(v) * 4u + ((rs >> (2 * k)) & 3u)];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        claim[k] = false;
+        if (hv[k] == kSent && rv[k] != 0u) claim[k] = atom_cas(&height[slot_of(nb, k)], kSent, d + 1u);
+    }
+}
+
 // One BFS level from height d to d+1 (gpu_push_relabel_fast.cu:103-139).
-// Top-down over the frontier list when it is small: frontier u claims each
-// unreached neighbour v with residual v->u > 0 by CAS.  Bottom-up over all
+// Top-down over the frontier list when it is small; bottom-up over all
 // vertices when the frontier is large (or for level 1, which has no list):
 // each unreached v checks its own arcs v->u for a residual arc into level d,
 // needing neither the reverse slot nor a CAS.
@@ -255,7 +290,7 @@ __global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->bfs_cnt[(d + 2u) % 3u] = 0u;
         if (cnt || d == 1u) ctl->bfs_levels += 1u;
-        if (bottom_up) ctl->bu_levels += 1u;
+        if (bottom_up && (cnt || d == 1u)) ctl->bu_levels += 1u;
     }
     const uint32_t stride = gridDim.x * kTPB;
     unsigned long long work = 0;
@@ -285,27 +320,71 @@ __global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint
     } else {
         for (uint32_t base = blockIdx.x * kTPB; base < cnt; base += stride) {
             const uint32_t i = base + threadIdx.x;
-            int4 nb = make_int4(-1, -1, -1, -1);
-            uint32_t rs = 0;
-            if (i < cnt) {
-                const uint32_t u = fin[i];
-                nb = nbr4[u];
-                rs = rslot[u];
-                ++work;
-            }
+            const bool valid = i < cnt;
+            const uint32_t u = valid ? fin[i] : 0u;
+            work += valid;
+            int4 nb;
+            bool claim[4];
+            td_claims(valid, u, d, nbr4, rslot, res, height, nb, claim);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int v = slot_of(nb, k);
-                bool claim = false;
-                if (v >= 0 && height[v] == kSent) {
-                    const uint32_t kv = (rs >> (2 * k)) & 3u;
-                    if (res[static_cast<size_t>(v) * 4u + kv] != 0u) claim = atom_cas(&height[v], kSent, d + 1u);
-                }
-                q_push(q, claim, static_cast<uint32_t>(v));
-            }
+            for (int k = 0; k < 4; ++k) q_push(q, claim[k], static_cast<uint32_t>(slot_of(nb, k)));
             q_flush(q, fout, out_cnt);
         }
         flush_counter(&ctl->bfs_pops, work);
+    }
+}
+
+// BFS tail: one workgroup expands level after level top-down, with
+// __syncthreads between levels instead of a launch per level, while the
+// frontier holds at most kTailMax vertices (up to max_levels levels).  Leaves
+// the frontier of the level it stops at in f[d & 1] with its count in the
+// counter ring, exactly as a sequence of k_bfs_level launches would.
+constexpr uint32_t kTailMax = 1024;
+__global__ void __launch_bounds__(kTPB) k_bfs_tail(uint32_t d0, uint32_t max_levels, uint32_t* f0, uint32_t* f1,
+                                                   const int4* __restrict__ nbr4, const uint8_t* __restrict__ rslot,
+                                                   const uint32_t* __restrict__ res, uint32_t* height, Ctl* ctl) {
+    __shared__ uint32_t s_cnt;
+    uint32_t* f[2] = {f0, f1};
+    uint32_t cur = ctl->bfs_cnt[d0 % 3u];
+    uint32_t d = d0, levels = 0;
+    unsigned long long pops = 0;
+    while (cur > 0u && cur <= kTailMax && levels < max_levels) {
+        const uint32_t* fin = f[d & 1u];
+        uint32_t* fout = f[(d + 1u) & 1u];
+        if (threadIdx.x == 0) s_cnt = 0u;
+        __syncthreads();
+        for (uint32_t base = 0; base < cur; base += kTPB) {
+            const uint32_t i = base + threadIdx.x;
+            const bool valid = i < cur;
+            const uint32_t u = valid ? fin[i] : 0u;
+            pops += valid;
+            int4 nb;
+            bool claim[4];
+            td_claims(valid, u, d, nbr4, rslot, res, height, nb, claim);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint64_t m = __ballot(claim[k]);
+                if (m == 0) continue;
+                const uint32_t rank = lane_rank(m);
+                const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+                uint32_t off = 0;
+                if (claim[k] && rank == 0) off = atomicAdd(&s_cnt, static_cast<uint32_t>(__popcll(m)));
+                off = __shfl(off, leader);
+                if (claim[k]) fout[off + rank] = static_cast<uint32_t>(slot_of(nb, k));
+            }
+        }
+        __syncthreads();
+        cur = s_cnt;
+        ++d;
+        ++levels;
+        __syncthreads();
+    }
+    flush_counter(&ctl->bfs_pops, pops);
+    if (threadIdx.x == 0) {
+        ctl->bfs_cnt[d % 3u] = cur;
+        ctl->bfs_cnt[(d + 1u) % 3u] = 0u;
+        ctl->tail_d = d;
+        ctl->bfs_levels += levels;
     }
 }
 
@@ -690,21 +769,31 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     // BFS from the sink; leaves the frontier buffers dead (wl[] reused).
     auto global_relabel = [&]() {
         ck(hipEventRecord(ev_b0.e, s), "event");
-        k_set_u32<<<1, 1, 0, s>>>(&ctl->bfs_cnt[1], 0u);
         k_bfs_seed<<<sweep_grid, tpb, 0, s>>>(n, d_term, height, ctl);
         ck(hipGetLastError(), "bfs seed");
-        uint32_t d = 1;
+        uint32_t d = 1, last = 0xFFFFFFFFu;
         const uint32_t batch = 8;
         for (;;) {
-            for (uint32_t b = 0; b < batch; ++b, ++d) {
+            if (d > 1u && last <= kTailMax) {
+                // small frontier: one workgroup runs the levels back to back
                 kt.begin(KernelTimer::kLevel);
-                k_bfs_level<<<grid, tpb, 0, s>>>(n, d, bu_threshold, wl[d & 1u], wl[(d + 1u) & 1u], nbr4, rslot,
-                                                 d_res, height, ctl);
+                k_bfs_tail<<<1, tpb, 0, s>>>(d, 4096u, wl[0], wl[1], nbr4, rslot, d_res, height, ctl);
                 kt.end();
+                ck(hipGetLastError(), "bfs tail");
+                read_ctl("bfs tail");
+                d = hctl->tail_d;
+            } else {
+                for (uint32_t b = 0; b < batch; ++b, ++d) {
+                    kt.begin(KernelTimer::kLevel);
+                    k_bfs_level<<<grid, tpb, 0, s>>>(n, d, bu_threshold, wl[d & 1u], wl[(d + 1u) & 1u], nbr4,
+                                                     rslot, d_res, height, ctl);
+                    kt.end();
+                }
+                ck(hipGetLastError(), "bfs expand");
+                read_ctl("bfs count");
             }
-            ck(hipGetLastError(), "bfs expand");
-            read_ctl("bfs count");
-            if (hctl->bfs_cnt[d % 3u] == 0u) break;
+            last = hctl->bfs_cnt[d % 3u];
+            if (last == 0u) break;
             if (static_cast<uint64_t>(d) > static_cast<uint64_t>(n) + 1ull)
                 fail(WTV_EINVARIANT, std::string(kPrefix) + "BFS exceeded n+1 levels");
         }
