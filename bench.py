@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--batch-graphs", type=int, default=256, help="batched leg: number of graphs (0 = skip)")
     ap.add_argument("--batch-max", type=int, default=4_000_000, help="batched leg: largest graph (tets)")
+    ap.add_argument("--serve-requests", type=int, default=48, help="serving leg: number of requests (0 = skip)")
+    ap.add_argument("--serve-load", type=float, default=0.8, help="offered load vs looped-solve capacity")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); "
@@ -203,6 +205,10 @@ def main():
         del res, term, part
         batched = batch_leg(a, torch, np, W, S, dev, stream)
 
+    serving = None
+    if a.serve_requests > 0:
+        serving = serve_leg(a, torch, np, W, S, dev, stream)
+
     value = ms_per_step / 1e3
     line = {
         "metric": "cut solve s @78M tets",
@@ -235,6 +241,7 @@ def main():
         "kernels_ms": {kk: round(vv[0], 3) for kk, vv in kernels.items()} if kernels else None,
         "cpu_baseline": cpu,
         "batched": batched,
+        "serving": serving,
         "gen_seconds": round(gen_s, 2),
     }
     print(json.dumps(line), flush=True)
@@ -300,6 +307,70 @@ def batch_leg(a, torch, np, W, S, dev, stream):
             "speedup": round(t_loop / t_b, 2), "identical_to_looped": True,
             "batched_work": {"sub_rounds": st.sub_rounds, "global_relabels": st.global_relabels,
                              "bfs_levels": st.bfs_levels}}
+
+
+def serve_leg(a, torch, np, W, S, dev, stream):
+    """Config-3 style serving: Poisson arrivals of device-resident cut requests (tets
+    log-uniform [100k, 20M], lambda 10) into the native server (continuous batching,
+    HBM-footprint admission under 200 GB); p50/p99 latency beside graphs/s."""
+    t = time.time()
+    graphs = S.batch_workload(a.serve_requests, a.seed + 3, 100_000, 20_000_000, device=dev, lam_range=(10.0, 10.0))
+    reqs = []
+    for g in graphs:
+        reqs.append({"n": g["N"], "nbr": g["nbr"], "res0": g["res"], "term0": g["term"],
+                     "res": g["res"].clone(), "term": g["term"].clone(),
+                     "part": torch.empty(g["N"], dtype=torch.uint8, device=dev)})
+    del graphs
+    torch.cuda.synchronize()
+    gen_s = time.time() - t
+    prm = dict(global_relabel_period=a.period, local_steps=a.local_steps, relabel_alpha=a.alpha)
+
+    # reference answers and capacity: every request solved alone, back to back
+    want_flow, want_part = [], []
+    t0 = time.perf_counter()
+    for r in reqs:
+        st = W.pr_solve(r["n"], r["nbr"].data_ptr(), r["res"].data_ptr(), r["term"].data_ptr(), r["part"].data_ptr(),
+                        1, W.params(**prm), stream)
+        want_flow.append(st.raw_flow)
+    torch.cuda.synchronize()
+    t_alone = time.perf_counter() - t0
+    for r in reqs:
+        want_part.append(r["part"].clone())
+        r["res"].copy_(r["res0"])
+        r["term"].copy_(r["term0"])
+        r["part"].zero_()
+    torch.cuda.synchronize()
+    qps = a.serve_load * len(reqs) / t_alone
+    rng = np.random.Generator(np.random.PCG64(a.seed + 4))
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, size=len(reqs)))
+    with W.Server(slots=2, max_batch=16, hbm_budget_bytes=200 * 10**9, **prm) as srv:
+        t0 = time.perf_counter()
+        tickets = []
+        for r, at in zip(reqs, arrivals):
+            delay = at - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(r["n"], r["nbr"].data_ptr(), r["res"].data_ptr(), r["term"].data_ptr(),
+                                      r["part"].data_ptr()))
+        results = [srv.wait(tk) for tk in tickets]
+        wall = time.perf_counter() - t0
+    for r, res, wf, wp in zip(reqs, results, want_flow, want_part):
+        assert res.status == 0 and res.raw_flow == wf, "served flow differs from the lone solve"
+        assert torch.equal(r["part"], wp), "served labels differ from the lone solve"
+    lat = np.array([x.end_s - x.submit_s for x in results]) * 1e3
+    bs = [x.batch_size for x in results]
+    sizes = [r["n"] for r in reqs]
+    log(f"[bench] serve: {len(reqs)} requests at {qps:.1f} qps, wall {wall:.2f}s, p50 {np.percentile(lat, 50):.1f} ms, "
+        f"p99 {np.percentile(lat, 99):.1f} ms, mean batch {np.mean(bs):.2f}")
+    return {"workload": f"{len(reqs)} Poisson-arriving soft-seed requests, tets log-uniform [100k, 20M], lambda 10, "
+                        "2 slots, 200 GB admission budget",
+            "requests": len(reqs), "tets": int(sum(sizes)), "max_tets": int(max(sizes)),
+            "qps_offered": round(qps, 3), "load_vs_alone_capacity": a.serve_load,
+            "graphs_per_s": round(len(reqs) / wall, 3), "alone_graphs_per_s": round(len(reqs) / t_alone, 3),
+            "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
+            "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
+            "latency_ms_mean": round(float(lat.mean()), 2), "mean_batch": round(float(np.mean(bs)), 2),
+            "max_batch": int(max(bs)), "identical_to_alone": True, "gen_seconds": round(gen_s, 1)}
 
 
 def _cpu_model():

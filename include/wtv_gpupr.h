@@ -16,6 +16,8 @@
  *                          (native/gpupr/wtivo_gpupr_bindings.cpp:100-397,581-596)
  *   wtv_batch_solve     <- no single reference function: G independent calls
  *                          of cellocut_gpu_push_relabel_fast solved together
+ *   wtv_server_*        <- no reference counterpart: a serving loop around
+ *                          the same solver (continuous batching, HBM admission)
  */
 #ifndef WTV_GPUPR_H
 #define WTV_GPUPR_H
@@ -161,6 +163,52 @@ int wtv_graph_cut_fast(const double* V, size_t nv, const int32_t* d_tets,
                        double fill_T, int threads, int global_relabel_period,
                        uint64_t max_rounds, int local_steps, uint8_t* labels_out,
                        wtv_cut_result* out, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Serving: requests are graphs already resident on the device.  Worker     */
+/* slots (one host thread + one stream each) take every waiting request     */
+/* that fits when they free up and solve them as one batch; admission also  */
+/* requires the HBM footprint of the batch to fit the budget.               */
+/* ---------------------------------------------------------------------- */
+typedef struct wtv_server_config {
+    int32_t slots;             /* concurrent batches on the device (default 2)   */
+    int32_t max_batch;         /* requests per batch (default 64)                */
+    uint64_t max_batch_nodes;  /* nodes per batch (default 2^30)                 */
+    uint64_t hbm_budget_bytes; /* footprint budget (default 200 GB)              */
+    wtv_params params;         /* solver knobs for every batch                   */
+} wtv_server_config;
+
+typedef struct wtv_request {
+    uint32_t n;                /* free nodes                                     */
+    int32_t* d_nbr;            /* int32[n*4], read only                          */
+    uint32_t* d_res;           /* uint32[n*4], in/out                            */
+    int32_t* d_term;           /* int32[n], in/out                               */
+    uint8_t* d_partition;      /* uint8[n] device, out                           */
+} wtv_request;
+
+typedef struct wtv_result {
+    int32_t status;            /* wtv_status of the solve                        */
+    int32_t batch_size;        /* requests solved together with this one         */
+    int32_t slot;              /* worker slot that ran it                        */
+    int32_t pad;
+    int64_t raw_flow;
+    uint64_t footprint_bytes;  /* admitted HBM footprint of this request          */
+    double submit_s;           /* seconds since server start                     */
+    double start_s;
+    double end_s;
+} wtv_result;
+
+/* Footprint a request of n nodes is admitted with (graph + workspace bytes). */
+uint64_t wtv_request_footprint(uint32_t n);
+
+/* Create a server on the current device; NULL on failure (see wtv_last_error). */
+void* wtv_server_create(const wtv_server_config* cfg);
+/* Queue a request; returns a ticket.  Fails if the request alone exceeds the budget. */
+int wtv_server_submit(void* server, const wtv_request* req, uint64_t* ticket);
+/* Block until the ticket's request finished and fill *out. */
+int wtv_server_wait(void* server, uint64_t ticket, wtv_result* out);
+/* Finish all queued work, stop the workers and free the server. */
+int wtv_server_destroy(void* server);
 
 /* Stats of the last wtv_pr_solve / wtv_graph_cut_fast on this thread. */
 void wtv_last_stats(wtv_stats* out);

@@ -78,3 +78,8 @@ def test_pybind_rejects_non_cuda_like_reference():
         G.graph_cut_fast(V, t, t, np.ones(1, np.uint8), 10.0)
     with pytest.raises(RuntimeError, match="vertices must be float64 Nx3"):
         G.graph_cut_fast(np.zeros((4, 2)), t, t, np.ones(1, np.uint8), 10.0)
+
+
+def test_request_footprint_is_linear():
+    f = W.lib().wtv_request_footprint
+    assert f(0) > 0 and f(2000) - f(1000) == f(3000) - f(2000)

@@ -306,3 +306,55 @@ def test_batch_rejects_bad_offsets_and_ids(torch_cuda):
     with pytest.raises(W.WtvError, match="device invariant failed at prepare code=[23]"):
         W.batch_solve(np.array([0, 20, 40], np.uint64), dn.data_ptr(), dr.data_ptr(), dt.data_ptr(),
                       part.data_ptr(), 1)
+
+
+# ------------------------------------------------------------ serving loop
+def _dev_graph(torch, nbr, res, term):
+    return (torch.from_numpy(np.ascontiguousarray(nbr, np.int32)).cuda(),
+            torch.from_numpy(np.ascontiguousarray(res, np.uint32).view(np.int32)).cuda(),
+            torch.from_numpy(np.ascontiguousarray(term, np.int32)).cuda(),
+            torch.empty(max(term.shape[0], 1), dtype=torch.uint8, device="cuda"))
+
+
+def test_server_results_equal_alone_and_in_the_mix(orc, torch_cuda):
+    graphs = [S.random_graph(int(80 + 29 * i), 900 + i, cap_max=40) for i in range(10)]
+    for k, seed in ((12, 1), (20, 2), (16, 3)):
+        _, _, g = _c0_soft(k, seed)
+        graphs.append((g.nbr, g.res, g.term))
+    want = [orc.bk_solve(*g) for g in graphs]
+    torch_cuda.cuda.synchronize()
+    dev = [_dev_graph(torch_cuda, *g) for g in graphs]
+    torch_cuda.cuda.synchronize()
+    with W.Server(slots=2, max_batch=8) as srv:
+        tickets = [srv.submit(g[2].shape[0], d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), d[3].data_ptr())
+                   for g, d in zip(graphs, dev)]
+        res = [srv.wait(t) for t in tickets]
+    sizes = [r.batch_size for r in res]
+    assert max(sizes) > 1, "nothing was batched"
+    for g, d, r, w in zip(graphs, dev, res, want):
+        assert r.status == 0
+        assert r.raw_flow == w.raw_flow
+        assert np.array_equal(d[3].cpu().numpy()[:g[2].shape[0]], w.partition)
+        assert r.submit_s <= r.start_s <= r.end_s
+        # the residuals written back are a maximum preflow of this graph alone
+        assert orc.certificate(g[0], g[1], g[2], d[1].cpu().numpy().view(np.uint32), d[2].cpu().numpy(),
+                               w.partition, r.raw_flow) == 0
+
+
+def test_server_admission_by_footprint(orc, torch_cuda):
+    graphs = [S.random_graph(3000, 950 + i, cap_max=40) for i in range(4)]
+    fp = W.lib().wtv_request_footprint(3000)
+    dev = [_dev_graph(torch_cuda, *g) for g in graphs]
+    torch_cuda.cuda.synchronize()
+    # the budget admits one request at a time: no batching, results unchanged
+    with W.Server(slots=2, max_batch=8, hbm_budget_bytes=int(fp * 1.5)) as srv:
+        tickets = [srv.submit(3000, d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), d[3].data_ptr())
+                   for d in dev]
+        res = [srv.wait(t) for t in tickets]
+        assert all(r.batch_size == 1 for r in res)
+        spans = sorted((r.start_s, r.end_s) for r in res)
+        assert all(spans[i][1] <= spans[i + 1][0] + 1e-6 for i in range(len(spans) - 1)), "budget exceeded"
+        with pytest.raises(W.WtvError, match="HBM budget"):
+            srv.submit(10000, dev[0][0].data_ptr(), dev[0][1].data_ptr(), dev[0][2].data_ptr(), dev[0][3].data_ptr())
+    for g, r in zip(graphs, res):
+        assert r.raw_flow == orc.bk_solve(*g).raw_flow

@@ -9,6 +9,12 @@
 #include "wtv_internal.h"
 
 namespace wtv {
+class Server;
+uint64_t request_footprint(uint32_t n);
+Server* server_new(const wtv_server_config& c);
+void server_delete(Server* s);
+uint64_t server_submit(Server* s, const wtv_request& r);
+wtv_result server_wait(Server* s, uint64_t ticket);
 void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t* dN, size_t N,
                     const uint8_t* lp, double fill_T, int requested_threads, int period, uint64_t max_rounds,
                     int local_steps, uint8_t* out_labels, wtv_cut_result& R, wtv_stats& st, void* stream_v);
@@ -97,6 +103,37 @@ int wtv_graph_cut_fast(const double* V, size_t nv, const int32_t* d_tets, const 
         wtv::set_last_stats(st);
         if (out) *out = r;
     });
+}
+
+uint64_t wtv_request_footprint(uint32_t n) { return wtv::request_footprint(n); }
+
+void* wtv_server_create(const wtv_server_config* cfg) {
+    wtv::Server* srv = nullptr;
+    const int rc = guarded([&]() {
+        wtv_server_config c{};
+        if (cfg) c = *cfg;
+        else wtv_params_default(&c.params);
+        srv = wtv::server_new(c);
+    });
+    return rc == WTV_OK ? srv : nullptr;
+}
+
+int wtv_server_submit(void* server, const wtv_request* req, uint64_t* ticket) {
+    return guarded([&]() {
+        if (!server || !req || !ticket) throw wtv::Error(WTV_EINVAL, "null server/request/ticket");
+        *ticket = wtv::server_submit(static_cast<wtv::Server*>(server), *req);
+    });
+}
+
+int wtv_server_wait(void* server, uint64_t ticket, wtv_result* out) {
+    return guarded([&]() {
+        if (!server || !out) throw wtv::Error(WTV_EINVAL, "null server/result");
+        *out = wtv::server_wait(static_cast<wtv::Server*>(server), ticket);
+    });
+}
+
+int wtv_server_destroy(void* server) {
+    return guarded([&]() { wtv::server_delete(static_cast<wtv::Server*>(server)); });
 }
 
 void wtv_last_stats(wtv_stats* out) {

@@ -15,7 +15,8 @@ WTV_OK, WTV_EINVAL, WTV_EOVERFLOW, WTV_EINVARIANT, WTV_ENOMEM, WTV_ELIMIT, WTV_E
 
 # every extern "C" symbol declared in include/wtv_gpupr.h
 EXPORTS = ["wtv_params_default", "wtv_pr_solve", "wtv_batch_solve", "wtv_graph_cut_fast", "wtv_last_stats",
-           "wtv_last_error", "wtv_version"]
+           "wtv_last_error", "wtv_version", "wtv_request_footprint", "wtv_server_create", "wtv_server_submit",
+           "wtv_server_wait", "wtv_server_destroy"]
 
 
 class WtvStats(C.Structure):
@@ -54,6 +55,25 @@ class WtvCutResult(C.Structure):
     ]
 
 
+class WtvServerConfig(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("max_batch", C.c_int32), ("max_batch_nodes", C.c_uint64),
+                ("hbm_budget_bytes", C.c_uint64), ("params", WtvParams)]
+
+
+class WtvRequest(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("d_nbr", C.c_void_p), ("d_res", C.c_void_p), ("d_term", C.c_void_p),
+                ("d_partition", C.c_void_p)]
+
+
+class WtvResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("batch_size", C.c_int32), ("slot", C.c_int32), ("pad", C.c_int32),
+                ("raw_flow", C.c_int64), ("footprint_bytes", C.c_uint64), ("submit_s", C.c_double),
+                ("start_s", C.c_double), ("end_s", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
 class WtvError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"[{code}] {msg}")
@@ -77,6 +97,16 @@ def lib():
         L.wtv_batch_solve.restype = C.c_int
         L.wtv_batch_solve.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_int, C.c_void_p, C.POINTER(WtvParams), C.POINTER(WtvStats), C.c_void_p]
+        L.wtv_request_footprint.restype = C.c_uint64
+        L.wtv_request_footprint.argtypes = [C.c_uint32]
+        L.wtv_server_create.restype = C.c_void_p
+        L.wtv_server_create.argtypes = [C.POINTER(WtvServerConfig)]
+        L.wtv_server_submit.restype = C.c_int
+        L.wtv_server_submit.argtypes = [C.c_void_p, C.POINTER(WtvRequest), C.POINTER(C.c_uint64)]
+        L.wtv_server_wait.restype = C.c_int
+        L.wtv_server_wait.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(WtvResult)]
+        L.wtv_server_destroy.restype = C.c_int
+        L.wtv_server_destroy.argtypes = [C.c_void_p]
         L.wtv_graph_cut_fast.restype = C.c_int
         L.wtv_graph_cut_fast.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_size_t,
                                          C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_uint64, C.c_int,
@@ -125,3 +155,38 @@ def batch_solve(offsets, d_nbr, d_res, d_term, d_or_h_partition, partition_on_de
                                int(partition_on_device), flows.ctypes.data, C.byref(prm), C.byref(st), stream)
     check(rc)
     return flows[:G], st
+
+
+class Server:
+    """Thin wrapper of the native serving loop (wtv_server_*)."""
+
+    def __init__(self, slots=2, max_batch=64, max_batch_nodes=0, hbm_budget_bytes=0, **solver_kw):
+        cfg = WtvServerConfig()
+        cfg.slots, cfg.max_batch = slots, max_batch
+        cfg.max_batch_nodes, cfg.hbm_budget_bytes = max_batch_nodes, hbm_budget_bytes
+        cfg.params = params(**solver_kw)
+        self._h = lib().wtv_server_create(C.byref(cfg))
+        if not self._h:
+            raise WtvError(WTV_EINVAL, lib().wtv_last_error().decode())
+
+    def submit(self, n, d_nbr, d_res, d_term, d_partition):
+        r = WtvRequest(n, d_nbr, d_res, d_term, d_partition)
+        t = C.c_uint64(0)
+        check(lib().wtv_server_submit(self._h, C.byref(r), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        out = WtvResult()
+        check(lib().wtv_server_wait(self._h, ticket, C.byref(out)))
+        return out
+
+    def close(self):
+        if self._h:
+            check(lib().wtv_server_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
