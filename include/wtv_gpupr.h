@@ -210,6 +210,32 @@ int wtv_server_wait(void* server, uint64_t ticket, wtv_result* out);
 /* Finish all queued work, stop the workers and free the server. */
 int wtv_server_destroy(void* server);
 
+/* Counters of the reduced-graph build, as in wtvo's classify/build loops
+ * (wtivo_gpupr_bindings.cpp:139-318). */
+typedef struct wtv_build_info {
+    uint64_t free_n;
+    uint64_t fixed_source;
+    uint64_t fixed_sink;
+    uint64_t full_refs;
+    uint64_t free_refs;
+    uint64_t terminal_edges;
+    int64_t base_flow;
+} wtv_build_info;
+
+/*
+ * The reduced-graph build of graph_cut_fast on its own, on the device
+ * (classify, reduced ids, capacity_exact_formula, terminal folding:
+ * wtivo_gpupr_bindings.cpp:139-318).  Outputs are device buffers sized for N
+ * nodes; the first free_n entries are written:
+ *   d_nbr_out int32[N*4], d_res_out uint32[N*4], d_term_out int32[N],
+ *   d_free_old uint32[N] (tet index of each reduced node).
+ */
+int wtv_build_reduced_graph(const double* V, size_t nv, const int32_t* d_tets,
+                            const int32_t* d_nbr, size_t N, const uint8_t* labels,
+                            double fill_T, int32_t* d_nbr_out, uint32_t* d_res_out,
+                            int32_t* d_term_out, uint32_t* d_free_old,
+                            wtv_build_info* info, void* stream);
+
 /* Stats of the last wtv_pr_solve / wtv_graph_cut_fast on this thread. */
 void wtv_last_stats(wtv_stats* out);
 

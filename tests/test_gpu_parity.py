@@ -358,3 +358,68 @@ def test_server_admission_by_footprint(orc, torch_cuda):
             srv.submit(10000, dev[0][0].data_ptr(), dev[0][1].data_ptr(), dev[0][2].data_ptr(), dev[0][3].data_ptr())
     for g, r in zip(graphs, res):
         assert r.raw_flow == orc.bk_solve(*g).raw_flow
+
+
+# ------------------------------------------------------------ device graph build
+@pytest.mark.parametrize("seed", range(8))
+def test_device_build_matches_host_formula(orc, torch_cuda, seed):
+    """wtv_build_reduced_graph == the oracle's restatement of wtivo_gpupr_bindings.cpp:139-318,
+    value by value (FP64 capacities with heavy jitter and several coordinate scales)."""
+    rng = np.random.default_rng(seed)
+    k = 5 + 3 * seed
+    scale = [0.01, 0.3, 1.0, 2.5][seed % 4]
+    m = S.kuhn_mesh(k, jitter=0.45, seed=seed, hi=(scale, scale * 0.7, scale * 1.3))
+    labels = rng.choice(np.array([0, 1, 2], np.uint8), size=m.N, p=[0.3, 0.6, 0.1])
+    fill_T = float(rng.uniform(0.0, 40.0))
+    want = orc.reduce(m.V, m.tets, m.nbrs, labels, fill_T)
+    T = torch_cuda.from_numpy(m.tets).cuda()
+    Nb = torch_cuda.from_numpy(m.nbrs).cuda()
+    gn = torch_cuda.empty((m.N, 4), dtype=torch_cuda.int32, device="cuda")
+    gr = torch_cuda.empty((m.N, 4), dtype=torch_cuda.int32, device="cuda")
+    gt = torch_cuda.empty(m.N, dtype=torch_cuda.int32, device="cuda")
+    fo = torch_cuda.empty(m.N, dtype=torch_cuda.int32, device="cuda")
+    info = W.WtvBuildInfo()
+    W.check(W.lib().wtv_build_reduced_graph(m.V.ctypes.data, m.V.shape[0], T.data_ptr(), Nb.data_ptr(), m.N,
+                                            labels.ctypes.data, fill_T, gn.data_ptr(), gr.data_ptr(), gt.data_ptr(),
+                                            fo.data_ptr(), C.byref(info), torch_cuda.cuda.current_stream().cuda_stream))
+    n = want.n
+    assert info.free_n == n
+    for key in ("fixed_source", "fixed_sink", "full_refs", "free_refs", "terminal_edges", "base_flow"):
+        assert getattr(info, key) == want.info[key], key
+    assert np.array_equal(fo.cpu().numpy()[:n].view(np.uint32), want.free_old)
+    assert np.array_equal(gn.cpu().numpy()[:n], want.nbr)
+    assert np.array_equal(gr.cpu().numpy()[:n].view(np.uint32), want.res)
+    assert np.array_equal(gt.cpu().numpy()[:n], want.term)
+
+
+def test_graph_cut_fast_device_build_large(orc, torch_cuda):
+    """A 1.57M-tet reference-model cut, full pipeline on the device, against the oracle."""
+    import wtivo_gpupr as G
+    k = 64
+    m = S.kuhn_mesh(k, seed=31)
+    shape = S.sphere_with_holes(31)
+    lab = S.band_labels(S.tet_centroids_sdf(m, shape), 1.0 / k)
+    r = orc.reduce(m.V, m.tets, m.nbrs, lab, 10.0)
+    sol = orc.bk_solve(r.nbr, r.res, r.term)
+    want = orc.scatter(lab, r.free_old, sol.partition)
+    out = G.graph_cut_fast(m.V, torch_cuda.from_numpy(m.tets).cuda(), torch_cuda.from_numpy(m.nbrs).cuda(), lab, 10.0)
+    assert out[1] == sol.raw_flow + r.info["base_flow"] and out[2] == sol.raw_flow
+    assert np.array_equal(out[0], want)
+
+
+def test_config4_integer_stress_full_size(orc, torch_cuda):
+    """20,250,000-tet (150^3) thin shell with 40 tunnels, lambda 100, log-normal(2)
+    capacities quantised to [1, 2^20]: labels and flow bit-exact against BK."""
+    k = 150
+    V, T, Nb = S.kuhn_mesh_t(k, seed=20260004, device="cuda")
+    shape = S.thin_shell_with_tunnels(20260004, 1.0 / k, tunnels=40)
+    lab = S.labels_from_sdf(S.tet_sdf_t(V, T, shape).cpu().numpy(), 0.0)
+    m = S.TetMesh(k, V.cpu().numpy(), T.cpu().numpy(), Nb.cpu().numpy())
+    del V, T, Nb
+    g = S.lognormal_int_graph(m, lab, 100.0, seed=20260005, rev_slots=S.KUHN_REV_SLOT)
+    assert m.N == 20_250_000 and g.res.max() <= 2 ** 20
+    part, st, r_out, t_out = gpu_solve(torch_cuda, g.nbr, g.res, g.term, device_partition=True)
+    b = orc.bk_solve(g.nbr, g.res, g.term)
+    assert st["raw_flow"] == b.raw_flow
+    assert np.array_equal(part, b.partition)
+    assert S.quantised_cut(g.nbr, g.res, g.term, part) == st["raw_flow"]

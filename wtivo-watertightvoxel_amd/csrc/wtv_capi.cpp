@@ -15,9 +15,6 @@ Server* server_new(const wtv_server_config& c);
 void server_delete(Server* s);
 uint64_t server_submit(Server* s, const wtv_request& r);
 wtv_result server_wait(Server* s, uint64_t ticket);
-void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t* dN, size_t N,
-                    const uint8_t* lp, double fill_T, int requested_threads, int period, uint64_t max_rounds,
-                    int local_steps, uint8_t* out_labels, wtv_cut_result& R, wtv_stats& st, void* stream_v);
 }
 
 namespace {
@@ -102,6 +99,25 @@ int wtv_graph_cut_fast(const double* V, size_t nv, const int32_t* d_tets, const 
                             max_rounds, local_steps, labels_out, r, st, stream);
         wtv::set_last_stats(st);
         if (out) *out = r;
+    });
+}
+
+int wtv_build_reduced_graph(const double* V, size_t nv, const int32_t* d_tets, const int32_t* d_nbr, size_t N,
+                            const uint8_t* labels, double fill_T, int32_t* d_nbr_out, uint32_t* d_res_out,
+                            int32_t* d_term_out, uint32_t* d_free_old, wtv_build_info* info, void* stream) {
+    return guarded([&]() {
+        wtv::BuildInfo bi{};
+        wtv::build_reduced_graph(V, nv, d_tets, d_nbr, N, labels, fill_T, d_nbr_out, d_res_out, d_term_out,
+                                 d_free_old, bi, stream);
+        if (info) {
+            info->free_n = bi.free_n;
+            info->fixed_source = bi.fixed_source;
+            info->fixed_sink = bi.fixed_sink;
+            info->full_refs = bi.full_refs;
+            info->free_refs = bi.free_refs;
+            info->terminal_edges = bi.terminal_edges;
+            info->base_flow = bi.base_flow;
+        }
     });
 }
 

@@ -1,0 +1,442 @@
+// wtv_graph_build.hip -- graph_cut_fast on the device: classify, reduced ids,
+// exact capacities, terminal folding, solve, label scatter.
+//
+// Restates graph_cut_gpupr_fast_v630 (native/gpupr/wtivo_gpupr_bindings.cpp:100-397)
+// with the topology never leaving HBM (the reference copies it to the host in
+// 262,144-row chunks twice and uploads the reduced graph chunk by chunk,
+// :139-148,227-232,305-313):
+//   k_classify  label 0 -> fixed source (-2); label != 0 with a hull face ->
+//               fixed sink (-3); otherwise free (:153-171); neighbour range
+//               check; per-tile free counts and reference counts
+//   k_tile_scan exclusive scan of the per-tile free counts
+//   k_assign    reduced id = rank among free tets in increasing tet order
+//               (:179-188), free_old[id] = tet
+//   k_build     capacity_exact_formula (:65-98) for every slot of every free
+//               tet: FP64 cross product, one rounding of 1e7*0.5*|c| to fp32,
+//               pen = area or area*(1+fill_T), truncation to int64, range
+//               [0, 2^30); free neighbour -> arc, fixed neighbour -> folded into
+//               term = src - snk, base += min(src, snk) (:253-294)
+//   k_scatter   out = labels; out[free_old[q]] = partition[q] (:357-364)
+// Bit-exactness of the FP64 part needs no FMA contraction (pragma below) and a
+// correctly rounded sqrt; both are checked against the host formula in the
+// GPU tests.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <limits>
+#include <sstream>
+#include <vector>
+
+#include "wtv_internal.h"
+
+namespace wtv {
+namespace {
+
+constexpr int kB = 256;           // threads per workgroup
+constexpr uint32_t kTile = 4096;  // tets per scan tile (16 per thread)
+
+inline void ckg(hipError_t e, const char* where) {
+    if (e == hipSuccess) return;
+    fail(WTV_EDEVICE, std::string(kPrefix) + "HIP at " + where + ": " + hipGetErrorString(e));
+}
+
+struct BuildCtl {
+    unsigned long long free_n, fixed_source, fixed_sink, full_refs, free_refs, terminal_edges;
+    long long base_flow;
+    int32_t bad_range;  // neighbour out of range (classify)
+    int32_t bad_build;  // 1 hull face on a free tet, 3 bad class, 4 term range, 5 bad vertex, 6 overflow
+};
+
+__device__ __forceinline__ unsigned long long wsum(unsigned long long x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+__device__ __forceinline__ long long wsum(long long x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+__device__ __forceinline__ void add_u64(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void add_i64(long long* p, long long v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void flag(int32_t* p, int v) {
+    __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int slot(const int4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+
+// Workgroup sum, added once per workgroup.
+template <class T>
+__device__ __forceinline__ void wg_add(T* dst, T x, T* lds) {
+    x = wsum(x);
+    if (__lane_id() == 0) lds[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T t = 0;
+        for (int i = 0; i < kB / 64; ++i) t += lds[i];
+        if (t) {
+            if constexpr (sizeof(T) == 8 && static_cast<T>(-1) < 0) add_i64(reinterpret_cast<long long*>(dst), t);
+            else add_u64(reinterpret_cast<unsigned long long*>(dst), static_cast<unsigned long long>(t));
+        }
+    }
+    __syncthreads();
+}
+
+// One tile of kTile consecutive tets per workgroup.
+__global__ void __launch_bounds__(kB) k_classify(uint32_t N, const int4* __restrict__ nbr4,
+                                                 const uint8_t* __restrict__ labels, int32_t* __restrict__ cls,
+                                                 uint32_t* __restrict__ tile_free, BuildCtl* ctl) {
+    __shared__ unsigned long long lds[kB / 64];
+    const uint32_t t0 = blockIdx.x * kTile;
+    unsigned long long nf = 0, ns = 0, nk = 0, refs = 0;
+    int bad = 0;
+    for (uint32_t j = threadIdx.x; j < kTile; j += kB) {
+        const uint32_t i = t0 + j;
+        if (i >= N) break;
+        const int4 nb = nbr4[i];
+        bool exterior = false;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int x = slot(nb, k);
+            if (x < 0) exterior = true;
+            else {
+                ++refs;
+                if (static_cast<uint32_t>(x) >= N) bad = 1;
+            }
+        }
+        int32_t c;
+        if (labels[i] == 0u) { c = -2; ++ns; }
+        else if (exterior) { c = -3; ++nk; }
+        else { c = -1; ++nf; }
+        cls[i] = c;
+    }
+    if (bad) flag(&ctl->bad_range, 1);
+    wg_add(&ctl->fixed_source, ns, lds);
+    wg_add(&ctl->fixed_sink, nk, lds);
+    wg_add(&ctl->full_refs, refs, lds);
+    // per-tile free count (also summed into free_n)
+    unsigned long long x = wsum(nf);
+    if (__lane_id() == 0) lds[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < kB / 64; ++i) t += lds[i];
+        tile_free[blockIdx.x] = static_cast<uint32_t>(t);
+        if (t) add_u64(&ctl->free_n, t);
+    }
+}
+
+// Exclusive scan of the tile counts (one workgroup; tiles <= 2^20 per call).
+__global__ void __launch_bounds__(kB) k_tile_scan(uint32_t tiles, uint32_t* __restrict__ tile_free) {
+    __shared__ uint32_t wsums[kB / 64];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0u;
+    __syncthreads();
+    for (uint32_t base = 0; base < tiles; base += kB) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < tiles ? tile_free[i] : 0u;
+        // inclusive scan within the wave
+        uint32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (static_cast<int>(__lane_id()) >= o) x += y;
+        }
+        if (__lane_id() == 63) wsums[threadIdx.x >> 6] = x;
+        __syncthreads();
+        uint32_t wpre = 0;
+        for (int w = 0; w < static_cast<int>(threadIdx.x >> 6); ++w) wpre += wsums[w];
+        const uint32_t c = carry;
+        if (i < tiles) tile_free[i] = c + wpre + x - v;
+        __syncthreads();
+        if (threadIdx.x == kB - 1) carry = c + wpre + x;
+        __syncthreads();
+    }
+}
+
+// Reduced ids in increasing tet order: rank of each free tet inside its tile,
+// processed as 16 rounds of 256 consecutive tets.
+__global__ void __launch_bounds__(kB) k_assign(uint32_t N, int32_t* __restrict__ cls,
+                                               const uint32_t* __restrict__ tile_base,
+                                               uint32_t* __restrict__ free_old) {
+    __shared__ uint32_t wcnt[kB / 64];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = tile_base[blockIdx.x];
+    __syncthreads();
+    const uint32_t t0 = blockIdx.x * kTile;
+    for (uint32_t j0 = 0; j0 < kTile; j0 += kB) {
+        const uint32_t i = t0 + j0 + threadIdx.x;
+        const bool fr = i < N && cls[i] == -1;
+        const uint64_t m = __ballot(fr);
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+        if (__lane_id() == 0) wcnt[threadIdx.x >> 6] = static_cast<uint32_t>(__popcll(m));
+        __syncthreads();
+        uint32_t wpre = 0;
+        for (int w = 0; w < static_cast<int>(threadIdx.x >> 6); ++w) wpre += wcnt[w];
+        const uint32_t c = carry;
+        if (fr) {
+            const uint32_t id = c + wpre + rank;
+            cls[i] = static_cast<int32_t>(id);
+            free_old[id] = i;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        __syncthreads();
+    }
+}
+
+// capacity_exact_formula (wtivo_gpupr_bindings.cpp:65-98); 0 ok, 1 bad vertex, 2 overflow.
+__device__ __forceinline__ int cap_exact(const double* __restrict__ V, size_t nv, int a, int b, int c, bool differ,
+                                         double fill_T, int32_t* out) {
+#pragma clang fp contract(off)
+    if (a < 0 || b < 0 || c < 0 || static_cast<size_t>(a) >= nv || static_cast<size_t>(b) >= nv ||
+        static_cast<size_t>(c) >= nv)
+        return 1;
+    const double* p0 = V + static_cast<size_t>(a) * 3u;
+    const double* p1 = V + static_cast<size_t>(b) * 3u;
+    const double* p2 = V + static_cast<size_t>(c) * 3u;
+    const double x1 = p1[0] - p0[0], y1 = p1[1] - p0[1], z1 = p1[2] - p0[2];
+    const double x2 = p2[0] - p0[0], y2 = p2[1] - p0[1], z2 = p2[2] - p0[2];
+    const double cx = y1 * z2 - z1 * y2;
+    const double cy = z1 * x2 - x1 * z2;
+    const double cz = x1 * y2 - y1 * x2;
+    const float area_f = static_cast<float>(1.0E7 * 0.5 * __builtin_sqrt(cx * cx + cy * cy + cz * cz));
+    const double area = static_cast<double>(area_f);
+    const double pen = differ ? area : area * (1.0 + fill_T);
+    if (!(pen > -9.2e18 && pen < 9.2e18)) return 2;
+    const long long cap64 = static_cast<long long>(pen);
+    if (cap64 < 0 || cap64 >= 1073741824LL) return 2;
+    *out = static_cast<int32_t>(cap64);
+    return 0;
+}
+
+__global__ void __launch_bounds__(kB) k_build(uint32_t N, const double* __restrict__ V, size_t nv,
+                                              const int4* __restrict__ tets4, const int4* __restrict__ nbr4,
+                                              const uint8_t* __restrict__ labels, const int32_t* __restrict__ cls,
+                                              double fill_T, int4* __restrict__ nbr_r, uint4* __restrict__ res_r,
+                                              int32_t* __restrict__ term_r, BuildCtl* ctl) {
+    __shared__ unsigned long long lds_u[kB / 64];
+    __shared__ long long lds_i[kB / 64];
+    constexpr int lut[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+    unsigned long long refs = 0, terms = 0;
+    long long base = 0;
+    int bad = 0;
+    const uint32_t stride = gridDim.x * kB;
+    for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < N; i += stride) {
+        const int32_t rr = cls[i];
+        if (rr < 0) continue;
+        const int4 tr = tets4[i];
+        const int4 nb = nbr4[i];
+        const int tv[4] = {tr.x, tr.y, tr.z, tr.w};
+        const uint8_t li = labels[i];
+        int nr[4] = {-1, -1, -1, -1};
+        uint32_t rs[4] = {0u, 0u, 0u, 0u};
+        long long src = 0, snk = 0;
+#pragma unroll
+        for (int fi = 0; fi < 4; ++fi) {
+            const int j = slot(nb, fi);
+            if (j < 0) { bad = 1; continue; }
+            if (static_cast<uint32_t>(j) >= N) { bad = 3; continue; }
+            const int32_t m = cls[j];
+            int32_t cap = 0;
+            const int rc = cap_exact(V, nv, tv[lut[fi][0]], tv[lut[fi][1]], tv[lut[fi][2]], li != labels[j], fill_T,
+                                     &cap);
+            if (rc == 1) { bad = 5; continue; }
+            if (rc == 2) { bad = 6; continue; }
+            if (m >= 0) {
+                nr[fi] = m;
+                rs[fi] = static_cast<uint32_t>(cap);
+                ++refs;
+            } else if (m == -2) { src += cap; ++terms; }
+            else if (m == -3) { snk += cap; ++terms; }
+            else bad = 3;
+        }
+        base += src < snk ? src : snk;
+        const long long diff = src - snk;
+        if (diff < -2147483648LL || diff > 2147483647LL) bad = 4;
+        nbr_r[rr] = make_int4(nr[0], nr[1], nr[2], nr[3]);
+        res_r[rr] = make_uint4(rs[0], rs[1], rs[2], rs[3]);
+        term_r[rr] = static_cast<int32_t>(diff);
+    }
+    if (bad) flag(&ctl->bad_build, bad);
+    wg_add(&ctl->free_refs, refs, lds_u);
+    wg_add(&ctl->terminal_edges, terms, lds_u);
+    wg_add(&ctl->base_flow, base, lds_i);
+}
+
+__global__ void __launch_bounds__(kB) k_scatter(uint32_t n, const uint32_t* __restrict__ free_old,
+                                                const uint8_t* __restrict__ part, uint8_t* __restrict__ out) {
+    const uint32_t stride = gridDim.x * kB;
+    for (uint32_t q = blockIdx.x * kB + threadIdx.x; q < n; q += stride) out[free_old[q]] = part[q] ? 1u : 0u;
+}
+
+struct Dev {
+    void* p = nullptr;
+    ~Dev() { if (p) (void)hipFree(p); }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace
+
+// Builds the reduced graph on the device.  Outputs are caller buffers sized
+// for N nodes (only the first free_n entries are written); their contents
+// match the reference's host build exactly.  Returns false when there are no
+// free nodes (the caller then returns the input labels).
+bool build_reduced(const double* V, size_t nv, const int32_t* dT, const int32_t* dN, size_t N, const uint8_t* d_lab,
+                   double fill_T, int32_t* gN, uint32_t* gR, int32_t* gT, uint32_t* free_old, BuildInfo& info,
+                   hipStream_t s) {
+    std::memset(&info, 0, sizeof(info));
+    const uint32_t n32 = static_cast<uint32_t>(N);
+    const uint32_t tiles = static_cast<uint32_t>((N + kTile - 1) / kTile);
+    if (tiles > (1u << 20)) fail(WTV_EINVAL, ">2^32 full tets unsupported");
+    Dev dCls, dTiles, dCtl, dV;
+    ckg(hipMalloc(&dCls.p, N * 4u), "malloc classes");
+    ckg(hipMalloc(&dTiles.p, static_cast<size_t>(tiles) * 4u), "malloc tiles");
+    ckg(hipMalloc(&dCtl.p, sizeof(BuildCtl)), "malloc build ctl");
+    ckg(hipMemsetAsync(dCtl.p, 0, sizeof(BuildCtl), s), "build ctl clear");
+    const int4* nbr4 = reinterpret_cast<const int4*>(dN);
+    const int4* tets4 = reinterpret_cast<const int4*>(dT);
+    BuildCtl* ctl = dCtl.as<BuildCtl>();
+    BuildCtl h{};
+    // classify + reduced ids (wtivo_gpupr_bindings.cpp:139-188)
+    k_classify<<<tiles, kB, 0, s>>>(n32, nbr4, d_lab, dCls.as<int32_t>(), dTiles.as<uint32_t>(), ctl);
+    ckg(hipGetLastError(), "classify");
+    ckg(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s), "classify D2H");
+    ckg(hipStreamSynchronize(s), "classify sync");
+    if (h.bad_range) fail(WTV_EINVAL, "neighbor out of range");
+    if (h.full_refs & 1ull) fail(WTV_EINVAL, "neighbor reference count not even");
+    info.free_n = h.free_n;
+    info.fixed_source = h.fixed_source;
+    info.fixed_sink = h.fixed_sink;
+    info.full_refs = h.full_refs;
+    if (h.free_n == 0) return false;
+    if (h.free_n > static_cast<uint64_t>(std::numeric_limits<int32_t>::max()))
+        fail(WTV_EINVAL, ">INT32_MAX reduced nodes unsupported");
+    ckg(hipMalloc(&dV.p, std::max<size_t>(nv, 1) * 24u), "malloc vertices");
+    if (nv) ckg(hipMemcpyAsync(dV.p, V, nv * 24u, hipMemcpyHostToDevice, s), "vertices H2D");
+    k_tile_scan<<<1, kB, 0, s>>>(tiles, dTiles.as<uint32_t>());
+    k_assign<<<tiles, kB, 0, s>>>(n32, dCls.as<int32_t>(), dTiles.as<uint32_t>(), free_old);
+    const int grid = static_cast<int>(std::min<uint64_t>((N + kB - 1) / kB, 8192));
+    k_build<<<grid, kB, 0, s>>>(n32, dV.as<double>(), nv, tets4, nbr4, d_lab, dCls.as<int32_t>(), fill_T,
+                                reinterpret_cast<int4*>(gN), reinterpret_cast<uint4*>(gR), gT, ctl);
+    ckg(hipGetLastError(), "build");
+    ckg(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s), "build D2H");
+    ckg(hipStreamSynchronize(s), "build sync");
+    // the reference's build lambda turns every failure, the capacity
+    // overflow_error included, into one runtime_error (:292,296-297)
+    if (h.bad_build == 6) fail(WTV_EINVAL, "invalid graph reduction/capacity state (capacity outside safe 30-bit range)");
+    if (h.bad_build == 5) fail(WTV_EINVAL, "invalid graph reduction/capacity state (invalid tet vertex)");
+    if (h.bad_build) fail(WTV_EINVAL, "invalid graph reduction/capacity state");
+    if (h.free_refs & 1ull) fail(WTV_EINVAL, "free reference count not even");
+    info.free_refs = h.free_refs;
+    info.terminal_edges = h.terminal_edges;
+    info.base_flow = h.base_flow;
+    return true;
+}
+
+void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t* dN, size_t N, const uint8_t* lp,
+                    double fill_T, int requested_threads, int period, uint64_t max_rounds, int local_steps,
+                    uint8_t* out_labels, wtv_cut_result& R, wtv_stats& st, void* stream_v) {
+    using Clock = std::chrono::steady_clock;
+    const auto all0 = Clock::now();
+    std::memset(&R, 0, sizeof(R));
+    std::memset(&st, 0, sizeof(st));
+    if ((!V && nv) || (!dT && N) || (!dN && N) || (!lp && N) || (!out_labels && N))
+        fail(WTV_EINVAL, "null pointer argument");
+    if (N > static_cast<size_t>(std::numeric_limits<uint32_t>::max()))
+        fail(WTV_EINVAL, ">2^32 full tets unsupported");
+    R.threads = std::max(1, requested_threads);
+    R.local_steps = local_steps;
+    if (N == 0) {
+        R.native_total_seconds = std::chrono::duration<double>(Clock::now() - all0).count();
+        return;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream_v);
+    Dev dLab;
+    ckg(hipMalloc(&dLab.p, N), "malloc labels");
+    ckg(hipMemcpyAsync(dLab.p, lp, N, hipMemcpyHostToDevice, s), "labels H2D");
+
+    // VRAM precheck with the reference's estimate (:200-212) plus this
+    // solver's workspace and the build's own buffers, for the worst case n = N
+    size_t gpu_free = 0, gpu_total = 0;
+    ckg(hipMemGetInfo(&gpu_free, &gpu_total), "hipMemGetInfo");
+    const uint64_t need = static_cast<uint64_t>(N) * (36ull + 29ull + 4ull + 4ull + 1ull) +
+                          static_cast<uint64_t>(nv) * 24u + (256ull << 20);
+    if (need > static_cast<uint64_t>(gpu_free)) fail(WTV_ENOMEM, "insufficient VRAM for reduced graph + solver");
+
+    const auto build0 = Clock::now();
+    Dev gN, gR, gT, dFreeOld;
+    ckg(hipMalloc(&gN.p, N * 16u), "malloc graph nbr");
+    ckg(hipMalloc(&gR.p, N * 16u), "malloc graph res");
+    ckg(hipMalloc(&gT.p, N * 4u), "malloc graph term");
+    ckg(hipMalloc(&dFreeOld.p, N * 4u), "malloc free_old");
+    BuildInfo bi{};
+    const bool any = build_reduced(V, nv, dT, dN, N, dLab.as<uint8_t>(), fill_T, gN.as<int32_t>(), gR.as<uint32_t>(),
+                                   gT.as<int32_t>(), dFreeOld.as<uint32_t>(), bi, s);
+    const auto build1 = Clock::now();
+    R.full_edges = bi.full_refs / 2u;
+    R.fixed_source = bi.fixed_source;
+    R.fixed_sink = bi.fixed_sink;
+    if (!any) {
+        std::memcpy(out_labels, lp, N);
+        R.native_total_seconds = std::chrono::duration<double>(Clock::now() - all0).count();
+        return;
+    }
+    const uint64_t free_N = bi.free_n;
+
+    wtv_params prm;
+    wtv_params_default(&prm);
+    prm.global_relabel_period = period;
+    prm.max_rounds = max_rounds;
+    prm.local_steps = local_steps;
+    Dev dPart, dOut;
+    ckg(hipMalloc(&dPart.p, free_N), "malloc partition");
+    pr_solve(static_cast<uint32_t>(free_N), gN.as<int32_t>(), gR.as<uint32_t>(), gT.as<int32_t>(), dPart.as<uint8_t>(),
+             true, prm, st, stream_v);
+
+    // labels (:357-364): out = labels; out[free_old[q]] = partition[q]
+    ckg(hipMalloc(&dOut.p, N), "malloc out labels");
+    ckg(hipMemcpyAsync(dOut.p, dLab.p, N, hipMemcpyDeviceToDevice, s), "labels copy");
+    k_scatter<<<static_cast<int>(std::min<uint64_t>((free_N + kB - 1) / kB, 8192)), kB, 0, s>>>(
+        static_cast<uint32_t>(free_N), dFreeOld.as<uint32_t>(), dPart.as<uint8_t>(), dOut.as<uint8_t>());
+    ckg(hipGetLastError(), "scatter");
+    ckg(hipMemcpyAsync(out_labels, dOut.p, N, hipMemcpyDeviceToHost, s), "labels D2H");
+    ckg(hipStreamSynchronize(s), "labels sync");
+
+    R.adjusted_flow = st.raw_flow + bi.base_flow;
+    R.raw_flow = st.raw_flow;
+    R.graph_bytes = free_N * 36ull;
+    R.workspace_bytes = st.solver_workspace_bytes;
+    R.free_edges = bi.free_refs / 2u;
+    R.terminal_edges = bi.terminal_edges;
+    R.macro_rounds = st.rounds;
+    R.global_relabels = st.global_relabels;
+    R.map_bytes = static_cast<uint64_t>(N) * sizeof(int32_t) + free_N * sizeof(uint32_t);
+    R.build_upload_seconds = std::chrono::duration<double>(build1 - build0).count();
+    R.solve_seconds = st.solve_seconds;
+    R.native_total_seconds = std::chrono::duration<double>(Clock::now() - all0).count();
+}
+
+void build_reduced_graph(const double* V, size_t nv, const int32_t* dT, const int32_t* dN, size_t N,
+                         const uint8_t* labels, double fill_T, int32_t* gN, uint32_t* gR, int32_t* gT,
+                         uint32_t* free_old, BuildInfo& info, void* stream_v) {
+    std::memset(&info, 0, sizeof(info));
+    if (!N) return;
+    if (!V || !dT || !dN || !labels || !gN || !gR || !gT || !free_old) fail(WTV_EINVAL, "null pointer argument");
+    if (N > static_cast<size_t>(std::numeric_limits<uint32_t>::max()))
+        fail(WTV_EINVAL, ">2^32 full tets unsupported");
+    hipStream_t s = static_cast<hipStream_t>(stream_v);
+    Dev dLab;
+    ckg(hipMalloc(&dLab.p, N), "malloc labels");
+    ckg(hipMemcpyAsync(dLab.p, labels, N, hipMemcpyHostToDevice, s), "labels H2D");
+    build_reduced(V, nv, dT, dN, N, dLab.as<uint8_t>(), fill_T, gN, gR, gT, free_old, info, s);
+    ckg(hipStreamSynchronize(s), "build");
+}
+
+}  // namespace wtv

@@ -32,6 +32,22 @@ void batch_solve(uint32_t G, const uint64_t* h_off, const int32_t* d_nbr, uint32
                  uint8_t* partition, bool partition_on_device, int64_t* h_flows, const wtv_params& p,
                  wtv_stats& st, void* stream);
 
+// Counters of the reduced-graph build (wtivo_gpupr_bindings.cpp:139-318).
+struct BuildInfo {
+    uint64_t free_n, fixed_source, fixed_sink, full_refs, free_refs, terminal_edges;
+    int64_t base_flow;
+};
+
+// Device build of the reduced graph (wtv_graph_build.hip).
+void build_reduced_graph(const double* V, size_t nv, const int32_t* dT, const int32_t* dN, size_t N,
+                         const uint8_t* labels, double fill_T, int32_t* gN, uint32_t* gR, int32_t* gT,
+                         uint32_t* free_old, BuildInfo& info, void* stream);
+
+// Full cut on the device (wtv_graph_build.hip).
+void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t* dN, size_t N,
+                    const uint8_t* lp, double fill_T, int requested_threads, int period, uint64_t max_rounds,
+                    int local_steps, uint8_t* out_labels, wtv_cut_result& R, wtv_stats& st, void* stream_v);
+
 // Thread-local stats of the last solve (wtv_capi.cpp).
 void set_last_stats(const wtv_stats& st);
 

@@ -15,7 +15,7 @@ WTV_OK, WTV_EINVAL, WTV_EOVERFLOW, WTV_EINVARIANT, WTV_ENOMEM, WTV_ELIMIT, WTV_E
 
 # every extern "C" symbol declared in include/wtv_gpupr.h
 EXPORTS = ["wtv_params_default", "wtv_pr_solve", "wtv_batch_solve", "wtv_graph_cut_fast", "wtv_last_stats",
-           "wtv_last_error", "wtv_version", "wtv_request_footprint", "wtv_server_create", "wtv_server_submit",
+           "wtv_last_error", "wtv_version", "wtv_build_reduced_graph", "wtv_request_footprint", "wtv_server_create", "wtv_server_submit",
            "wtv_server_wait", "wtv_server_destroy"]
 
 
@@ -74,6 +74,15 @@ class WtvResult(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
+class WtvBuildInfo(C.Structure):
+    _fields_ = [("free_n", C.c_uint64), ("fixed_source", C.c_uint64), ("fixed_sink", C.c_uint64),
+                ("full_refs", C.c_uint64), ("free_refs", C.c_uint64), ("terminal_edges", C.c_uint64),
+                ("base_flow", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class WtvError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"[{code}] {msg}")
@@ -97,6 +106,10 @@ def lib():
         L.wtv_batch_solve.restype = C.c_int
         L.wtv_batch_solve.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_int, C.c_void_p, C.POINTER(WtvParams), C.POINTER(WtvStats), C.c_void_p]
+        L.wtv_build_reduced_graph.restype = C.c_int
+        L.wtv_build_reduced_graph.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_size_t,
+                                              C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.POINTER(WtvBuildInfo), C.c_void_p]
         L.wtv_request_footprint.restype = C.c_uint64
         L.wtv_request_footprint.argtypes = [C.c_uint32]
         L.wtv_server_create.restype = C.c_void_p

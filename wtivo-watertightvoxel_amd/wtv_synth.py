@@ -477,7 +477,7 @@ def quantised_cut(nbr, res0, term0, partition) -> int:
 
 
 def lognormal_int_graph(mesh: TetMesh, labels: np.ndarray, lam: float, seed: int, sigma=2.0,
-                        cap_hi=(1 << 20)) -> SoftGraph:
+                        cap_hi=(1 << 20), rev_slots=None) -> SoftGraph:
     """Integer stress model: facet capacities log-normal(sigma) quantised to [1, cap_hi]
     (one value per facet), terminals lam * round(volume-proportional) integers."""
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -485,7 +485,7 @@ def lognormal_int_graph(mesh: TetMesh, labels: np.ndarray, lam: float, seed: int
     nb = mesh.nbrs
     raw = rng.lognormal(mean=0.0, sigma=sigma, size=(N, 4))
     q = np.clip(np.rint(raw * 64.0), 1, cap_hi).astype(np.int64)
-    rs = reverse_slots(nb)
+    rs = reverse_slots(nb) if rev_slots is None else np.broadcast_to(np.asarray(rev_slots, np.int64), (N, 4))
     idx = np.arange(N, dtype=np.int64)[:, None]
     j = nb.astype(np.int64)
     take = (j >= 0) & (j < idx)
