@@ -16,6 +16,8 @@
  *                          (native/gpupr/wtivo_gpupr_bindings.cpp:100-397,581-596)
  *   wtv_batch_solve     <- no single reference function: G independent calls
  *                          of cellocut_gpu_push_relabel_fast solved together
+ *   wtv_surface_extract <- surface_extraction_topology_cuda
+ *                          (native/gpupr/wtivo_gpupr_bindings.cpp:400-577,597-599)
  *   wtv_server_*        <- no reference counterpart: a serving loop around
  *                          the same solver (continuous batching, HBM admission)
  */
@@ -235,6 +237,23 @@ int wtv_build_reduced_graph(const double* V, size_t nv, const int32_t* d_tets,
                             double fill_T, int32_t* d_nbr_out, uint32_t* d_res_out,
                             int32_t* d_term_out, uint32_t* d_free_old,
                             wtv_build_info* info, void* stream);
+
+/* Boundary surface of the label-0 region (surface_extraction_topology_cuda):
+ * oriented triangles of label-0 tets on the hull or next to a label != 0 tet,
+ * tet-major / slot-minor, with the used vertices compacted in ascending
+ * original id.  The library allocates vertices (float64[n_vertices*3]) and
+ * faces (int32[n_faces*3]) with malloc; release them with wtv_surface_free. */
+typedef struct wtv_surface {
+    size_t n_vertices;
+    size_t n_faces;
+    double* vertices;
+    int32_t* faces;
+} wtv_surface;
+
+int wtv_surface_extract(const uint8_t* labels, const double* V, size_t nv,
+                        const int32_t* d_tets, const int32_t* d_nbr, size_t N,
+                        wtv_surface* out, void* stream);
+void wtv_surface_free(wtv_surface* s);
 
 /* Stats of the last wtv_pr_solve / wtv_graph_cut_fast on this thread. */
 void wtv_last_stats(wtv_stats* out);

@@ -226,3 +226,30 @@ def certificate(nbr, res0, term0, res, term, partition, raw_flow) -> int:
     return int(lib().wtvo_certificate(n, _p(nbr, C.c_int32), _p(res0, C.c_uint32), _p(term0, C.c_int32),
                                       _p(res, C.c_uint32), _p(term, C.c_int32), _p(part, C.c_uint8),
                                       int(raw_flow)))
+
+
+def surface(labels, V, tets, nbrs):
+    """surface_extraction_topology_cuda (wtivo_gpupr_bindings.cpp:400-577): (OV float64 [u,3], OF int32 [F,3])."""
+    L = lib()
+    if not hasattr(L, "_surf_ready"):
+        L.wtvo_surface.restype = C.c_int
+        L.wtvo_surface.argtypes = [C.c_size_t, C.POINTER(C.c_uint8), C.POINTER(C.c_double), C.c_size_t,
+                                   C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_size_t),
+                                   C.POINTER(C.c_size_t), C.c_void_p, C.c_void_p]
+        L._surf_ready = True
+    labels = np.ascontiguousarray(labels, dtype=np.uint8).reshape(-1)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    tets = np.ascontiguousarray(tets, dtype=np.int32)
+    nbrs = np.ascontiguousarray(nbrs, dtype=np.int32)
+    nf, nu = C.c_size_t(0), C.c_size_t(0)
+    args = (labels.shape[0], _p(labels, C.c_uint8), _p(V, C.c_double), V.shape[0], _p(tets, C.c_int32),
+            _p(nbrs, C.c_int32), C.byref(nf), C.byref(nu))
+    rc = L.wtvo_surface(*args, None, None)
+    if rc != OK:
+        raise OracleError(rc, "surface")
+    OV = np.empty((nu.value, 3), np.float64)
+    OF = np.empty((nf.value, 3), np.int32)
+    rc = L.wtvo_surface(*args, OV.ctypes.data, OF.ctypes.data)
+    if rc != OK:
+        raise OracleError(rc, "surface")
+    return OV, OF

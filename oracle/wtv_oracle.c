@@ -819,3 +819,81 @@ int wtvo_certificate(uint32_t n, const int32_t* nbr, const uint32_t* res0,
     free(p2);
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Surface extraction: wtivo_gpupr_bindings.cpp:400-577.                      */
+/* Boundary triangles of label-0 tets (hull faces and faces towards a tet     */
+/* with label != 0), tet-major / slot-minor order; vertices sorted, then      */
+/* oriented by the sign of the 4th vertex's volume; vertices compacted in     */
+/* ascending original id.  Two calls: first with faces == NULL to get the     */
+/* counts, then with buffers of those sizes.                                  */
+/* Returns 0 ok, 1 neighbour out of range, 2 invalid tet vertex.              */
+/* ------------------------------------------------------------------------ */
+int wtvo_surface(size_t N, const uint8_t* labels, const double* V, size_t nv,
+                 const int32_t* tets, const int32_t* nbrs, size_t* n_faces,
+                 size_t* n_used, double* out_v, int32_t* out_f)
+{
+    static const int opp[4] = {3, 2, 1, 0};
+    size_t F = 0;
+    for (size_t i = 0; i < N; ++i) {
+        if (labels[i] != 0u) continue;
+        for (int fi = 0; fi < 4; ++fi) {
+            const int32_t j = nbrs[i * 4u + (size_t)fi];
+            if (j >= 0 && (size_t)j >= N) return 1;
+            if (j < 0 || labels[j] != 0u) ++F;
+        }
+    }
+    *n_faces = F;
+    int32_t* raw = (int32_t*)malloc((F * 3u + 1u) * sizeof(int32_t));
+    int32_t* remap = (int32_t*)malloc((nv + 1u) * sizeof(int32_t));
+    if (!raw || !remap) { free(raw); free(remap); return WTVO_ENOMEM; }
+    size_t p = 0;
+    for (size_t i = 0; i < N; ++i) {
+        if (labels[i] != 0u) continue;
+        const int32_t* tr = tets + i * 4u;
+        for (int fi = 0; fi < 4; ++fi) {
+            const int32_t j = nbrs[i * 4u + (size_t)fi];
+            if (!(j < 0 || labels[j] != 0u)) continue;
+            int a0 = tr[k_face_lut[fi][0]], b0 = tr[k_face_lut[fi][1]], c0 = tr[k_face_lut[fi][2]];
+            const int fourth = tr[opp[fi]];
+            if (a0 < 0 || b0 < 0 || c0 < 0 || fourth < 0 || (size_t)a0 >= nv || (size_t)b0 >= nv ||
+                (size_t)c0 >= nv || (size_t)fourth >= nv) {
+                free(raw); free(remap);
+                return 2;
+            }
+            int t;
+            if (a0 > b0) { t = a0; a0 = b0; b0 = t; }
+            if (b0 > c0) { t = b0; b0 = c0; c0 = t; }
+            if (a0 > b0) { t = a0; a0 = b0; b0 = t; }
+            const double* v0 = V + (size_t)a0 * 3u;
+            const double* v1 = V + (size_t)b0 * 3u;
+            const double* v2 = V + (size_t)c0 * 3u;
+            const double* vp = V + (size_t)fourth * 3u;
+            const double x1 = v1[0] - v0[0], y1 = v1[1] - v0[1], z1 = v1[2] - v0[2];
+            const double x2 = v2[0] - v0[0], y2 = v2[1] - v0[1], z2 = v2[2] - v0[2];
+            const double cx = y1 * z2 - z1 * y2;
+            const double cy = z1 * x2 - x1 * z2;
+            const double cz = x1 * y2 - y1 * x2;
+            const double px = vp[0] - v0[0], py = vp[1] - v0[1], pz = vp[2] - v0[2];
+            const double vol = cx * px + cy * py + cz * pz;
+            raw[3u * p] = a0;
+            raw[3u * p + 1u] = (vol < 0.0) ? b0 : c0;
+            raw[3u * p + 2u] = (vol < 0.0) ? c0 : b0;
+            ++p;
+        }
+    }
+    for (size_t v = 0; v < nv; ++v) remap[v] = -1;
+    for (size_t q = 0; q < 3u * F; ++q) remap[raw[q]] = -2;
+    size_t used = 0;
+    for (size_t v = 0; v < nv; ++v)
+        if (remap[v] == -2) remap[v] = (int32_t)used++;
+    *n_used = used;
+    if (out_v && out_f) {
+        for (size_t v = 0; v < nv; ++v)
+            if (remap[v] >= 0) memcpy(out_v + (size_t)remap[v] * 3u, V + v * 3u, 3u * sizeof(double));
+        for (size_t q = 0; q < 3u * F; ++q) out_f[q] = remap[raw[q]];
+    }
+    free(raw);
+    free(remap);
+    return WTVO_OK;
+}

@@ -423,3 +423,40 @@ def test_config4_integer_stress_full_size(orc, torch_cuda):
     assert st["raw_flow"] == b.raw_flow
     assert np.array_equal(part, b.partition)
     assert S.quantised_cut(g.nbr, g.res, g.term, part) == st["raw_flow"]
+
+
+# ------------------------------------------------------------ surface extraction
+@pytest.mark.parametrize("seed", range(4))
+def test_surface_extraction_matches_oracle(orc, torch_cuda, seed):
+    import wtivo_gpupr as G
+    k = [8, 20, 33, 48][seed]
+    m = S.kuhn_mesh(k, seed=40 + seed)
+    shape = S.sphere_with_holes(40 + seed)
+    lab = S.labels_from_sdf(S.tet_centroids_sdf(m, shape), 0.03, seed=50 + seed)
+    if seed == 3:
+        lab[lab == 1] = 2  # any label != 0 counts as outside
+    want_v, want_f = orc.surface(lab, m.V, m.tets, m.nbrs)
+    OV, OF = G.surface_extraction_topology_cuda(lab, m.V, torch_cuda.from_numpy(m.tets).cuda(),
+                                                torch_cuda.from_numpy(m.nbrs).cuda(), 8)
+    assert OF.dtype == np.int32 and OV.dtype == np.float64
+    assert np.array_equal(OF, want_f)
+    assert np.array_equal(OV, want_v)
+
+
+def test_surface_after_graph_cut_and_errors(orc, torch_cuda):
+    """The pipeline order of wtivo.py:842-859: cut, then surface of the new labels."""
+    import wtivo_gpupr as G
+    m, lab, _ = _c0_soft(24, 60)
+    T = torch_cuda.from_numpy(m.tets).cuda()
+    Nb = torch_cuda.from_numpy(m.nbrs).cuda()
+    new_labels = G.graph_cut_fast(m.V, T, Nb, lab, 10.0)[0]
+    OV, OF = G.surface_extraction_topology_cuda(new_labels, m.V, T, Nb)
+    want_v, want_f = orc.surface(new_labels, m.V, m.tets, m.nbrs)
+    assert np.array_equal(OF, want_f) and np.array_equal(OV, want_v)
+    bad = m.nbrs.copy()
+    i = int(np.flatnonzero(new_labels == 0)[0])
+    bad[i, 0] = m.N + 7
+    with pytest.raises(RuntimeError, match="surface neighbor out of range"):
+        G.surface_extraction_topology_cuda(new_labels, m.V, T, torch_cuda.from_numpy(bad).cuda())
+    with pytest.raises(RuntimeError, match="contiguous CUDA int32 Nx4 with same N"):
+        G.surface_extraction_topology_cuda(new_labels, m.V, T.long(), Nb)

@@ -270,3 +270,44 @@ def test_golden_fixture_reproduces(orc, name):
         sol = orc.bk_solve(z["nbr"], z["res"], z["term"])
         assert sol.raw_flow == int(z["raw_flow"])
         assert np.array_equal(sol.partition, z["partition"])
+
+
+# ------------------------------------------------------------ surface extraction
+def test_surface_single_tet_outward(orc):
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    OV, OF = orc.surface(np.zeros(1, np.uint8), V, np.array([[0, 1, 2, 3]], np.int32), -np.ones((1, 4), np.int32))
+    assert np.array_equal(OV, V)
+    # every face normal points away from the opposite vertex
+    cen = V.mean(0)
+    for f in OF:
+        a, b, c = V[f]
+        nrm = np.cross(b - a, c - a)
+        assert np.dot(nrm, a - cen) > 0
+    assert sorted(tuple(sorted(f)) for f in OF.tolist()) == [(0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)]
+    # a label != 0 tet contributes nothing
+    OV, OF = orc.surface(np.ones(1, np.uint8), V, np.array([[0, 1, 2, 3]], np.int32), -np.ones((1, 4), np.int32))
+    assert OF.shape == (0, 3) and OV.shape == (0, 3)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_surface_is_closed_and_encloses_the_region(orc, seed):
+    """Interior blob: each directed edge appears once and its reverse once (closed,
+    consistently oriented) and the enclosed volume equals the region's tet volume."""
+    m = S.kuhn_mesh(10, jitter=0.1, seed=seed)
+    shape = S.Sphere((0.5, 0.5, 0.5), 0.3 + 0.05 * seed)
+    lab = S.labels_from_sdf(S.tet_centroids_sdf(m, shape), 0.0)
+    OV, OF = orc.surface(lab, m.V, m.tets, m.nbrs)
+    edges = {}
+    for f in OF.tolist():
+        for a, b in ((f[0], f[1]), (f[1], f[2]), (f[2], f[0])):
+            edges[(a, b)] = edges.get((a, b), 0) + 1
+    assert all(c == 1 for c in edges.values())
+    assert all((b, a) in edges for (a, b) in edges)
+    P = OV[OF]
+    vol_surf = np.einsum("ij,ij->i", P[:, 0], np.cross(P[:, 1], P[:, 2])).sum() / 6.0
+    T = m.V[m.tets[lab == 0]]
+    vol_tets = np.abs(np.einsum("ij,ij->i", T[:, 1] - T[:, 0], np.cross(T[:, 2] - T[:, 0], T[:, 3] - T[:, 0]))).sum() / 6
+    assert abs(vol_surf - vol_tets) < 1e-9 * max(1.0, vol_tets)
+    # vertices compacted in ascending original id
+    used = np.unique(np.concatenate([m.tets[lab == 0].ravel()]))
+    assert OV.shape[0] <= used.shape[0]

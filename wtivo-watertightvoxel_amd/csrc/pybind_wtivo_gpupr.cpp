@@ -10,7 +10,10 @@
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 
+#include <algorithm>
+#include <chrono>
 #include <cstdint>
+#include <cstring>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -106,6 +109,50 @@ py::tuple graph_cut_fast(py::array_t<double, py::array::c_style | py::array::for
                           py::float_(r.solve_seconds), py::float_(r.native_total_seconds));
 }
 
+// surface_extraction_topology_cuda (wtivo_gpupr_bindings.cpp:400-577): same
+// checks and messages (:410-423); returns (vertices float64 [u,3], faces int32 [F,3]).
+py::tuple surface_extraction(py::array_t<std::uint8_t, py::array::c_style | py::array::forcecast> labels,
+                             py::array_t<double, py::array::c_style | py::array::forcecast> vertices,
+                             py::object tets_cuda, py::object nbr_cuda, int threads) {
+    if (vertices.ndim() != 2 || vertices.shape(1) != 3) throw std::runtime_error("vertices must be float64 Nx3");
+    if (!tets_cuda.attr("is_cuda").cast<bool>() || !nbr_cuda.attr("is_cuda").cast<bool>() || !is_int32(tets_cuda) ||
+        !is_int32(nbr_cuda) || !tets_cuda.attr("is_contiguous")().cast<bool>() ||
+        !nbr_cuda.attr("is_contiguous")().cast<bool>() || tets_cuda.attr("dim")().cast<int>() != 2 ||
+        nbr_cuda.attr("dim")().cast<int>() != 2 || tets_cuda.attr("size")(1).cast<long long>() != 4 ||
+        nbr_cuda.attr("size")(1).cast<long long>() != 4 ||
+        tets_cuda.attr("size")(0).cast<long long>() != nbr_cuda.attr("size")(0).cast<long long>())
+        throw std::runtime_error("tets/neighbors must be contiguous CUDA int32 Nx4 with same N");
+    const size_t nv = static_cast<size_t>(vertices.shape(0));
+    const size_t N = static_cast<size_t>(tets_cuda.attr("size")(0).cast<long long>());
+    if (labels.ndim() != 1 || static_cast<size_t>(labels.shape(0)) != N)
+        throw std::runtime_error("labels size mismatch");
+    const auto* dT = reinterpret_cast<const int32_t*>(tets_cuda.attr("data_ptr")().cast<uintptr_t>());
+    const auto* dN = reinterpret_cast<const int32_t*>(nbr_cuda.attr("data_ptr")().cast<uintptr_t>());
+    void* stream = current_stream();
+    wtv_surface surf{};
+    int rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    {
+        const std::uint8_t* lp = labels.data();
+        const double* V = vertices.data();
+        py::gil_scoped_release release;
+        rc = wtv_surface_extract(lp, V, nv, dT, dN, N, &surf, stream);
+    }
+    check(rc);
+    py::array_t<double> OV({static_cast<py::ssize_t>(surf.n_vertices), py::ssize_t(3)});
+    py::array_t<std::int32_t> OF({static_cast<py::ssize_t>(surf.n_faces), py::ssize_t(3)});
+    if (surf.n_vertices) std::memcpy(OV.mutable_data(), surf.vertices, surf.n_vertices * 3u * sizeof(double));
+    if (surf.n_faces) std::memcpy(OF.mutable_data(), surf.faces, surf.n_faces * 3u * sizeof(std::int32_t));
+    wtv_surface_free(&surf);
+    if (!quiet()) {
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::printf("[WTiVo-Surface] threads=%d | faces=%zu | topology on device | total=%.6gs\n", std::max(1, threads),
+                    static_cast<size_t>(OF.shape(0)), dt);
+        std::fflush(stdout);
+    }
+    return py::make_tuple(OV, OF);
+}
+
 py::dict stats_dict(const wtv_stats& s) {
     py::dict d;
     d["solve_seconds"] = s.solve_seconds;
@@ -189,6 +236,8 @@ PYBIND11_MODULE(wtivo_gpupr, m) {
     m.def("pr_solve", &pr_solve, py::arg("nbr_cuda"), py::arg("res_cuda"), py::arg("term_cuda"),
           py::arg("global_relabel_period") = 1024, py::arg("max_rounds") = 2000000ULL, py::arg("local_steps") = 8,
           py::arg("grid_wgs") = 0, py::arg("profile") = 0);
+    m.def("surface_extraction_topology_cuda", &surface_extraction, py::arg("labels"), py::arg("vertices"),
+          py::arg("tets_cuda"), py::arg("neighbors_cuda"), py::arg("threads") = 16);
     m.def("last_stats", &last_stats);
     m.def("version", []() { return std::string(wtv_version()); });
 }

@@ -1,9 +1,12 @@
 // wtv_capi.cpp -- extern "C" entry points of libwtv_gpupr.so (include/wtv_gpupr.h).
 // C++ exceptions never cross this boundary: they become status codes plus a
 // thread-local message (wtv_last_error).
+#include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <algorithm>
 #include <new>
+#include <vector>
 #include <string>
 
 #include "wtv_internal.h"
@@ -119,6 +122,36 @@ int wtv_build_reduced_graph(const double* V, size_t nv, const int32_t* d_tets, c
             info->base_flow = bi.base_flow;
         }
     });
+}
+
+int wtv_surface_extract(const uint8_t* labels, const double* V, size_t nv, const int32_t* d_tets,
+                        const int32_t* d_nbr, size_t N, wtv_surface* out, void* stream) {
+    return guarded([&]() {
+        if (!out) throw wtv::Error(WTV_EINVAL, "null surface output");
+        std::memset(out, 0, sizeof(*out));
+        std::vector<double> OV;
+        std::vector<int32_t> OF;
+        wtv::surface_extract(labels, V, nv, d_tets, d_nbr, N, OV, OF, stream);
+        out->n_vertices = OV.size() / 3u;
+        out->n_faces = OF.size() / 3u;
+        out->vertices = static_cast<double*>(std::malloc(std::max<size_t>(OV.size(), 1) * sizeof(double)));
+        out->faces = static_cast<int32_t*>(std::malloc(std::max<size_t>(OF.size(), 1) * sizeof(int32_t)));
+        if (!out->vertices || !out->faces) {
+            std::free(out->vertices);
+            std::free(out->faces);
+            std::memset(out, 0, sizeof(*out));
+            throw std::bad_alloc();
+        }
+        if (!OV.empty()) std::memcpy(out->vertices, OV.data(), OV.size() * sizeof(double));
+        if (!OF.empty()) std::memcpy(out->faces, OF.data(), OF.size() * sizeof(int32_t));
+    });
+}
+
+void wtv_surface_free(wtv_surface* s) {
+    if (!s) return;
+    std::free(s->vertices);
+    std::free(s->faces);
+    std::memset(s, 0, sizeof(*s));
 }
 
 uint64_t wtv_request_footprint(uint32_t n) { return wtv::request_footprint(n); }

@@ -276,6 +276,201 @@ __global__ void __launch_bounds__(kB) k_scatter(uint32_t n, const uint32_t* __re
     for (uint32_t q = blockIdx.x * kB + threadIdx.x; q < n; q += stride) out[free_old[q]] = part[q] ? 1u : 0u;
 }
 
+// ------------------------------------------------------------------------
+// Surface extraction (wtivo_gpupr_bindings.cpp:400-577): oriented boundary
+// triangles of label-0 tets, tet-major / slot-minor, then the used vertices
+// compacted in ascending original id.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ int surf_faces(uint8_t li, const int4& nb, uint32_t N, const uint8_t* __restrict__ labels,
+                                          int* take4, int* bad) {
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int j = slot(nb, k);
+        int t = 0;
+        if (li == 0u) {
+            if (j < 0) t = 1;
+            else if (static_cast<uint32_t>(j) >= N) *bad = 1;
+            else t = labels[j] != 0u;
+        }
+        take4[k] = t;
+        c += t;
+    }
+    return c;
+}
+
+__global__ void __launch_bounds__(kB) k_surf_count(uint32_t N, const int4* __restrict__ nbr4,
+                                                   const uint8_t* __restrict__ labels, uint32_t* __restrict__ tile_cnt,
+                                                   BuildCtl* ctl) {
+    __shared__ unsigned long long lds[kB / 64];
+    const uint32_t t0 = blockIdx.x * kTile;
+    unsigned long long c = 0;
+    int bad = 0;
+    for (uint32_t j = threadIdx.x; j < kTile; j += kB) {
+        const uint32_t i = t0 + j;
+        if (i >= N) break;
+        const uint8_t li = labels[i];
+        if (li != 0u) continue;
+        int take4[4];
+        c += surf_faces(li, nbr4[i], N, labels, take4, &bad);
+    }
+    if (bad) flag(&ctl->bad_range, 1);
+    unsigned long long x = wsum(c);
+    if (__lane_id() == 0) lds[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < kB / 64; ++i) t += lds[i];
+        tile_cnt[blockIdx.x] = static_cast<uint32_t>(t);
+        if (t) add_u64(&ctl->free_n, t);  // total faces
+    }
+}
+
+__global__ void __launch_bounds__(kB) k_surf_emit(uint32_t N, const int4* __restrict__ tets4,
+                                                  const int4* __restrict__ nbr4, const uint8_t* __restrict__ labels,
+                                                  const double* __restrict__ V, size_t nv,
+                                                  const uint32_t* __restrict__ tile_off, int32_t* __restrict__ raw,
+                                                  uint8_t* __restrict__ used, BuildCtl* ctl) {
+#pragma clang fp contract(off)
+    constexpr int lut[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+    constexpr int opp[4] = {3, 2, 1, 0};
+    __shared__ uint32_t wsums[kB / 64];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = tile_off[blockIdx.x];
+    __syncthreads();
+    const uint32_t t0 = blockIdx.x * kTile;
+    int bad = 0;
+    for (uint32_t j0 = 0; j0 < kTile; j0 += kB) {
+        const uint32_t i = t0 + j0 + threadIdx.x;
+        int take4[4] = {0, 0, 0, 0};
+        uint32_t c = 0;
+        int4 nb = make_int4(-1, -1, -1, -1);
+        uint8_t li = 1u;
+        if (i < N) {
+            li = labels[i];
+            if (li == 0u) {
+                nb = nbr4[i];
+                int dummy = 0;
+                c = static_cast<uint32_t>(surf_faces(li, nb, N, labels, take4, &dummy));
+            }
+        }
+        // exclusive scan of c over the workgroup round
+        uint32_t x = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (static_cast<int>(__lane_id()) >= o) x += y;
+        }
+        if (__lane_id() == 63) wsums[threadIdx.x >> 6] = x;
+        __syncthreads();
+        uint32_t wpre = 0;
+        for (int w = 0; w < static_cast<int>(threadIdx.x >> 6); ++w) wpre += wsums[w];
+        const uint32_t base = carry;
+        uint32_t pos = base + wpre + x - c;
+        if (c) {
+            const int4 tr = tets4[i];
+            const int tv[4] = {tr.x, tr.y, tr.z, tr.w};
+            for (int k = 0; k < 4; ++k) {
+                if (!take4[k]) continue;
+                int a0 = tv[lut[k][0]], b0 = tv[lut[k][1]], c0 = tv[lut[k][2]];
+                const int fourth = tv[opp[k]];
+                if (a0 < 0 || b0 < 0 || c0 < 0 || fourth < 0 || static_cast<size_t>(a0) >= nv ||
+                    static_cast<size_t>(b0) >= nv || static_cast<size_t>(c0) >= nv ||
+                    static_cast<size_t>(fourth) >= nv) {
+                    bad = 2;
+                    raw[3u * pos] = raw[3u * pos + 1u] = raw[3u * pos + 2u] = 0;
+                    ++pos;
+                    continue;
+                }
+                int t;
+                if (a0 > b0) { t = a0; a0 = b0; b0 = t; }
+                if (b0 > c0) { t = b0; b0 = c0; c0 = t; }
+                if (a0 > b0) { t = a0; a0 = b0; b0 = t; }
+                const double* v0 = V + static_cast<size_t>(a0) * 3u;
+                const double* v1 = V + static_cast<size_t>(b0) * 3u;
+                const double* v2 = V + static_cast<size_t>(c0) * 3u;
+                const double* vp = V + static_cast<size_t>(fourth) * 3u;
+                const double x1 = v1[0] - v0[0], y1 = v1[1] - v0[1], z1 = v1[2] - v0[2];
+                const double x2 = v2[0] - v0[0], y2 = v2[1] - v0[1], z2 = v2[2] - v0[2];
+                const double cx = y1 * z2 - z1 * y2;
+                const double cy = z1 * x2 - x1 * z2;
+                const double cz = x1 * y2 - y1 * x2;
+                const double px = vp[0] - v0[0], py = vp[1] - v0[1], pz = vp[2] - v0[2];
+                const double vol = cx * px + cy * py + cz * pz;
+                raw[3u * pos] = a0;
+                raw[3u * pos + 1u] = vol < 0.0 ? b0 : c0;
+                raw[3u * pos + 2u] = vol < 0.0 ? c0 : b0;
+                used[a0] = 1u;
+                used[b0] = 1u;
+                used[c0] = 1u;
+                ++pos;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == kB - 1) carry = base + wpre + x;
+        __syncthreads();
+    }
+    if (bad) flag(&ctl->bad_build, bad);
+}
+
+// Compaction of a 0/1 flag array in index order: per-tile counts, then ranks.
+__global__ void __launch_bounds__(kB) k_flag_count(uint32_t n, const uint8_t* __restrict__ flags,
+                                                   uint32_t* __restrict__ tile_cnt) {
+    __shared__ uint32_t lds[kB / 64];
+    const uint32_t t0 = blockIdx.x * kTile;
+    uint32_t c = 0;
+    for (uint32_t j = threadIdx.x; j < kTile; j += kB) {
+        const uint32_t i = t0 + j;
+        if (i < n) c += flags[i] != 0u;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if (__lane_id() == 0) lds[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = lds[0] + lds[1] + lds[2] + lds[3];
+}
+
+__global__ void __launch_bounds__(kB) k_flag_rank(uint32_t n, const uint8_t* __restrict__ flags,
+                                                  const uint32_t* __restrict__ tile_base, int32_t* __restrict__ rank_out) {
+    __shared__ uint32_t wcnt[kB / 64];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = tile_base[blockIdx.x];
+    __syncthreads();
+    const uint32_t t0 = blockIdx.x * kTile;
+    for (uint32_t j0 = 0; j0 < kTile; j0 += kB) {
+        const uint32_t i = t0 + j0 + threadIdx.x;
+        const bool f = i < n && flags[i] != 0u;
+        const uint64_t m = __ballot(f);
+        const uint32_t r = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+        if (__lane_id() == 0) wcnt[threadIdx.x >> 6] = static_cast<uint32_t>(__popcll(m));
+        __syncthreads();
+        uint32_t wpre = 0;
+        for (int w = 0; w < static_cast<int>(threadIdx.x >> 6); ++w) wpre += wcnt[w];
+        const uint32_t c = carry;
+        if (i < n) rank_out[i] = f ? static_cast<int32_t>(c + wpre + r) : -1;
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kB) k_surf_finish(uint32_t nv, uint64_t nidx, const double* __restrict__ V,
+                                                    const int32_t* __restrict__ remap, double* __restrict__ ov,
+                                                    int32_t* __restrict__ raw) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kB;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kB + threadIdx.x; i < nv; i += stride) {
+        const int32_t r = remap[i];
+        if (r >= 0) {
+            ov[3u * static_cast<size_t>(r)] = V[3u * i];
+            ov[3u * static_cast<size_t>(r) + 1u] = V[3u * i + 1u];
+            ov[3u * static_cast<size_t>(r) + 2u] = V[3u * i + 2u];
+        }
+    }
+    for (uint64_t q = static_cast<uint64_t>(blockIdx.x) * kB + threadIdx.x; q < nidx; q += stride)
+        raw[q] = remap[raw[q]];
+}
+
 struct Dev {
     void* p = nullptr;
     ~Dev() { if (p) (void)hipFree(p); }
@@ -437,6 +632,80 @@ void build_reduced_graph(const double* V, size_t nv, const int32_t* dT, const in
     ckg(hipMemcpyAsync(dLab.p, labels, N, hipMemcpyHostToDevice, s), "labels H2D");
     build_reduced(V, nv, dT, dN, N, dLab.as<uint8_t>(), fill_T, gN, gR, gT, free_old, info, s);
     ckg(hipStreamSynchronize(s), "build");
+}
+
+void surface_extract(const uint8_t* lp, const double* V, size_t nv, const int32_t* dT, const int32_t* dN, size_t N,
+                     std::vector<double>& OV, std::vector<int32_t>& OF, void* stream_v) {
+    OV.clear();
+    OF.clear();
+    if ((!lp && N) || (!V && nv) || (!dT && N) || (!dN && N)) fail(WTV_EINVAL, "null pointer argument");
+    if (N > static_cast<size_t>(std::numeric_limits<uint32_t>::max()) ||
+        nv > static_cast<size_t>(std::numeric_limits<int32_t>::max()))
+        fail(WTV_EINVAL, ">2^32 full tets unsupported");
+    if (N == 0) return;
+    hipStream_t s = static_cast<hipStream_t>(stream_v);
+    const uint32_t n32 = static_cast<uint32_t>(N);
+    const uint32_t tiles = static_cast<uint32_t>((N + kTile - 1) / kTile);
+    const uint32_t vtiles = static_cast<uint32_t>((std::max<size_t>(nv, 1) + kTile - 1) / kTile);
+    Dev dLab, dTiles, dCtl, dV, dUsed, dRemap, dVt;
+    ckg(hipMalloc(&dLab.p, N), "malloc labels");
+    ckg(hipMalloc(&dTiles.p, static_cast<size_t>(tiles) * 4u), "malloc tiles");
+    ckg(hipMalloc(&dCtl.p, sizeof(BuildCtl)), "malloc ctl");
+    ckg(hipMemcpyAsync(dLab.p, lp, N, hipMemcpyHostToDevice, s), "labels H2D");
+    ckg(hipMemsetAsync(dCtl.p, 0, sizeof(BuildCtl), s), "ctl clear");
+    BuildCtl* ctl = dCtl.as<BuildCtl>();
+    BuildCtl h{};
+    const int4* nbr4 = reinterpret_cast<const int4*>(dN);
+    const int4* tets4 = reinterpret_cast<const int4*>(dT);
+    k_surf_count<<<tiles, kB, 0, s>>>(n32, nbr4, dLab.as<uint8_t>(), dTiles.as<uint32_t>(), ctl);
+    ckg(hipGetLastError(), "surface count");
+    ckg(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s), "surface D2H");
+    ckg(hipStreamSynchronize(s), "surface count sync");
+    if (h.bad_range) fail(WTV_EINVAL, "surface neighbor out of range");
+    const uint64_t F = h.free_n;
+    if (F * 3u > 0x7FFFFFFFull) fail(WTV_EINVAL, "surface too large");
+    Dev dRaw;
+    ckg(hipMalloc(&dRaw.p, std::max<uint64_t>(F, 1) * 12u), "malloc faces");
+    ckg(hipMalloc(&dV.p, std::max<size_t>(nv, 1) * 24u), "malloc vertices");
+    ckg(hipMalloc(&dUsed.p, std::max<size_t>(nv, 1)), "malloc used");
+    ckg(hipMalloc(&dRemap.p, std::max<size_t>(nv, 1) * 4u), "malloc remap");
+    ckg(hipMalloc(&dVt.p, static_cast<size_t>(vtiles) * 4u), "malloc vertex tiles");
+    if (nv) ckg(hipMemcpyAsync(dV.p, V, nv * 24u, hipMemcpyHostToDevice, s), "vertices H2D");
+    ckg(hipMemsetAsync(dUsed.p, 0, std::max<size_t>(nv, 1), s), "used clear");
+    k_tile_scan<<<1, kB, 0, s>>>(tiles, dTiles.as<uint32_t>());
+    k_surf_emit<<<tiles, kB, 0, s>>>(n32, tets4, nbr4, dLab.as<uint8_t>(), dV.as<double>(), nv, dTiles.as<uint32_t>(),
+                                     dRaw.as<int32_t>(), dUsed.as<uint8_t>(), ctl);
+    ckg(hipGetLastError(), "surface emit");
+    ckg(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s), "surface D2H");
+    ckg(hipStreamSynchronize(s), "surface emit sync");
+    if (h.bad_build) fail(WTV_EINVAL, "surface invalid tet vertex index");
+    const uint32_t nv32 = static_cast<uint32_t>(nv);
+    k_flag_count<<<vtiles, kB, 0, s>>>(nv32, dUsed.as<uint8_t>(), dVt.as<uint32_t>());
+    k_tile_scan<<<1, kB, 0, s>>>(vtiles, dVt.as<uint32_t>());
+    k_flag_rank<<<vtiles, kB, 0, s>>>(nv32, dUsed.as<uint8_t>(), dVt.as<uint32_t>(), dRemap.as<int32_t>());
+    uint32_t last_base = 0, last_cnt = 0;
+    // used count = last tile base + flags in the last tile
+    ckg(hipMemcpyAsync(&last_base, dVt.as<uint32_t>() + (vtiles - 1), 4, hipMemcpyDeviceToHost, s), "used D2H");
+    ckg(hipStreamSynchronize(s), "used sync");
+    std::vector<uint8_t> tail(std::min<size_t>(nv, kTile));
+    const size_t tail0 = static_cast<size_t>(vtiles - 1) * kTile;
+    if (nv > tail0) {
+        ckg(hipMemcpyAsync(tail.data(), dUsed.as<uint8_t>() + tail0, nv - tail0, hipMemcpyDeviceToHost, s), "tail");
+        ckg(hipStreamSynchronize(s), "tail sync");
+        for (size_t i = 0; i < nv - tail0; ++i) last_cnt += tail[i] != 0u;
+    }
+    const size_t used = static_cast<size_t>(last_base) + last_cnt;
+    Dev dOV;
+    ckg(hipMalloc(&dOV.p, std::max<size_t>(used, 1) * 24u), "malloc out vertices");
+    const int grid = static_cast<int>(std::min<uint64_t>((std::max<uint64_t>(nv, F * 3u) + kB - 1) / kB, 8192));
+    k_surf_finish<<<std::max(grid, 1), kB, 0, s>>>(nv32, F * 3u, dV.as<double>(), dRemap.as<int32_t>(),
+                                                   dOV.as<double>(), dRaw.as<int32_t>());
+    ckg(hipGetLastError(), "surface finish");
+    OV.resize(used * 3u);
+    OF.resize(F * 3u);
+    if (used) ckg(hipMemcpyAsync(OV.data(), dOV.p, used * 24u, hipMemcpyDeviceToHost, s), "OV D2H");
+    if (F) ckg(hipMemcpyAsync(OF.data(), dRaw.p, F * 12u, hipMemcpyDeviceToHost, s), "OF D2H");
+    ckg(hipStreamSynchronize(s), "surface sync");
 }
 
 }  // namespace wtv

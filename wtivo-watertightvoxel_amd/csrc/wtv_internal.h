@@ -4,6 +4,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "wtv_gpupr.h"
 
@@ -47,6 +48,10 @@ void build_reduced_graph(const double* V, size_t nv, const int32_t* dT, const in
 void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t* dN, size_t N,
                     const uint8_t* lp, double fill_T, int requested_threads, int period, uint64_t max_rounds,
                     int local_steps, uint8_t* out_labels, wtv_cut_result& R, wtv_stats& st, void* stream_v);
+
+// surface_extraction_topology_cuda on the device (wtv_graph_build.hip).
+void surface_extract(const uint8_t* labels, const double* V, size_t nv, const int32_t* dT, const int32_t* dN,
+                     size_t N, std::vector<double>& OV, std::vector<int32_t>& OF, void* stream);
 
 // Thread-local stats of the last solve (wtv_capi.cpp).
 void set_last_stats(const wtv_stats& st);

@@ -15,7 +15,8 @@ WTV_OK, WTV_EINVAL, WTV_EOVERFLOW, WTV_EINVARIANT, WTV_ENOMEM, WTV_ELIMIT, WTV_E
 
 # every extern "C" symbol declared in include/wtv_gpupr.h
 EXPORTS = ["wtv_params_default", "wtv_pr_solve", "wtv_batch_solve", "wtv_graph_cut_fast", "wtv_last_stats",
-           "wtv_last_error", "wtv_version", "wtv_build_reduced_graph", "wtv_request_footprint", "wtv_server_create", "wtv_server_submit",
+           "wtv_last_error", "wtv_version", "wtv_build_reduced_graph", "wtv_surface_extract",
+           "wtv_surface_free", "wtv_request_footprint", "wtv_server_create", "wtv_server_submit",
            "wtv_server_wait", "wtv_server_destroy"]
 
 
