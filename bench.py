@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--batch-max", type=int, default=4_000_000, help="batched leg: largest graph (tets)")
     ap.add_argument("--serve-requests", type=int, default=48, help="serving leg: number of requests (0 = skip)")
     ap.add_argument("--serve-load", type=float, default=0.8, help="offered load vs looped-solve capacity")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy three times (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); "
@@ -65,6 +67,16 @@ def main():
 
     import numpy as np
     import torch
+
+    if a.copy_probe:
+        src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 1 << 30, "copy_probe_bytes_written": 1 << 30, "launches": 3}))
+        return
+
     import wtv_capi as W
     import wtv_synth as S
 
