@@ -72,6 +72,9 @@ typedef struct wtv_stats {
     uint64_t n_level;
     uint64_t bfs_bu_levels;  /* BFS levels expanded bottom-up                */
     uint64_t bfs_bu_checks;  /* unreached vertices examined bottom-up        */
+    double t_tail_ms;        /* one-workgroup discharge launches (profile)   */
+    uint64_t n_tail;
+    uint64_t tail_sub_rounds; /* sub-rounds run by the one-workgroup kernel  */
 } wtv_stats;
 
 /* Solver knobs.  global_relabel_period, max_rounds and local_steps keep the
@@ -86,6 +89,10 @@ typedef struct wtv_params {
     int32_t grid_wgs;              /* 0 = default            */
     double time_limit_s;           /* 0 = 240 s (reference)  */
     int32_t profile;               /* 1 = time every kernel with HIP events */
+    int32_t no_graph;              /* 1 = launch kernels one by one instead of
+                                      replaying a captured macro round      */
+    int32_t tail_cap;              /* active lists up to this size run in one
+                                      workgroup; 0 = default 2048, < 0 = off */
     double relabel_alpha;          /* also relabel globally once the height
                                       increases since the last global relabel
                                       exceed alpha * (active vertices after it)

@@ -460,3 +460,41 @@ def test_surface_after_graph_cut_and_errors(orc, torch_cuda):
         G.surface_extraction_topology_cuda(new_labels, m.V, T, torch_cuda.from_numpy(bad).cuda())
     with pytest.raises(RuntimeError, match="contiguous CUDA int32 Nx4 with same N"):
         G.surface_extraction_topology_cuda(new_labels, m.V, T.long(), Nb)
+
+
+def test_graph_replay_equals_plain_launches(orc, torch_cuda):
+    """Macro rounds replayed from a captured hipGraph (default) give the same
+    answer as kernel-by-kernel launches, over many replays and odd/even local_steps."""
+    _, _, g = _c0_soft(24, 70)
+    want = orc.bk_solve(g.nbr, g.res, g.term)
+    for ls in (1, 3, 8):
+        for no_graph in (0, 1):
+            gpu = gpu_solve(torch_cuda, g.nbr, g.res, g.term, local_steps=ls, no_graph=no_graph)
+            assert_same(orc, g.nbr, g.res, g.term, gpu, want.raw_flow, want.partition)
+            assert gpu[1]["rounds"] >= 1
+
+
+def test_profile_mode_counts_kernels(torch_cuda):
+    _, _, g = _c0_soft(16, 71)
+    part, st, _, _ = gpu_solve(torch_cuda, g.nbr, g.res, g.term, profile=1)
+    assert st["sub_rounds"] > 0 and st["n_level"] > 0 and st["t_level_ms"] > 0
+    assert st["n_push"] + st["tail_sub_rounds"] >= st["sub_rounds"]
+    assert st["n_push"] == st["n_relabel"] and (st["n_tail"] > 0) == (st["tail_sub_rounds"] > 0)
+    _, st2, _, _ = gpu_solve(torch_cuda, g.nbr, g.res, g.term, profile=1, tail_cap=-1)
+    assert st2["n_tail"] == 0 and st2["n_push"] >= st2["sub_rounds"] and st2["t_push_ms"] > 0
+
+
+@pytest.mark.parametrize("tail_cap", [-1, 16, 2048, 1 << 20])
+def test_discharge_tail_mode_same_answer(orc, torch_cuda, tail_cap):
+    """The one-workgroup discharge (small active lists) and the multi-workgroup
+    rounds compute the same flow and labels, whichever handles which phase."""
+    _, _, g = _c0_soft(24, 80)
+    want = orc.bk_solve(g.nbr, g.res, g.term)
+    gpu = gpu_solve(torch_cuda, g.nbr, g.res, g.term, tail_cap=tail_cap)
+    assert_same(orc, g.nbr, g.res, g.term, gpu, want.raw_flow, want.partition)
+    graphs = [S.random_graph(int(200 + 50 * i), 1200 + i, cap_max=30) for i in range(5)]
+    off, flows, part, _, _, _ = gpu_batch(torch_cuda, graphs, tail_cap=tail_cap)
+    for i, gr in enumerate(graphs):
+        b = orc.bk_solve(*gr)
+        assert flows[i] == b.raw_flow
+        assert np.array_equal(part[int(off[i]):int(off[i + 1])], b.partition)

@@ -177,6 +177,9 @@ py::dict stats_dict(const wtv_stats& s) {
     d["n_level"] = s.n_level;
     d["bfs_bu_levels"] = s.bfs_bu_levels;
     d["bfs_bu_checks"] = s.bfs_bu_checks;
+    d["t_tail_ms"] = s.t_tail_ms;
+    d["n_tail"] = s.n_tail;
+    d["tail_sub_rounds"] = s.tail_sub_rounds;
     return d;
 }
 

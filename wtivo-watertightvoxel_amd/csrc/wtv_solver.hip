@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <sstream>
 #include <vector>
@@ -55,8 +56,8 @@ struct Ctl {
     uint32_t rl_cnt[2];   // relabel list counts (ping-pong)
     uint32_t bfs_cnt[3];  // BFS frontier counts, ring indexed by level % 3
     uint32_t bfs_levels;
-    uint32_t tail_d;      // level reached by the one-workgroup tail kernel
-    uint32_t pad0;
+    uint32_t tail_d;      // level reached by the one-workgroup BFS tail kernel
+    uint32_t tail_sub;    // sub-rounds run by the one-workgroup discharge kernel
     unsigned long long sink_flow;
     unsigned long long visits, pushes, relabels, bfs_pops;
     unsigned long long bu_levels, bu_checks;
@@ -416,6 +417,109 @@ __global__ void __launch_bounds__(kTPB) k_build_active(uint32_t n, const long lo
     }
 }
 
+// Push step of one worklist vertex v (heights read-only during the phase).
+// first: v claimed its relabel-list slot; cand[k]: neighbour k received its
+// first excess of the phase and claimed a slot.
+template <bool kB>
+__device__ __forceinline__ void push_vertex(bool valid, uint32_t v, uint32_t n, const int4* __restrict__ nbr4,
+                                            const uint8_t* __restrict__ rslot, uint32_t* res,
+                                            int32_t* __restrict__ term, long long* excess,
+                                            const uint32_t* __restrict__ height, uint32_t* mark,
+                                            const uint32_t* __restrict__ gid, const uint32_t* __restrict__ gn,
+                                            unsigned long long* __restrict__ gflow, bool& first, bool cand[4],
+                                            int4& nb, unsigned long long& visits, unsigned long long& pushes,
+                                            unsigned long long& sinkf) {
+    first = valid && atom_xchg(&mark[v], 1u) == 0u;
+    long long e = valid ? atom_load(&excess[v]) : 0LL;
+    const uint32_t h = valid ? height[v] : kSent;
+    const bool act = e > 0 && h < (valid ? src_h<kB>(v, n, gid, gn) : 0u);
+    nb = make_int4(-1, -1, -1, -1);
+    uint4 r = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t rs = 0;
+    uint32_t hu[4] = {kSent, kSent, kSent, kSent};
+    long long pushed = 0;
+    if (act) {
+        ++visits;
+        nb = nbr4[v];
+        r = reinterpret_cast<const uint4*>(res)[v];
+        rs = rslot[v];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int u = slot_of(nb, k);
+            if (u >= 0) hu[k] = height[u];
+        }
+        // v -> sink (gpu_push_relabel_fast.cu:220-236)
+        const int32_t t = term[v];
+        if (t < 0 && h == 1u) {
+            const long long cap = -static_cast<long long>(t);
+            const long long dlt = e < cap ? e : cap;
+            term[v] = static_cast<int32_t>(static_cast<long long>(t) + dlt);
+            e -= dlt;
+            pushed += dlt;
+            sinkf += static_cast<unsigned long long>(dlt);
+            ++pushes;
+            if constexpr (kB) atom_add(&gflow[gid[v]], static_cast<unsigned long long>(dlt));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int u = slot_of(nb, k);
+        const uint32_t rk = slot_of(r, k);
+        cand[k] = false;
+        if (e > 0 && u >= 0 && rk != 0u && h == hu[k] + 1u) {
+            const uint32_t dlt = static_cast<long long>(rk) < e ? rk : static_cast<uint32_t>(e);
+            res[static_cast<size_t>(v) * 4u + k] = rk - dlt;
+            const uint32_t back = (rs >> (2 * k)) & 3u;
+            atom_add(&res[static_cast<size_t>(u) * 4u + back], dlt);
+            const long long old = atom_add(&excess[u], static_cast<long long>(dlt));
+            e -= dlt;
+            pushed += dlt;
+            ++pushes;
+            if (old == 0) cand[k] = atom_xchg(&mark[u], 1u) == 0u;
+        }
+    }
+    if (pushed) atom_add(&excess[v], -pushed);
+}
+
+// Relabel step of one relabel-list vertex: height = 1 + min over residual
+// arcs (sink 0 when term < 0, source = own source height when term > 0);
+// returns whether v stays active.
+template <bool kB>
+__device__ __forceinline__ bool relabel_vertex(uint32_t v, uint32_t n, const int4* __restrict__ nbr4,
+                                               const uint32_t* __restrict__ res, const int32_t* __restrict__ term,
+                                               const long long* __restrict__ excess, uint32_t* height,
+                                               uint32_t* __restrict__ mark, const uint32_t* __restrict__ gid,
+                                               const uint32_t* __restrict__ gn, Ctl* ctl,
+                                               unsigned long long& relabels) {
+    mark[v] = 0u;
+    const long long e = excess[v];
+    uint32_t h = height[v];
+    const uint32_t sh = src_h<kB>(v, n, gid, gn);
+    if (!(e > 0 && h < sh)) return false;
+    const int32_t t = term[v];
+    uint32_t mh = kSent;
+    if (t < 0) mh = 0u;
+    else if (t > 0) mh = sh;
+    const int4 nb = nbr4[v];
+    const uint4 r = reinterpret_cast<const uint4*>(res)[v];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int u = slot_of(nb, k);
+        if (u >= 0 && slot_of(r, k) != 0u) {
+            const uint32_t x = height[u];
+            mh = x < mh ? x : mh;
+        }
+    }
+    if (mh == kSent) {
+        set_error(ctl, 5);
+    } else if (mh + 1u > h) {
+        h = mh + 1u;
+        height[v] = h;
+        ++relabels;
+    }
+    return h < sh;
+}
+
 // Push phase of sub-round with parity p (heights read-only).
 template <bool kB>
 __global__ void __launch_bounds__(kTPB) k_push(uint32_t n, uint32_t p, const uint32_t* __restrict__ wl,
@@ -436,60 +540,14 @@ __global__ void __launch_bounds__(kTPB) k_push(uint32_t n, uint32_t p, const uin
         const uint32_t i = base + threadIdx.x;
         const bool valid = i < cnt;
         const uint32_t v = valid ? wl[i] : 0u;
+        bool first, cand[4];
+        int4 nb;
+        push_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, mark, gid, gn, gflow, first, cand, nb,
+                        visits, pushes, sinkf);
         // every worklist vertex is relabel-checked exactly once this sub-round
-        const bool first = valid && atom_xchg(&mark[v], 1u) == 0u;
         q_push(q, first, v);
-
-        long long e = valid ? atom_load(&excess[v]) : 0LL;
-        const uint32_t h = valid ? height[v] : kSent;
-        const bool act = e > 0 && h < (valid ? src_h<kB>(v, n, gid, gn) : 0u);
-        int4 nb = make_int4(-1, -1, -1, -1);
-        uint4 r = make_uint4(0u, 0u, 0u, 0u);
-        uint32_t rs = 0;
-        uint32_t hu[4] = {kSent, kSent, kSent, kSent};
-        long long pushed = 0;
-        if (act) {
-            ++visits;
-            nb = nbr4[v];
-            r = reinterpret_cast<const uint4*>(res)[v];
-            rs = rslot[v];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int u = slot_of(nb, k);
-                if (u >= 0) hu[k] = height[u];
-            }
-            // v -> sink (gpu_push_relabel_fast.cu:220-236)
-            const int32_t t = term[v];
-            if (t < 0 && h == 1u) {
-                const long long cap = -static_cast<long long>(t);
-                const long long dlt = e < cap ? e : cap;
-                term[v] = static_cast<int32_t>(static_cast<long long>(t) + dlt);
-                e -= dlt;
-                pushed += dlt;
-                sinkf += static_cast<unsigned long long>(dlt);
-                ++pushes;
-                if constexpr (kB) atom_add(&gflow[gid[v]], static_cast<unsigned long long>(dlt));
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int u = slot_of(nb, k);
-            const uint32_t rk = slot_of(r, k);
-            bool cand = false;
-            if (e > 0 && u >= 0 && rk != 0u && h == hu[k] + 1u) {
-                const uint32_t dlt = static_cast<long long>(rk) < e ? rk : static_cast<uint32_t>(e);
-                res[static_cast<size_t>(v) * 4u + k] = rk - dlt;
-                const uint32_t back = (rs >> (2 * k)) & 3u;
-                atom_add(&res[static_cast<size_t>(u) * 4u + back], dlt);
-                const long long old = atom_add(&excess[u], static_cast<long long>(dlt));
-                e -= dlt;
-                pushed += dlt;
-                ++pushes;
-                if (old == 0) cand = atom_xchg(&mark[u], 1u) == 0u;
-            }
-            q_push(q, cand, static_cast<uint32_t>(u));
-        }
-        if (pushed) atom_add(&excess[v], -pushed);
+        for (int k = 0; k < 4; ++k) q_push(q, cand[k], static_cast<uint32_t>(slot_of(nb, k)));
         q_flush(q, rl, rl_cnt);
     }
     flush_counter(&ctl->visits, visits);
@@ -519,39 +577,116 @@ __global__ void __launch_bounds__(kTPB) k_relabel(uint32_t n, uint32_t p, const 
         uint32_t v = 0;
         if (i < cnt) {
             v = rl[i];
-            mark[v] = 0u;
-            const long long e = excess[v];
-            uint32_t h = height[v];
-            const uint32_t sh = src_h<kB>(v, n, gid, gn);
-            if (e > 0 && h < sh) {
-                const int32_t t = term[v];
-                uint32_t mh = kSent;
-                if (t < 0) mh = 0u;
-                else if (t > 0) mh = sh;
-                const int4 nb = nbr4[v];
-                const uint4 r = reinterpret_cast<const uint4*>(res)[v];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int u = slot_of(nb, k);
-                    if (u >= 0 && slot_of(r, k) != 0u) {
-                        const uint32_t x = height[u];
-                        mh = x < mh ? x : mh;
-                    }
-                }
-                if (mh == kSent) {
-                    set_error(ctl, 5);
-                } else if (mh + 1u > h) {
-                    h = mh + 1u;
-                    height[v] = h;
-                    ++relabels;
-                }
-                act = h < sh;
-            }
+            act = relabel_vertex<kB>(v, n, nbr4, res, term, excess, height, mark, gid, gn, ctl, relabels);
         }
         q_push(q, act, v);
         q_flush(q, wl_next, wl_cnt);
     }
     flush_counter(&ctl->relabels, relabels);
+}
+
+// LDS-counter append for a single-workgroup kernel: ranks from ballot + mbcnt,
+// one LDS atomic per wave, the item goes straight to the global list.
+__device__ __forceinline__ void lds_append(bool pred, uint32_t val, uint32_t* list, uint32_t* s_cnt) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const uint32_t rank = lane_rank(m);
+    const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+    uint32_t off = 0;
+    if (pred && rank == 0) off = atomicAdd(s_cnt, static_cast<uint32_t>(__popcll(m)));
+    off = __shfl(off, leader);
+    if (pred) list[off + rank] = val;
+}
+
+// Discharge tail: when the active list is small, one workgroup runs the same
+// synchronous sub-rounds back to back, with __syncthreads between the push
+// and relabel phases instead of kernel boundaries.  Stops when the list
+// empties, grows beyond `cap`, after max_sub sub-rounds, or once rel_budget
+// height increases were made (the host then runs a global relabel).  Leaves
+// the counters exactly as the same number of k_push/k_relabel pairs would.
+template <bool kB>
+__global__ void __launch_bounds__(kTPB) k_discharge_tail(uint32_t n, uint32_t p0, uint32_t cap, uint32_t max_sub,
+                                                         unsigned long long rel_budget, uint32_t* wl0, uint32_t* wl1,
+                                                         uint32_t* rl, const int4* __restrict__ nbr4,
+                                                         const uint8_t* __restrict__ rslot, uint32_t* res,
+                                                         int32_t* __restrict__ term, long long* excess,
+                                                         uint32_t* height, uint32_t* mark,
+                                                         const uint32_t* __restrict__ gid,
+                                                         const uint32_t* __restrict__ gn,
+                                                         unsigned long long* __restrict__ gflow, Ctl* ctl) {
+    __shared__ uint32_t s_cnt;
+    __shared__ uint32_t s_cur;
+    __shared__ unsigned long long s_rel;
+    __shared__ unsigned long long s_wrel[kTPB / 64];
+    uint32_t* wl[2] = {wl0, wl1};
+    if (threadIdx.x == 0) {
+        s_cur = ctl->wl_cnt[p0];
+        s_rel = 0ull;
+    }
+    __syncthreads();
+    unsigned long long visits = 0, pushes = 0, sinkf = 0, relabels_all = 0;
+    uint32_t sub = 0;
+    for (;;) {
+        const uint32_t cur = s_cur;
+        if (cur == 0u || cur > cap || sub >= max_sub || s_rel >= rel_budget) break;
+        const uint32_t p = (p0 + sub) & 1u;
+        // push phase
+        if (threadIdx.x == 0) s_cnt = 0u;
+        __syncthreads();
+        for (uint32_t base = 0; base < cur; base += kTPB) {
+            const uint32_t i = base + threadIdx.x;
+            const bool valid = i < cur;
+            const uint32_t v = valid ? wl[p][i] : 0u;
+            bool first, cand[4];
+            int4 nb;
+            push_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, mark, gid, gn, gflow, first, cand,
+                            nb, visits, pushes, sinkf);
+            lds_append(first, v, rl, &s_cnt);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) lds_append(cand[k], static_cast<uint32_t>(slot_of(nb, k)), rl, &s_cnt);
+        }
+        __syncthreads();
+        const uint32_t rcnt = s_cnt;
+        __syncthreads();
+        // relabel phase
+        if (threadIdx.x == 0) s_cnt = 0u;
+        __syncthreads();
+        unsigned long long rel = 0;
+        for (uint32_t base = 0; base < rcnt; base += kTPB) {
+            const uint32_t i = base + threadIdx.x;
+            bool act = false;
+            uint32_t v = 0;
+            if (i < rcnt) {
+                v = rl[i];
+                act = relabel_vertex<kB>(v, n, nbr4, res, term, excess, height, mark, gid, gn, ctl, rel);
+            }
+            lds_append(act, v, wl[p ^ 1u], &s_cnt);
+        }
+        relabels_all += rel;
+        rel = wave_sum(rel);
+        if (__lane_id() == 0) s_wrel[threadIdx.x >> 6] = rel;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long t = 0;
+            for (int w = 0; w < kTPB / 64; ++w) t += s_wrel[w];
+            s_rel += t;
+            s_cur = s_cnt;
+        }
+        __syncthreads();
+        ++sub;
+    }
+    flush_counter(&ctl->visits, visits);
+    flush_counter(&ctl->pushes, pushes);
+    flush_counter(&ctl->sink_flow, sinkf);
+    flush_counter(&ctl->relabels, relabels_all);
+    if (threadIdx.x == 0) {
+        const uint32_t pf = (p0 + sub) & 1u;
+        ctl->wl_cnt[pf] = s_cur;
+        ctl->wl_cnt[pf ^ 1u] = 0u;
+        ctl->rl_cnt[0] = 0u;
+        ctl->rl_cnt[1] = 0u;
+        ctl->tail_sub = sub;
+    }
 }
 
 // partition = reaches the sink (gpu_push_relabel_fast.cu:315-324); counts any
@@ -596,49 +731,68 @@ struct Event {
 };
 
 // Per-launch HIP event pairs on the solver's stream (profile mode only).
-// Pairs are harvested after each host synchronisation and recycled.
+// Completed pairs are harvested in launch order; events are not reused within
+// a solve, so pairs of launches still in flight can stay pending.
 struct KernelTimer {
-    enum Kind { kPush = 0, kRelabel = 1, kLevel = 2 };
+    enum Kind { kPush = 0, kRelabel = 1, kLevel = 2, kTail = 3 };
     bool on = false;
     hipStream_t s = nullptr;
     std::vector<hipEvent_t> pool;
     std::vector<std::pair<int, size_t>> pending;  // (kind, index of start event)
-    size_t used = 0;
-    double ms[3] = {0, 0, 0};
-    uint64_t count[3] = {0, 0, 0};
+    size_t head = 0;                              // first pending entry not yet harvested
+    double ms[4] = {0, 0, 0, 0};
+    uint64_t count[4] = {0, 0, 0, 0};
     ~KernelTimer() { for (auto e : pool) (void)hipEventDestroy(e); }
     hipEvent_t next() {
-        if (used == pool.size()) {
-            hipEvent_t e;
-            ck(hipEventCreate(&e), "event create");
-            pool.push_back(e);
-        }
-        return pool[used++];
+        hipEvent_t e;
+        ck(hipEventCreate(&e), "event create");
+        pool.push_back(e);
+        return e;
     }
     void begin(int kind) {
         if (!on) return;
-        pending.emplace_back(kind, used);
+        pending.emplace_back(kind, pool.size());
         ck(hipEventRecord(next(), s), "event");
     }
     void end() {
         if (!on) return;
         ck(hipEventRecord(next(), s), "event");
     }
-    // call only after the stream has been synchronised
     void harvest() {
         if (!on) return;
-        for (auto& pr : pending) {
+        while (head < pending.size()) {
+            const auto& pr = pending[head];
+            if (hipEventQuery(pool[pr.second + 1]) != hipSuccess) break;
             float t = 0.f;
             ck(hipEventElapsedTime(&t, pool[pr.second], pool[pr.second + 1]), "event time");
             ms[pr.first] += t;
             count[pr.first] += 1;
+            ++head;
         }
-        pending.clear();
-        used = 0;
     }
 };
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// One non-blocking stream per host thread and device for the solver's own
+// work (default priority).  Graph capture needs a stream other than the null
+// stream; the caller's stream is ordered before and after through events.
+hipStream_t solver_stream() {
+    thread_local std::vector<std::pair<int, hipStream_t>> cache;
+    int dev = 0;
+    ck(hipGetDevice(&dev), "hipGetDevice");
+    for (auto& e : cache)
+        if (e.first == dev) return e.second;
+    hipStream_t st = nullptr;
+    ck(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream create");
+    cache.emplace_back(dev, st);
+    return st;
+}
+
+struct GraphExec {
+    hipGraphExec_t exec = nullptr;
+    ~GraphExec() { if (exec) (void)hipGraphExecDestroy(exec); }
+};
 
 struct Layout {
     size_t height, excess, mark, rslot, wl0, wl1, rl, ctl, gid, nbrg, gn, goff, gflow, total;
@@ -698,7 +852,15 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     const int sweep_grid = static_cast<int>(std::min<uint64_t>((static_cast<uint64_t>(n) + tpb - 1) / tpb, 8192));
     // a level whose frontier holds more than n/32 vertices expands bottom-up
     const uint32_t bu_threshold = std::max<uint32_t>(n / 32u, 4096u);
-    hipStream_t s = static_cast<hipStream_t>(stream_v);
+    hipStream_t cs = static_cast<hipStream_t>(stream_v);
+    hipStream_t s = solver_stream();
+    Event ev_caller, ev_done;
+    ck(hipEventRecord(ev_caller.e, cs), "event");
+    ck(hipStreamWaitEvent(s, ev_caller.e, 0), "stream wait");
+    const bool use_graph = prm.profile == 0 && prm.no_graph == 0;
+    // WTV_TRACE=1: one stderr line per macro round / global relabel (diagnostics)
+    const char* trace_env = std::getenv("WTV_TRACE");
+    const bool trace = trace_env && *trace_env && *trace_env != '0';
 
     const uint32_t G = kB ? B->G : 0u;
     const Layout L = layout_for(n, G);
@@ -720,8 +882,9 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     unsigned long long* gflow = kB ? reinterpret_cast<unsigned long long*>(base + L.gflow) : nullptr;
 
     HostBuf hctl_buf;
-    ck(hipHostMalloc(&hctl_buf.p, sizeof(Ctl), hipHostMallocDefault), "pinned ctl");
+    ck(hipHostMalloc(&hctl_buf.p, 3 * sizeof(Ctl), hipHostMallocDefault), "pinned ctl");
     Ctl* hctl = static_cast<Ctl*>(hctl_buf.p);
+    Ctl* snap = hctl + 1;  // two pipelined snapshots
 
     KernelTimer* ktp = nullptr;
     auto read_ctl = [&](const char* where) {
@@ -803,6 +966,9 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         ck(hipEventElapsedTime(&ms, ev_b0.e, ev_b1.e), "event time");
         bfs_ms += ms;
         ++st.global_relabels;
+        if (trace)
+            std::fprintf(stderr, "[wtv] global relabel %llu: levels=%u %.3f ms\n",
+                         static_cast<unsigned long long>(st.global_relabels), hctl->bfs_levels, ms);
     };
     auto build_active = [&]() -> uint32_t {
         const uint32_t p = static_cast<uint32_t>(sub & 1u);
@@ -825,41 +991,163 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     uint64_t relabels_at_gr = hctl->relabels;
     uint64_t active_at_gr = active;
 
+    // One macro round = local_steps sub-rounds (push + relabel).  Rounds are
+    // replayed from a captured graph (one per starting parity) and pipelined:
+    // round r+1 is enqueued before the counters of round r are read, so the
+    // device never idles on the host's decision.
+    GraphExec gexec[2];
+    auto launch_round = [&]() {
+        const uint32_t p0 = static_cast<uint32_t>(sub & 1u);
+        auto enqueue = [&](uint64_t sub0) {
+            for (int j = 0; j < local_steps; ++j) {
+                const uint32_t p = static_cast<uint32_t>((sub0 + static_cast<uint64_t>(j)) & 1u);
+                kt.begin(KernelTimer::kPush);
+                k_push<kB><<<grid, tpb, 0, s>>>(n, p, wl[p], rl, nbr4, rslot, d_res, d_term, excess, height, mark,
+                                                gid, gn, gflow, ctl);
+                kt.end();
+                kt.begin(KernelTimer::kRelabel);
+                k_relabel<kB><<<grid, tpb, 0, s>>>(n, p, rl, wl[p ^ 1u], nbr4, d_res, d_term, excess, height, mark,
+                                                   gid, gn, ctl);
+                kt.end();
+            }
+        };
+        if (use_graph) {
+            if (!gexec[p0].exec) {
+                hipGraph_t g = nullptr;
+                ck(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+                enqueue(sub);
+                ck(hipStreamEndCapture(s, &g), "end capture");
+                const hipError_t e = hipGraphInstantiate(&gexec[p0].exec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                ck(e, "graph instantiate");
+            }
+            ck(hipGraphLaunch(gexec[p0].exec, s), "graph launch");
+        } else {
+            enqueue(sub);
+        }
+        ck(hipGetLastError(), "discharge");
+        sub += static_cast<uint64_t>(local_steps);
+    };
+    struct Pending {
+        int slot;
+        uint64_t sub_after;
+        Event* ev;
+    };
+    Event snap_ev[2];
+    int next_slot = 0;
+    auto snapshot = [&]() -> Pending {
+        const int sl = next_slot;
+        next_slot ^= 1;
+        ck(hipMemcpyAsync(&snap[sl], ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s), "snapshot");
+        ck(hipEventRecord(snap_ev[sl].e, s), "event");
+        return Pending{sl, sub, &snap_ev[sl]};
+    };
+
+    // Small active lists (<= tail_cap) go to the one-workgroup discharge kernel.
+    const uint32_t tail_cap = prm.tail_cap == 0 ? 2048u : (prm.tail_cap < 0 ? 0u : static_cast<uint32_t>(prm.tail_cap));
+    uint64_t relabels_now = relabels_at_gr;
+    auto do_global_relabel = [&]() {
+        global_relabel();
+        active = build_active();
+        relabels_now = relabels_at_gr = hctl->relabels;
+        active_at_gr = active;
+    };
+    auto work_trigger = [&]() -> bool {
+        return alpha > 0.0 &&
+               static_cast<double>(relabels_now - relabels_at_gr) > alpha * static_cast<double>(active_at_gr) + 1024.0;
+    };
+    auto check_device_error = [&](const Ctl& c) {
+        if (c.error) {
+            std::ostringstream oss;
+            oss << kPrefix << "device invariant failed at discharge code=" << c.error;
+            fail(WTV_EINVARIANT, oss.str());
+        }
+    };
+
     const auto solve0 = Clock::now();
     while (active) {
         if (st.rounds >= max_rounds) fail(WTV_ELIMIT, std::string(kPrefix) + "max macro rounds exceeded");
-        for (int j = 0; j < local_steps; ++j, ++sub) {
-            const uint32_t p = static_cast<uint32_t>(sub & 1u);
-            kt.begin(KernelTimer::kPush);
-            k_push<kB><<<grid, tpb, 0, s>>>(n, p, wl[p], rl, nbr4, rslot, d_res, d_term, excess, height, mark, gid,
-                                            gn, gflow, ctl);
+        const double el = std::chrono::duration<double>(Clock::now() - solve0).count();
+        if (el > time_limit) fail(WTV_ELIMIT, std::string(kPrefix) + "solve exceeded 240s safety limit");
+        if (active <= tail_cap) {
+            // ---- one workgroup, sub-rounds back to back
+            unsigned long long budget = ~0ull;
+            if (alpha > 0.0) {
+                const double left = alpha * static_cast<double>(active_at_gr) + 1024.0 -
+                                    static_cast<double>(relabels_now - relabels_at_gr);
+                if (left <= 0.0) {
+                    do_global_relabel();
+                    continue;
+                }
+                budget = static_cast<unsigned long long>(left) + 1ull;
+            }
+            const uint64_t rounds_left = max_rounds - st.rounds;
+            const uint32_t max_sub = static_cast<uint32_t>(
+                std::min<uint64_t>(4096ull, rounds_left * static_cast<uint64_t>(local_steps)));
+            kt.begin(KernelTimer::kTail);
+            k_discharge_tail<kB><<<1, tpb, 0, s>>>(n, static_cast<uint32_t>(sub & 1u), tail_cap, max_sub, budget, wl[0],
+                                                   wl[1], rl, nbr4, rslot, d_res, d_term, excess, height, mark, gid, gn,
+                                                   gflow, ctl);
             kt.end();
-            kt.begin(KernelTimer::kRelabel);
-            k_relabel<kB><<<grid, tpb, 0, s>>>(n, p, rl, wl[p ^ 1u], nbr4, d_res, d_term, excess, height, mark,
-                                               gid, gn, ctl);
-            kt.end();
+            ck(hipGetLastError(), "discharge tail");
+            read_ctl("discharge tail");
+            check_device_error(*hctl);
+            const uint32_t ts = hctl->tail_sub;
+            st.tail_sub_rounds += ts;
+            const uint64_t before = st.rounds;
+            sub += ts;
+            st.rounds += std::max<uint64_t>(1ull, (ts + static_cast<uint64_t>(local_steps) - 1u) /
+                                                      static_cast<uint64_t>(local_steps));
+            relabels_now = hctl->relabels;
+            active = hctl->wl_cnt[sub & 1u];
+            const bool by_period = macro_period > 0 && st.rounds / static_cast<uint64_t>(macro_period) !=
+                                                           before / static_cast<uint64_t>(macro_period);
+            if (active && (by_period || work_trigger())) do_global_relabel();
+            continue;
         }
-        ck(hipGetLastError(), "discharge");
+        // ---- large active list: pipelined macro rounds
+        launch_round();
         ++st.rounds;
-        read_ctl("next count");
-        active = hctl->wl_cnt[sub & 1u];
-        if (hctl->error) {
-            std::ostringstream oss;
-            oss << kPrefix << "device invariant failed at discharge code=" << hctl->error;
-            fail(WTV_EINVARIANT, oss.str());
-        }
-        const bool by_period = macro_period > 0 && (st.rounds % static_cast<uint64_t>(macro_period)) == 0ull;
-        const bool by_work = alpha > 0.0 && static_cast<double>(hctl->relabels - relabels_at_gr) >
-                                                alpha * static_cast<double>(active_at_gr) + 1024.0;
-        if (active && (by_period || by_work)) {
-            global_relabel();
-            active = build_active();
-            relabels_at_gr = hctl->relabels;
-            active_at_gr = active;
-        }
-        if ((st.rounds & 255ull) == 0ull) {
-            const double el = std::chrono::duration<double>(Clock::now() - solve0).count();
-            if (el > time_limit) fail(WTV_ELIMIT, std::string(kPrefix) + "solve exceeded 240s safety limit");
+        Pending prev = snapshot();
+        for (;;) {
+            // speculative next round (a no-op on the device if prev drained the list)
+            const bool spec = st.rounds < max_rounds;
+            Pending cur{};
+            if (spec) {
+                launch_round();
+                cur = snapshot();
+            }
+            ck(hipEventSynchronize(prev.ev->e), "snapshot wait");
+            if (ktp) ktp->harvest();
+            const Ctl& S = snap[prev.slot];
+            check_device_error(S);
+            const uint32_t act_prev = S.wl_cnt[prev.sub_after & 1u];
+            relabels_now = S.relabels;
+            if (trace)
+                std::fprintf(stderr, "[wtv] round %llu: active=%u relabels=%llu\n",
+                             static_cast<unsigned long long>(st.rounds), act_prev,
+                             static_cast<unsigned long long>(S.relabels));
+            if (act_prev == 0u) {
+                if (spec) ck(hipEventSynchronize(cur.ev->e), "drain");
+                active = 0;
+                break;
+            }
+            if (!spec) fail(WTV_ELIMIT, std::string(kPrefix) + "max macro rounds exceeded");
+            ++st.rounds;
+            const bool by_period = macro_period > 0 && (st.rounds % static_cast<uint64_t>(macro_period)) == 0ull;
+            if (by_period || work_trigger()) {
+                do_global_relabel();  // runs after the speculative round
+                break;
+            }
+            if (act_prev <= tail_cap) {
+                ck(hipEventSynchronize(cur.ev->e), "switch");
+                const Ctl& C2 = snap[cur.slot];
+                check_device_error(C2);
+                active = C2.wl_cnt[cur.sub_after & 1u];
+                relabels_now = C2.relabels;
+                break;
+            }
+            prev = cur;
         }
     }
     check_err("solve completion");
@@ -870,6 +1158,13 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     st.final_relabel_seconds = std::chrono::duration<double>(Clock::now() - fr0).count();
 
     uint8_t* d_part = partition_on_device ? partition : reinterpret_cast<uint8_t*>(rl);
+    struct CallerOrder {  // the caller's stream waits for the solver's work, also on failure
+        hipStream_t cs, s;
+        hipEvent_t e;
+        ~CallerOrder() {
+            if (hipEventRecord(e, s) == hipSuccess) (void)hipStreamWaitEvent(cs, e, 0);
+        }
+    } order{cs, s, ev_done.e};
     k_partition<kB><<<sweep_grid, tpb, 0, s>>>(n, height, excess, d_part, gid, gn, ctl);
     ck(hipGetLastError(), "partition");
     ck(hipEventRecord(ev_t1.e, s), "event");
@@ -920,6 +1215,8 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     st.n_push = kt.count[KernelTimer::kPush];
     st.n_relabel = kt.count[KernelTimer::kRelabel];
     st.n_level = kt.count[KernelTimer::kLevel];
+    st.t_tail_ms = kt.ms[KernelTimer::kTail];
+    st.n_tail = kt.count[KernelTimer::kTail];
 }
 
 void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,
