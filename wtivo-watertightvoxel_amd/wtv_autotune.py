@@ -55,8 +55,9 @@ def tune(run, reference=None, space=None, start=None, reps=1, passes=1, log=None
     best_t = measure(best)
     for _ in range(passes):
         for key, values in space.items():
+            tried = best[key]
             for v in values:
-                if v == best[key]:
+                if v == tried:
                     continue
                 cand = dict(best)
                 cand[key] = v
