@@ -120,6 +120,23 @@ struct WgQueue {
     uint32_t buf[kQCap];
 };
 
+// Direct append: one global atomic per wave (used when the list being built
+// is known to stay small, so the counter sees few atomics).
+__device__ __forceinline__ void wave_append(bool pred, uint32_t val, uint32_t* list, uint32_t* cnt) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const uint32_t rank = lane_rank(m);
+    const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+    uint32_t base = 0;
+    if (pred && rank == 0) base = atom_add(cnt, static_cast<uint32_t>(__popcll(m)));
+    base = __shfl(base, leader);
+    if (pred) list[base + rank] = val;
+}
+
+// Below this many input items a kernel appends per wave, above it through the
+// LDS-staged workgroup queue (fewer barriers per iteration vs. fewer atomics).
+constexpr uint32_t kWaveAppendMax = 65536;
+
 // All active lanes of the wave must call this (pred may be false).
 __device__ __forceinline__ void q_push(WgQueue& q, bool pred, uint32_t val) {
     const uint64_t m = __ballot(pred);
@@ -327,9 +344,14 @@ __global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint
             int4 nb;
             bool claim[4];
             td_claims(valid, u, d, nbr4, rslot, res, height, nb, claim);
+            if (cnt <= kWaveAppendMax) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) q_push(q, claim[k], static_cast<uint32_t>(slot_of(nb, k)));
-            q_flush(q, fout, out_cnt);
+                for (int k = 0; k < 4; ++k) wave_append(claim[k], static_cast<uint32_t>(slot_of(nb, k)), fout, out_cnt);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) q_push(q, claim[k], static_cast<uint32_t>(slot_of(nb, k)));
+                q_flush(q, fout, out_cnt);
+            }
         }
         flush_counter(&ctl->bfs_pops, work);
     }
@@ -471,11 +493,14 @@ __device__ __forceinline__ void push_vertex(bool valid, uint32_t v, uint32_t n, 
             res[static_cast<size_t>(v) * 4u + k] = rk - dlt;
             const uint32_t back = (rs >> (2 * k)) & 3u;
             atom_add(&res[static_cast<size_t>(u) * 4u + back], dlt);
-            const long long old = atom_add(&excess[u], static_cast<long long>(dlt));
+            atom_add(&excess[u], static_cast<long long>(dlt));
             e -= dlt;
             pushed += dlt;
             ++pushes;
-            if (old == 0) cand[k] = atom_xchg(&mark[u], 1u) == 0u;
+            // every receiver joins the relabel list once (the exchange dedups it
+            // against its own entry and other pushers), so the excess add needs
+            // no returned value
+            cand[k] = atom_xchg(&mark[u], 1u) == 0u;
         }
     }
     if (pushed) atom_add(&excess[v], -pushed);
@@ -532,6 +557,7 @@ __global__ void __launch_bounds__(kTPB) k_push(uint32_t n, uint32_t p, const uin
     __shared__ WgQueue q;
     q_init(q);
     const uint32_t cnt = ctl->wl_cnt[p];
+    const bool small = cnt <= kWaveAppendMax;  // uniform across the grid
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->wl_cnt[p ^ 1u] = 0u;
     uint32_t* rl_cnt = &ctl->rl_cnt[p];
     const uint32_t stride = gridDim.x * kTPB;
@@ -545,10 +571,16 @@ __global__ void __launch_bounds__(kTPB) k_push(uint32_t n, uint32_t p, const uin
         push_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, mark, gid, gn, gflow, first, cand, nb,
                         visits, pushes, sinkf);
         // every worklist vertex is relabel-checked exactly once this sub-round
-        q_push(q, first, v);
+        if (small) {
+            wave_append(first, v, rl, rl_cnt);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) q_push(q, cand[k], static_cast<uint32_t>(slot_of(nb, k)));
-        q_flush(q, rl, rl_cnt);
+            for (int k = 0; k < 4; ++k) wave_append(cand[k], static_cast<uint32_t>(slot_of(nb, k)), rl, rl_cnt);
+        } else {
+            q_push(q, first, v);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) q_push(q, cand[k], static_cast<uint32_t>(slot_of(nb, k)));
+            q_flush(q, rl, rl_cnt);
+        }
     }
     flush_counter(&ctl->visits, visits);
     flush_counter(&ctl->pushes, pushes);
@@ -567,6 +599,7 @@ __global__ void __launch_bounds__(kTPB) k_relabel(uint32_t n, uint32_t p, const 
     __shared__ WgQueue q;
     q_init(q);
     const uint32_t cnt = ctl->rl_cnt[p];
+    const bool small = cnt <= kWaveAppendMax;
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rl_cnt[p ^ 1u] = 0u;
     uint32_t* wl_cnt = &ctl->wl_cnt[p ^ 1u];
     const uint32_t stride = gridDim.x * kTPB;
@@ -579,8 +612,12 @@ __global__ void __launch_bounds__(kTPB) k_relabel(uint32_t n, uint32_t p, const 
             v = rl[i];
             act = relabel_vertex<kB>(v, n, nbr4, res, term, excess, height, mark, gid, gn, ctl, relabels);
         }
-        q_push(q, act, v);
-        q_flush(q, wl_next, wl_cnt);
+        if (small) {
+            wave_append(act, v, wl_next, wl_cnt);
+        } else {
+            q_push(q, act, v);
+            q_flush(q, wl_next, wl_cnt);
+        }
     }
     flush_counter(&ctl->relabels, relabels);
 }
