@@ -57,8 +57,12 @@ def main():
     ap.add_argument("--batch-max", type=int, default=4_000_000, help="batched leg: largest graph (tets)")
     ap.add_argument("--serve-requests", type=int, default=48, help="serving leg: number of requests (0 = skip)")
     ap.add_argument("--serve-load", type=float, default=0.8, help="offered load vs looped-solve capacity")
+    ap.add_argument("--delaunay-points", type=int, default=400_000,
+                    help="Delaunay leg: genus-3 point-cloud size (0 = skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy three times (calibrates rocprofv3 FETCH/WRITE_SIZE)")
+    ap.add_argument("--atomic-probe", action="store_true",
+                    help="only run the atomic-add probe (calibrates rocprofv3's atomic counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); "
@@ -75,6 +79,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": 1 << 30, "copy_probe_bytes_written": 1 << 30, "launches": 3}))
+        return
+    if a.atomic_probe:
+        print(json.dumps(atomic_probe(torch)))
         return
 
     import wtv_capi as W
@@ -135,21 +142,9 @@ def main():
     raw_flow = flows[0]
 
     # -------------------------------------------------- certificate (GPU)
-    p = part.bool()
-    t64 = term0.to(torch.int64)
-    cut = int(torch.where(~p & (t64 < 0), -t64, 0).sum() + torch.where(p & (t64 > 0), t64, 0).sum())
-    nb64 = nbr.to(torch.int64)
-    ok = nb64 >= 0
-    pj = torch.where(ok, p[torch.where(ok, nb64, 0)], False)
-    cutm = ok & (~p)[:, None] & pj
-    r64 = res0.to(torch.int64) & 0xFFFFFFFF
-    cut += int(torch.where(cutm, r64, 0).sum())
+    cut, energy = cut_capacity(torch, nbr, res0, term0, part, g["cap_f32"], g["term_f32"])
     assert cut == raw_flow, f"cut capacity {cut} != flow {raw_flow}"
-    tf = g["term_f32"].double()
-    energy = float(torch.where(~p & (tf < 0), -tf, 0).sum() + torch.where(p & (tf > 0), tf, 0).sum()
-                   + torch.where(cutm, g["cap_f32"].double(), 0).sum())
     part_h = part.cpu().numpy()
-    del pj, cutm, r64, nb64, t64, tf
 
     # -------------------------------------------------- profiled solve
     prof = None
@@ -213,6 +208,27 @@ def main():
                          f"1 thread; flow and labels equal to the GPU's",
                "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
 
+    atomics = None
+    if prof:
+        probe = atomic_probe(torch)
+        am = (3 * prof["pushes"] + 2 * prof["visits"]) / max(prof["n_push"], 1)
+        push_rate = am / (prof["t_push_ms"] / max(prof["n_push"], 1) * 1e-3)
+        counters = None
+        ap_ = os.path.join(ROOT, "profiles", "atomics.json")
+        if os.path.exists(ap_):
+            try:
+                counters = json.load(open(ap_))
+            except Exception:
+                counters = None
+        atomics = {"probe": probe,
+                   "k_push": {"atomic_ops_per_launch_model": int(am), "atomic_ops_per_s": round(push_rate, 1),
+                              "frac_of_probe_u32_random": round(push_rate / probe["u32_random_atomics_per_s"], 4)},
+                   "counters": counters}
+
+    delaunay = None
+    if a.delaunay_points > 0:
+        delaunay = delaunay_leg(a, torch, np, W, S, dev, stream)
+
     batched = None
     if a.batch_graphs > 0:
         del res, term, part
@@ -253,11 +269,114 @@ def main():
         "roofline": roofline,
         "kernels_ms": {kk: round(vv[0], 3) for kk, vv in kernels.items()} if kernels else None,
         "cpu_baseline": cpu,
+        "atomics": atomics,
+        "delaunay": delaunay,
         "batched": batched,
         "serving": serving,
         "gen_seconds": round(gen_s, 2),
     }
     print(json.dumps(line), flush=True)
+
+
+def cut_capacity(torch, nbr, res0, term0, part, cap_f32=None, term_f32=None):
+    """Integer capacity of the cut `part` (1 = sink side) on the pristine graph,
+    and its fp64 energy from the fp32 capacities when given."""
+    p = part.bool()
+    t64 = term0.to(torch.int64)
+    cut = int(torch.where(~p & (t64 < 0), -t64, 0).sum() + torch.where(p & (t64 > 0), t64, 0).sum())
+    nb64 = nbr.to(torch.int64)
+    ok = nb64 >= 0
+    pj = torch.where(ok, p[torch.where(ok, nb64, 0)], False)
+    cutm = ok & (~p)[:, None] & pj
+    r64 = res0.to(torch.int64) & 0xFFFFFFFF
+    cut += int(torch.where(cutm, r64, 0).sum())
+    energy = None
+    if cap_f32 is not None:
+        tf = term_f32.double()
+        energy = float(torch.where(~p & (tf < 0), -tf, 0).sum() + torch.where(p & (tf > 0), tf, 0).sum()
+                       + torch.where(cutm, cap_f32.double(), 0).sum())
+    return cut, energy
+
+
+def atomic_probe(torch, n=1 << 26, reps=3):
+    """Throughput of plain device-scope atomic adds on distinct addresses
+    (torch index_add_ on int32 and int64, indices coalesced and randomly
+    permuted): the attainable atomic rate the solver's atomics are compared with."""
+    out = {"elements": n, "launches_each": reps}
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    idx_seq = torch.arange(n, dtype=torch.int64, device="cuda")
+    idx_rnd = torch.randperm(n, device="cuda", generator=g)
+    for dt, name in ((torch.int32, "u32"), (torch.int64, "i64")):
+        dst = torch.zeros(n, dtype=dt, device="cuda")
+        src = torch.ones(n, dtype=dt, device="cuda")
+        for kind, idx in (("coalesced", idx_seq), ("random", idx_rnd)):
+            dst.index_add_(0, idx, src)  # warm
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(reps):
+                dst.index_add_(0, idx, src)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            out[f"{name}_{kind}_atomics_per_s"] = round(n / (ms * 1e-3), 1)
+        assert int(dst[:16].min()) == 2 * reps + 2  # every address got every add
+    return out
+
+
+def delaunay_leg(a, torch, np, W, S, dev, stream):
+    """Delaunay mesh of a genus-3 point cloud (95% on the fused surface with
+    half-voxel normal noise, 5% uniform), soft-seed capacities, lambda 10:
+    the production mesh family at a size scipy's Qhull builds in bench time."""
+    t = time.time()
+    g, mesh, meta = S.delaunay_genus3_graph(a.delaunay_points, a.seed + 5, device=dev)
+    del mesh
+    nbr, res0, term0 = g["nbr"], g["res"], g["term"]
+    N = int(meta["tets"])
+    res, term = res0.clone(), term0.clone()
+    part = torch.empty(N, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    gen_s = time.time() - t
+    prm = W.params(global_relabel_period=a.period, local_steps=a.local_steps, relabel_alpha=a.alpha)
+
+    def step():
+        res.copy_(res0)
+        term.copy_(term0)
+        return W.pr_solve(N, nbr.data_ptr(), res.data_ptr(), term.data_ptr(), part.data_ptr(), 1, prm, stream)
+
+    step()
+    torch.cuda.synchronize()
+    steps = max(a.steps, 3)
+    t0 = time.perf_counter()
+    flows = [step().raw_flow for _ in range(steps)]
+    torch.cuda.synchronize()
+    t_s = (time.perf_counter() - t0) / steps
+    assert len(set(flows)) == 1
+    st = step().as_dict()
+    cut, energy = cut_capacity(torch, nbr, res0, term0, part, g["cap_f32"], g["term_f32"])
+    assert cut == flows[0], f"Delaunay leg: cut capacity {cut} != flow {flows[0]}"
+    out = {"workload": f"Delaunay (scipy Qhull) of {a.delaunay_points} genus-3 points, 95% surface + 5% uniform, "
+                       "Morton-ordered tets, soft seeds, lambda 10, 2% flips",
+           "points": a.delaunay_points, "tets": N, "voxel_res": round(meta["res"], 2),
+           "inside_frac": round(meta["inside_frac"], 4), "s_per_solve": round(t_s, 5),
+           "tets_per_s": round(N / t_s, 1), "raw_flow": flows[0], "cut_energy_fp64": energy,
+           "certificate": "flow == integer cut capacity",
+           "work": {k2: st[k2] for k2 in ("rounds", "sub_rounds", "global_relabels", "bfs_levels",
+                                          "tail_sub_rounds")},
+           "gen_seconds": round(gen_s, 1)}
+    if not a.no_cpu:
+        import oracle
+        nbr_h = nbr.cpu().numpy()
+        tc = time.perf_counter()
+        b = oracle.bk_solve(nbr_h, res0.cpu().numpy().view(np.uint32), term0.cpu().numpy())
+        cpu_s = time.perf_counter() - tc
+        assert b.raw_flow == flows[0] and np.array_equal(b.partition, part.cpu().numpy()), \
+            "Delaunay leg: CPU oracle differs"
+        out["cpu_bk_s"] = round(cpu_s, 3)
+        out["identical_to_cpu"] = True
+    log(f"[bench] delaunay: {N} tets, {t_s * 1e3:.2f} ms/solve, gen {gen_s:.1f}s")
+    return out
 
 
 def batch_leg(a, torch, np, W, S, dev, stream):

@@ -221,6 +221,20 @@ def test_larger_graph_certificate_and_bk(orc, torch_cuda):
     assert b.raw_flow == st["raw_flow"] and np.array_equal(b.partition, part)
 
 
+@pytest.mark.parametrize("npts,morton", [(60_000, True), (60_000, False), (150_000, True)])
+def test_delaunay_genus3_vs_bk(orc, torch_cuda, npts, morton):
+    """Delaunay mesh of a genus-3 point cloud (95% on the surface with half-voxel
+    noise, 5% uniform), soft-seed capacities, lambda 10, 2% flips: GPU flow and
+    labels equal BK's, and the GPU's final residuals pass the certificate."""
+    P, shape, res = S.genus3_point_cloud(npts, 40 + npts // 1000)
+    m = S.delaunay_mesh(P, morton=morton)
+    lab = S.labels_from_sdf(S.tet_centroids_sdf(m, shape), 0.02, seed=npts)
+    g = S.soft_graph(m, lab, shape, 10.0, 1.0 / res)
+    gpu = gpu_solve(torch_cuda, g.nbr, g.res, g.term, device_partition=True)
+    assert_same(orc, g.nbr, g.res, g.term, gpu)
+    assert S.quantised_cut(g.nbr, g.res, g.term, gpu[0]) == gpu[1]["raw_flow"]
+
+
 # ------------------------------------------------------------ batched mode
 def _batch(graphs):
     off = np.zeros(len(graphs) + 1, np.uint64)

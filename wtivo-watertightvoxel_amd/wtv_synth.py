@@ -556,3 +556,121 @@ def batch_workload(count: int, seed: int, n_min: int = 50_000, n_max: int = 4_00
         out.append({"k": k, "N": N, "lam": lam, "nbr": g["nbr"], "res": g["res"], "term": g["term"]})
         del V, T, Nb, sdf, lab, g
     return out
+
+
+# --------------------------------------------------------------------------
+# Delaunay meshes of a genus-3 point cloud (the production-shape mesh family)
+# --------------------------------------------------------------------------
+def _torus_frame(t: Torus):
+    a = np.asarray(t.ax, np.float64)
+    a = a / np.linalg.norm(a)
+    u = np.cross(a, [1.0, 0.0, 0.0] if abs(a[0]) < 0.9 else [0.0, 1.0, 0.0])
+    u /= np.linalg.norm(u)
+    return a, u, np.cross(a, u)
+
+
+def torus_surface_points(t: Torus, m: int, rng: np.random.Generator) -> np.ndarray:
+    """m points uniform by area on a torus surface.  The angle around the tube,
+    phi, has density proportional to R + r cos(phi) (rejection sampling)."""
+    a, u, w = _torus_frame(t)
+    phis = np.empty(0)
+    while phis.size < m:
+        cand = rng.uniform(0.0, 2.0 * np.pi, size=2 * (m - phis.size) + 64)
+        keep = rng.uniform(size=cand.size) * (t.R + t.r) < t.R + t.r * np.cos(cand)
+        phis = np.concatenate([phis, cand[keep]])
+    phi = phis[:m]
+    th = rng.uniform(0.0, 2.0 * np.pi, size=m)
+    rad = t.R + t.r * np.cos(phi)
+    return (np.asarray(t.c, np.float64)[None, :] + (rad * np.cos(th))[:, None] * u[None, :]
+            + (rad * np.sin(th))[:, None] * w[None, :] + (t.r * np.sin(phi))[:, None] * a[None, :])
+
+
+def genus3_point_cloud(npts: int, seed: int, res: float | None = None, surface_frac: float = 0.95):
+    """Point cloud of the genus-3 solid (three fused tori, genus3(seed)):
+    surface_frac of the points uniform by area on the fused surface (each torus's
+    surface minus the parts inside another torus) with N(0, 0.5 voxel) normal
+    noise, the rest uniform in the unit box.  The voxel is 1/res; res defaults to
+    1536 * sqrt(npts / 12e6), which keeps the noise-to-spacing ratio of 12M points
+    at resolution 1536.  Returns (points float64 [npts,3], shape, res)."""
+    shape = genus3(seed)
+    if res is None:
+        res = 1536.0 * math.sqrt(npts / 12e6)
+    rng = np.random.Generator(np.random.PCG64(seed + 11))
+    ns = int(round(surface_frac * npts))
+    tori = shape.parts
+    got, need = [], ns
+    while need > 0:
+        m = int(need * 1.5) + 256
+        which = rng.integers(0, len(tori), size=m)
+        P = np.empty((m, 3))
+        for i, t in enumerate(tori):
+            sel = which == i
+            P[sel] = torus_surface_points(t, int(sel.sum()), rng)
+        pt = torch.from_numpy(P)
+        outside_others = torch.ones(m, dtype=torch.bool)
+        for i, t in enumerate(tori):
+            d = t.sdf(pt)
+            outside_others &= torch.from_numpy(which == i) | (d >= 0)
+        kept = P[outside_others.numpy()][:need]
+        got.append(kept)
+        need -= kept.shape[0]
+    S = np.concatenate(got)
+    S += rng.normal(0.0, 0.5 / res, size=S.shape)
+    U = rng.uniform(0.0, 1.0, size=(npts - ns, 3))
+    return np.concatenate([S, U]), shape, res
+
+
+def _morton_key(P: np.ndarray, bits: int = 21) -> np.ndarray:
+    lo = P.min(0)
+    span = max(float((P.max(0) - lo).max()), 1e-300)
+    q = np.clip(((P - lo) / span * ((1 << bits) - 1)).astype(np.int64), 0, (1 << bits) - 1)
+    key = np.zeros(P.shape[0], np.int64)
+    for b in range(bits):
+        for ax in range(3):
+            key |= ((q[:, ax] >> b) & 1) << (3 * b + ax)
+    return key
+
+
+def delaunay_mesh(points: np.ndarray, morton: bool = True) -> TetMesh:
+    """3D Delaunay tetrahedralisation (scipy's Qhull) in the 4-slot layout:
+    slot s holds the neighbour across the face opposite local vertex 3-s, -1 on
+    the hull (Qhull's neighbours[:, j] is the one opposite vertex j).  With
+    morton=True the tets are renumbered in Morton order of their centroids, the
+    spatially coherent order a mesher that inserts points along a space-filling
+    curve hands over; the solver itself takes any order."""
+    from scipy.spatial import Delaunay
+
+    d = Delaunay(points)
+    tets = np.ascontiguousarray(d.simplices, dtype=np.int64)
+    nbrs = np.ascontiguousarray(d.neighbors[:, ::-1], dtype=np.int64)
+    if morton:
+        order = np.argsort(_morton_key(points[tets].mean(axis=1)), kind="stable")
+        inv = np.empty_like(order)
+        inv[order] = np.arange(order.size)
+        tets = tets[order]
+        nbrs = nbrs[order]
+        nbrs = np.where(nbrs >= 0, inv[np.maximum(nbrs, 0)], -1)
+    return TetMesh(0, np.ascontiguousarray(points, dtype=np.float64), tets.astype(np.int32),
+                   nbrs.astype(np.int32))
+
+
+def delaunay_genus3_graph(npts: int, seed: int, lam: float = 10.0, flip: float = 0.02, device=None,
+                          res: float | None = None):
+    """Soft-seed graph of the Delaunay mesh of genus3_point_cloud(npts, seed):
+    labels from the shape SDF at tet centroids with `flip` seeded flips,
+    terminal lam * volume, facet area * exp(-|UDF|/h) with h one voxel (1/res).
+    Returns (graph dict as soft_graph_t, mesh, meta)."""
+    dev = _dev(device)
+    P, shape, res = genus3_point_cloud(npts, seed, res)
+    mesh = delaunay_mesh(P)
+    V = torch.from_numpy(mesh.V).to(dev)
+    T = torch.from_numpy(mesh.tets).to(dev)
+    Nb = torch.from_numpy(mesh.nbrs).to(dev)
+    sdf = tet_sdf_t(V, T, shape)
+    lab = (sdf >= 0).to(torch.uint8)
+    if flip > 0:
+        lab ^= torch.from_numpy(flip_mask(mesh.N, flip, seed + 1).astype(np.uint8)).to(dev)
+    g = soft_graph_t(V, T, Nb, lab, shape, lam, 1.0 / res)
+    meta = {"points": npts, "tets": mesh.N, "res": res, "h": 1.0 / res, "lam": lam, "flip": flip,
+            "inside_frac": float((lab == 0).float().mean())}
+    return g, mesh, meta
