@@ -155,14 +155,18 @@ def main():
         prof = pst
     s0 = stats[len(stats) // 2]
     n = N
-    # SURVEY byte model; a bottom-up BFS level costs a 4 B height sweep over all
-    # n vertices plus 48 B (nbr 16 + res 16 + 4 neighbour heights 16) per
-    # unreached vertex it examines.
+    # Byte model of the k_bfs_level launches: 61 B per top-down frontier
+    # expansion; a bottom-up level costs a 4 B height sweep over all n vertices
+    # plus 48 B (nbr 16 + res 16 + 4 neighbour heights 16) per unreached vertex
+    # it examines.
     def bfs_bytes(x):
         return 61 * x["bfs_pops"] + 4 * n * x["bfs_bu_levels"] + 48 * x["bfs_bu_checks"]
 
+    # Whole solve: discharge + BFS levels + per relabel the level-1/2 sweep
+    # (term read 4 + height write 4 per vertex, 48 B per term >= 0 vertex
+    # examined) + 2 B/vertex for the partition.
     bytes_alg = (69 * s0["visits"] + 32 * s0["pushes"] + 4 * s0["relabels"] + bfs_bytes(s0)
-                 + 20 * n * s0["global_relabels"] + 2 * n)
+                 + 8 * n * s0["global_relabels"] + 48 * s0["bfs_seed_checks"] + 2 * n)
     t_solve_s = s0["t_solve_ms"] / 1e3
     roofline = None
     kernels = {}
@@ -263,7 +267,7 @@ def main():
         "certificate": "flow == integer cut capacity",
         "work": {k2: s0[k2] for k2 in ("rounds", "sub_rounds", "global_relabels", "visits", "pushes",
                                         "relabels", "bfs_pops", "bfs_levels", "bfs_bu_levels",
-                                        "bfs_bu_checks", "tail_sub_rounds")},
+                                        "bfs_bu_checks", "bfs_seed_checks", "tail_sub_rounds")},
         "solve_roofline": {"bytes_alg": bytes_alg, "achieved_GBs": round(bytes_alg / t_solve_s / 1e9, 2),
                            "frac": round(bytes_alg / t_solve_s / 1e9 / HBM_PEAK_GBS, 5)},
         "roofline": roofline,

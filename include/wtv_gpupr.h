@@ -75,6 +75,8 @@ typedef struct wtv_stats {
     double t_tail_ms;        /* one-workgroup discharge launches (profile)   */
     uint64_t n_tail;
     uint64_t tail_sub_rounds; /* sub-rounds run by the one-workgroup kernel  */
+    uint64_t bfs_seed_checks; /* term >= 0 vertices examined by the sweep that
+                                 expands BFS levels 1 and 2, all relabels    */
 } wtv_stats;
 
 /* Solver knobs.  global_relabel_period, max_rounds and local_steps keep the

@@ -36,7 +36,7 @@ def test_params_defaults_match_reference():
 
 def test_struct_sizes_match_c_layout():
     # 8 doubles + 17 uint64/int64 in wtv_stats
-    assert C.sizeof(W.WtvStats) == 25 * 8
+    assert C.sizeof(W.WtvStats) == 26 * 8
     assert C.sizeof(W.WtvCutResult) == 17 * 8
 
 

@@ -180,6 +180,7 @@ py::dict stats_dict(const wtv_stats& s) {
     d["t_tail_ms"] = s.t_tail_ms;
     d["n_tail"] = s.n_tail;
     d["tail_sub_rounds"] = s.tail_sub_rounds;
+    d["bfs_seed_checks"] = s.bfs_seed_checks;
     return d;
 }
 

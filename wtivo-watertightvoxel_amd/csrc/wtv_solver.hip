@@ -31,7 +31,11 @@
 // Global relabel = level-synchronous BFS from the sink over residual arcs
 // (gpu_push_relabel_fast.cu:85-147,360-402), with reverse slots precomputed
 // once per solve into one byte per vertex (2 bits per slot) instead of the
-// reference's per-arc 4-slot search (gpu_push_relabel_fast.cu:56-67).
+// reference's per-arc 4-slot search (gpu_push_relabel_fast.cu:56-67).  Levels
+// 1 and 2 come from one sweep over the terminals; a global relabel never
+// lowers a height, so the active list after it is the previous list filtered
+// by the new heights rather than a sweep over all vertices.  Vertices the BFS
+// does not reach keep a sentinel height that every reader takes as n+1.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -51,6 +55,18 @@ constexpr uint32_t kSent = 0xFFFFFFFFu;
 constexpr int kTPB = 256;
 
 // Device control block: worklist counters, flow, work counters, error word.
+enum : int {
+    kStVisits,
+    kStPushes,
+    kStRelabels,
+    kStBfsPops,
+    kStSinkFlow,
+    kStBuChecks,
+    kStSeedChecks,  // vertices with term >= 0 examined by k_bfs_seed
+    kNumStats
+};
+constexpr int kStatSlots = 64;
+
 struct Ctl {
     uint32_t wl_cnt[2];   // active lists W[0], W[1]
     uint32_t rl_cnt[2];   // relabel list counts (ping-pong)
@@ -58,12 +74,21 @@ struct Ctl {
     uint32_t bfs_levels;
     uint32_t tail_d;      // level reached by the one-workgroup BFS tail kernel
     uint32_t tail_sub;    // sub-rounds run by the one-workgroup discharge kernel
-    unsigned long long sink_flow;
-    unsigned long long visits, pushes, relabels, bfs_pops;
-    unsigned long long bu_levels, bu_checks;
+    unsigned long long bu_levels;
     int32_t error;
     uint32_t violations;
+    // Work counters and the sink flow, each spread over kStatSlots addresses:
+    // device-scope atomics on one address serialise at ~12 ns each on MI355X
+    // (about 83M/s, returning or not), so one counter per wave of a large
+    // launch would cost tens of microseconds.  Readers sum the slots.
+    unsigned long long stat[kNumStats][kStatSlots];
 };
+
+inline unsigned long long stat_sum(const Ctl& c, int f) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kStatSlots; ++i) t += c.stat[f][i];
+    return t;
+}
 
 // ---------------------------------------------------------------- atomics
 __device__ __forceinline__ uint32_t atom_add(uint32_t* p, uint32_t v) {
@@ -101,41 +126,31 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
     return x;
 }
 
-// Sum per-lane counters over the wave and add them once per wave.
-__device__ __forceinline__ void flush_counter(unsigned long long* dst, unsigned long long x) {
+// Sum per-lane counters over the wave and add them once per wave, to one of
+// kStatSlots addresses picked by the wave's index in the grid.
+__device__ __forceinline__ void flush_stat(Ctl* ctl, int f, unsigned long long x) {
     x = wave_sum(x);
-    if (__lane_id() == 0 && x) atom_add(dst, x);
+    if (__lane_id() == 0 && x) {
+        const uint32_t w = blockIdx.x * (kTPB / 64u) + (threadIdx.x >> 6);
+        atom_add(&ctl->stat[f][w % kStatSlots], x);
+    }
 }
 
 // ------------------------------------------- LDS-staged workgroup queue
 // Items are appended to an LDS buffer (one LDS atomic per wave, ranks from
 // ballot + mbcnt), then the workgroup reserves space in the global list with
-// a single device atomic and copies the items out coalesced.  A single global
-// counter sustains only ~1e8 atomics/s, so this is what keeps frontier and
-// worklist appends off the critical path.
-constexpr int kQCap = kTPB * 5;  // max appends per thread per iteration: 5
+// a single device atomic and copies the items out coalesced.  The queue is
+// flushed lazily -- when the next iteration could overflow it, and at kernel
+// end -- because device-scope atomics on one address serialise at ~12 ns each
+// (~83M/s) on MI355X: one flush per iteration of a 78M-vertex sweep would
+// cost milliseconds.  16 KiB of LDS per workgroup.
+constexpr int kQCap = 4096;
+static_assert(kQCap >= 5 * kTPB, "one iteration appends at most 5 items per thread");
 struct WgQueue {
     uint32_t cnt;
     uint32_t base;
     uint32_t buf[kQCap];
 };
-
-// Direct append: one global atomic per wave (used when the list being built
-// is known to stay small, so the counter sees few atomics).
-__device__ __forceinline__ void wave_append(bool pred, uint32_t val, uint32_t* list, uint32_t* cnt) {
-    const uint64_t m = __ballot(pred);
-    if (m == 0) return;
-    const uint32_t rank = lane_rank(m);
-    const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
-    uint32_t base = 0;
-    if (pred && rank == 0) base = atom_add(cnt, static_cast<uint32_t>(__popcll(m)));
-    base = __shfl(base, leader);
-    if (pred) list[base + rank] = val;
-}
-
-// Below this many input items a kernel appends per wave, above it through the
-// LDS-staged workgroup queue (fewer barriers per iteration vs. fewer atomics).
-constexpr uint32_t kWaveAppendMax = 65536;
 
 // All active lanes of the wave must call this (pred may be false).
 __device__ __forceinline__ void q_push(WgQueue& q, bool pred, uint32_t val) {
@@ -164,6 +179,26 @@ __device__ __forceinline__ void q_flush(WgQueue& q, uint32_t* list, uint32_t* gc
     __syncthreads();
 }
 
+// Flushes only when fewer than `room` slots are free, i.e. when the next
+// iteration (at most `room` appends) could overflow.  Uniform: every thread
+// reads the count between two barriers.  Every thread of the workgroup must
+// call this the same number of times, and call q_flush once at the end.
+__device__ __forceinline__ void q_flush_if(WgQueue& q, uint32_t* list, uint32_t* gcnt, uint32_t room) {
+    __syncthreads();
+    const uint32_t c = q.cnt;
+    __syncthreads();
+    if (c + room > static_cast<uint32_t>(kQCap)) {
+        if (threadIdx.x == 0) {
+            q.base = atom_add(gcnt, c);
+            q.cnt = 0u;
+        }
+        __syncthreads();
+        const uint32_t b = q.base;
+        for (uint32_t i = threadIdx.x; i < c; i += kTPB) list[b + i] = q.buf[i];
+        __syncthreads();
+    }
+}
+
 __device__ __forceinline__ void q_init(WgQueue& q) {
     if (threadIdx.x == 0) q.cnt = 0u;
     __syncthreads();
@@ -186,7 +221,6 @@ __device__ __forceinline__ uint32_t src_h(uint32_t v, uint32_t n, const uint32_t
 }
 
 // ---------------------------------------------------------------- kernels
-__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
 // Batched mode: graph g occupies nodes [off[g], off[g+1]) of the union and
 // its neighbour ids are local to g.  Writes union-wide ids and the graph id.
@@ -221,43 +255,112 @@ __global__ void __launch_bounds__(kTPB) k_globalize(uint32_t n, uint32_t G, cons
 __global__ void __launch_bounds__(kTPB) k_prepare(uint32_t n, const int4* __restrict__ nbr4,
                                                   const int32_t* __restrict__ term, uint8_t* __restrict__ rslot,
                                                   long long* __restrict__ excess, uint32_t* __restrict__ mark,
-                                                  Ctl* ctl) {
+                                                  uint32_t* __restrict__ wl0, Ctl* ctl) {
+    __shared__ WgQueue q;
+    q_init(q);
     const uint32_t stride = gridDim.x * kTPB;
-    for (uint32_t v = blockIdx.x * kTPB + threadIdx.x; v < n; v += stride) {
-        const int4 nb = nbr4[v];
-        uint32_t code = 0;
-        int err = 0;
+    // uniform trip count per workgroup: every thread reaches each queue flush
+    for (uint32_t base = blockIdx.x * kTPB; base < n; base += stride) {
+        const uint32_t v = base + threadIdx.x;
+        bool cand = false;
+        if (v < n) {
+            const int4 nb = nbr4[v];
+            uint32_t code = 0;
+            int err = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int u = slot_of(nb, k);
-            if (u < 0) continue;
-            if (static_cast<uint32_t>(u) >= n) { err = 3; continue; }
-            const int4 nu = nbr4[u];
-            const int iv = static_cast<int>(v);
-            const int j = nu.x == iv ? 0 : nu.y == iv ? 1 : nu.z == iv ? 2 : nu.w == iv ? 3 : -1;
-            if (j < 0) { err = 2; continue; }
-            code |= static_cast<uint32_t>(j) << (2 * k);
+            for (int k = 0; k < 4; ++k) {
+                const int u = slot_of(nb, k);
+                if (u < 0) continue;
+                if (static_cast<uint32_t>(u) >= n) { err = 3; continue; }
+                const int4 nu = nbr4[u];
+                const int iv = static_cast<int>(v);
+                const int j = nu.x == iv ? 0 : nu.y == iv ? 1 : nu.z == iv ? 2 : nu.w == iv ? 3 : -1;
+                if (j < 0) { err = 2; continue; }
+                code |= static_cast<uint32_t>(j) << (2 * k);
+            }
+            if (err) set_error(ctl, err);
+            rslot[v] = static_cast<uint8_t>(code);
+            const int32_t t = term[v];
+            excess[v] = t > 0 ? static_cast<long long>(t) : 0LL;
+            mark[v] = 0u;
+            cand = t > 0;  // every vertex with excess joins the first candidate list
         }
-        if (err) set_error(ctl, err);
-        rslot[v] = static_cast<uint8_t>(code);
-        const int32_t t = term[v];
-        excess[v] = t > 0 ? static_cast<long long>(t) : 0LL;
-        mark[v] = 0u;
+        q_push(q, cand, v);
+        q_flush_if(q, wl0, &ctl->wl_cnt[0], kTPB);
     }
+    q_flush(q, wl0, &ctl->wl_cnt[0]);
 }
 
 // BFS seeds: vertices with remaining sink capacity get height 1
 // (gpu_push_relabel_fast.cu:85-101); every other vertex starts unreached.
 // No seed list is built: level 1 always expands bottom-up.
+__global__ void k_bfs_begin(Ctl* ctl, uint32_t p) {
+    ctl->bfs_cnt[0] = 0u;
+    ctl->bfs_cnt[1] = 0u;
+    ctl->bfs_cnt[2] = 0u;
+    ctl->wl_cnt[p ^ 1u] = 0u;  // receives the filtered active list
+    ctl->rl_cnt[0] = 0u;       // the next push phase starts an empty relabel list
+    ctl->rl_cnt[1] = 0u;
+    ctl->bfs_levels += 1u;     // level 1 is expanded by k_bfs_seed
+}
+
+// BFS start (gpu_push_relabel_fast.cu:90-139, levels 1 and 2 in one sweep).
+// Level 1 is every vertex with residual sink capacity (term < 0).  Level 2 is
+// found bottom-up in the same pass: a vertex with term >= 0 and a residual arc
+// into a term < 0 neighbour.  Testing the neighbour's terminal instead of its
+// height removes the dependency on the level-1 writes, so one sweep replaces a
+// seed sweep plus a bottom-up level.  Each thread takes kSeedUnroll vertices
+// per iteration (coalesced, one stride of kTPB apart), so the workgroup queue
+// flushes once per kSeedUnroll * kTPB vertices.  Level 2 goes to `fout` with
+// its count in bfs_cnt[2] (zeroed by k_bfs_begin).
+constexpr int kSeedUnroll = 4;
+static_assert(kSeedUnroll * kTPB <= kQCap, "queue holds one append per vertex per iteration");
 __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __restrict__ term,
-                                                   uint32_t* __restrict__ height, Ctl* ctl) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->bfs_cnt[1] = 0u;
-        ctl->bfs_cnt[2] = 0u;
+                                                   const int4* __restrict__ nbr4, const uint32_t* __restrict__ res,
+                                                   uint32_t* __restrict__ height, uint32_t* __restrict__ fout,
+                                                   Ctl* ctl) {
+    __shared__ WgQueue q;
+    q_init(q);
+    const uint32_t span = kTPB * kSeedUnroll;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * span;
+    unsigned long long work = 0;
+    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * span; base < n; base += stride) {
+        uint32_t v[kSeedUnroll];
+        int32_t t[kSeedUnroll];
+#pragma unroll
+        for (int j = 0; j < kSeedUnroll; ++j) {
+            const uint64_t x = base + static_cast<uint64_t>(j) * kTPB + threadIdx.x;
+            v[j] = static_cast<uint32_t>(x);
+            t[j] = x < n ? term[x] : -1;
+        }
+        bool found[kSeedUnroll];
+#pragma unroll
+        for (int j = 0; j < kSeedUnroll; ++j) {
+            found[j] = false;
+            if (static_cast<uint64_t>(base) + static_cast<uint64_t>(j) * kTPB + threadIdx.x < n && t[j] >= 0) {
+                ++work;
+                const int4 nb = nbr4[v[j]];
+                const uint4 r = reinterpret_cast<const uint4*>(res)[v[j]];
+                int32_t tu[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int u = slot_of(nb, k);
+                    tu[k] = (u >= 0 && slot_of(r, k) != 0u) ? term[u] : 0;
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) found[j] |= tu[k] < 0;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kSeedUnroll; ++j) {
+            if (static_cast<uint64_t>(base) + static_cast<uint64_t>(j) * kTPB + threadIdx.x < n)
+                height[v[j]] = t[j] < 0 ? 1u : (found[j] ? 2u : kSent);
+            q_push(q, found[j], v[j]);
+        }
+        q_flush_if(q, fout, &ctl->bfs_cnt[2], kSeedUnroll * kTPB);
     }
-    const uint32_t stride = gridDim.x * kTPB;
-    for (uint32_t v = blockIdx.x * kTPB + threadIdx.x; v < n; v += stride)
-        height[v] = term[v] < 0 ? 1u : kSent;
+    q_flush(q, fout, &ctl->bfs_cnt[2]);
+    flush_stat(ctl, kStSeedChecks, work);
 }
 
 // Top-down claims of frontier vertex u (valid lanes only): each unreached
@@ -291,11 +394,11 @@ __device__ __forceinline__ void td_claims(bool valid, uint32_t u, uint32_t d, co
     }
 }
 
-// One BFS level from height d to d+1 (gpu_push_relabel_fast.cu:103-139).
-// Top-down over the frontier list when it is small; bottom-up over all
-// vertices when the frontier is large (or for level 1, which has no list):
-// each unreached v checks its own arcs v->u for a residual arc into level d,
-// needing neither the reverse slot nor a CAS.
+// One BFS level from height d to d+1 (gpu_push_relabel_fast.cu:103-139),
+// d >= 2.  Top-down over the frontier list when it is small; bottom-up over
+// all vertices when the frontier is large: each unreached v checks its own
+// arcs v->u for a residual arc into level d, needing neither the reverse slot
+// nor a CAS.
 __global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint32_t bu_threshold,
                                                     const uint32_t* __restrict__ fin, uint32_t* __restrict__ fout,
                                                     const int4* __restrict__ nbr4, const uint8_t* __restrict__ rslot,
@@ -304,16 +407,15 @@ __global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint
     q_init(q);
     const uint32_t cnt = ctl->bfs_cnt[d % 3u];
     uint32_t* out_cnt = &ctl->bfs_cnt[(d + 1u) % 3u];
-    const bool bottom_up = d == 1u || cnt > bu_threshold;
+    const bool bottom_up = cnt > bu_threshold;  // bu_threshold >= 4096, so cnt > 0
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->bfs_cnt[(d + 2u) % 3u] = 0u;
-        if (cnt || d == 1u) ctl->bfs_levels += 1u;
-        if (bottom_up && (cnt || d == 1u)) ctl->bu_levels += 1u;
+        if (cnt) ctl->bfs_levels += 1u;
+        if (bottom_up) ctl->bu_levels += 1u;
     }
     const uint32_t stride = gridDim.x * kTPB;
     unsigned long long work = 0;
     if (bottom_up) {
-        if (d != 1u && cnt == 0u) return;
         for (uint32_t base = blockIdx.x * kTPB; base < n; base += stride) {
             const uint32_t v = base + threadIdx.x;
             bool found = false;
@@ -332,9 +434,10 @@ __global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint
                 if (found) height[v] = d + 1u;
             }
             q_push(q, found, v);
-            q_flush(q, fout, out_cnt);
+            q_flush_if(q, fout, out_cnt, kTPB);
         }
-        flush_counter(&ctl->bu_checks, work);
+        q_flush(q, fout, out_cnt);
+        flush_stat(ctl, kStBuChecks, work);
     } else {
         for (uint32_t base = blockIdx.x * kTPB; base < cnt; base += stride) {
             const uint32_t i = base + threadIdx.x;
@@ -344,16 +447,12 @@ __global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint
             int4 nb;
             bool claim[4];
             td_claims(valid, u, d, nbr4, rslot, res, height, nb, claim);
-            if (cnt <= kWaveAppendMax) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) wave_append(claim[k], static_cast<uint32_t>(slot_of(nb, k)), fout, out_cnt);
-            } else {
-#pragma unroll
-                for (int k = 0; k < 4; ++k) q_push(q, claim[k], static_cast<uint32_t>(slot_of(nb, k)));
-                q_flush(q, fout, out_cnt);
-            }
+            for (int k = 0; k < 4; ++k) q_push(q, claim[k], static_cast<uint32_t>(slot_of(nb, k)));
+            q_flush_if(q, fout, out_cnt, 4 * kTPB);
         }
-        flush_counter(&ctl->bfs_pops, work);
+        q_flush(q, fout, out_cnt);
+        flush_stat(ctl, kStBfsPops, work);
     }
 }
 
@@ -402,7 +501,7 @@ __global__ void __launch_bounds__(kTPB) k_bfs_tail(uint32_t d0, uint32_t max_lev
         ++levels;
         __syncthreads();
     }
-    flush_counter(&ctl->bfs_pops, pops);
+    flush_stat(ctl, kStBfsPops, pops);
     if (threadIdx.x == 0) {
         ctl->bfs_cnt[d % 3u] = cur;
         ctl->bfs_cnt[(d + 1u) % 3u] = 0u;
@@ -411,31 +510,59 @@ __global__ void __launch_bounds__(kTPB) k_bfs_tail(uint32_t d0, uint32_t max_lev
     }
 }
 
-// Active set {excess > 0, height < n} (gpu_push_relabel_fast.cu:149-166);
-// unreached vertices are normalised to n+1 (gpu_push_relabel_fast.cu:141-147).
+// Active list after a global relabel (gpu_push_relabel_fast.cu:149-166:
+// {excess > 0, height < n}).  A global relabel only raises heights (BFS
+// distances bound every valid labelling from above), so no vertex becomes
+// active: the new list is the old one -- which holds every active vertex --
+// filtered by the new heights.  Entries are distinct, so survivors are too.
+// Unreached vertices keep the sentinel height, which every reader treats as
+// the reference's n+1 (gpu_push_relabel_fast.cu:141-147).
 template <bool kB>
-__global__ void __launch_bounds__(kTPB) k_build_active(uint32_t n, const long long* __restrict__ excess,
-                                                       uint32_t* __restrict__ height, uint32_t* __restrict__ wl,
-                                                       uint32_t p, const uint32_t* __restrict__ gid,
-                                                       const uint32_t* __restrict__ gn, Ctl* ctl) {
+__global__ void __launch_bounds__(kTPB) k_filter_active(uint32_t n, uint32_t p, const uint32_t* __restrict__ src,
+                                                        uint32_t* __restrict__ dst,
+                                                        const long long* __restrict__ excess,
+                                                        const uint32_t* __restrict__ height,
+                                                        const uint32_t* __restrict__ gid,
+                                                        const uint32_t* __restrict__ gn, Ctl* ctl) {
     __shared__ WgQueue q;
     q_init(q);
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rl_cnt[p] = 0u;
+    const uint32_t cnt = ctl->wl_cnt[p];
+    uint32_t* out_cnt = &ctl->wl_cnt[p ^ 1u];
     const uint32_t stride = gridDim.x * kTPB;
-    for (uint32_t base = blockIdx.x * kTPB; base < n; base += stride) {
-        const uint32_t v = base + threadIdx.x;
+    for (uint32_t base = blockIdx.x * kTPB; base < cnt; base += stride) {
+        const uint32_t i = base + threadIdx.x;
         bool a = false;
-        if (v < n) {
-            const uint32_t sh = src_h<kB>(v, n, gid, gn);
-            uint32_t h = height[v];
-            if (h == kSent) {
-                h = sh + 1u;
-                height[v] = h;
-            }
-            a = h < sh && excess[v] > 0;
+        uint32_t v = 0;
+        if (i < cnt) {
+            v = src[i];
+            a = height[v] < src_h<kB>(v, n, gid, gn) && excess[v] > 0;
         }
         q_push(q, a, v);
-        q_flush(q, wl, &ctl->wl_cnt[p]);
+        q_flush_if(q, dst, out_cnt, kTPB);
+    }
+    q_flush(q, dst, out_cnt);
+}
+
+// Batched mode: adds each lane's sink flow to its graph's total.  The lanes
+// of a wave nearly always belong to one graph, so the wave adds one sum per
+// distinct graph id instead of one atomic per sink push (atomics on one
+// address serialise).  All lanes of the wave must call this.
+template <bool kB>
+__device__ __forceinline__ void flush_graph_flow(bool valid, uint32_t v, unsigned long long f,
+                                                 const uint32_t* __restrict__ gid,
+                                                 unsigned long long* __restrict__ gflow) {
+    if constexpr (kB) {
+        const uint32_t g = valid && f ? gid[v] : 0u;
+        uint64_t pending = __ballot(f != 0ull);
+        while (pending) {
+            const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+            const uint32_t gl = __shfl(g, leader);
+            const bool mine = f != 0ull && g == gl;
+            const unsigned long long sum = wave_sum(mine ? f : 0ull);
+            if (__lane_id() == static_cast<unsigned>(leader)) atom_add(&gflow[gl], sum);
+            if (mine) f = 0ull;
+            pending = __ballot(f != 0ull);
+        }
     }
 }
 
@@ -448,9 +575,10 @@ __device__ __forceinline__ void push_vertex(bool valid, uint32_t v, uint32_t n, 
                                             int32_t* __restrict__ term, long long* excess,
                                             const uint32_t* __restrict__ height, uint32_t* mark,
                                             const uint32_t* __restrict__ gid, const uint32_t* __restrict__ gn,
-                                            unsigned long long* __restrict__ gflow, bool& first, bool cand[4],
-                                            int4& nb, unsigned long long& visits, unsigned long long& pushes,
-                                            unsigned long long& sinkf) {
+                                            bool& first, bool cand[4], int4& nb, unsigned long long& visits,
+                                            unsigned long long& pushes, unsigned long long& sinkf,
+                                            unsigned long long& sdlt) {
+    sdlt = 0ull;
     first = valid && atom_xchg(&mark[v], 1u) == 0u;
     long long e = valid ? atom_load(&excess[v]) : 0LL;
     const uint32_t h = valid ? height[v] : kSent;
@@ -479,8 +607,8 @@ __device__ __forceinline__ void push_vertex(bool valid, uint32_t v, uint32_t n, 
             e -= dlt;
             pushed += dlt;
             sinkf += static_cast<unsigned long long>(dlt);
+            sdlt = static_cast<unsigned long long>(dlt);
             ++pushes;
-            if constexpr (kB) atom_add(&gflow[gid[v]], static_cast<unsigned long long>(dlt));
         }
     }
 #pragma unroll
@@ -531,7 +659,11 @@ __device__ __forceinline__ bool relabel_vertex(uint32_t v, uint32_t n, const int
     for (int k = 0; k < 4; ++k) {
         const int u = slot_of(nb, k);
         if (u >= 0 && slot_of(r, k) != 0u) {
-            const uint32_t x = height[u];
+            uint32_t x = height[u];
+            // a vertex the last BFS did not reach keeps the sentinel; it
+            // stands for the reference's normalised height n+1
+            // (gpu_push_relabel_fast.cu:141-147)
+            x = x == kSent ? sh + 1u : x;
             mh = x < mh ? x : mh;
         }
     }
@@ -557,7 +689,6 @@ __global__ void __launch_bounds__(kTPB) k_push(uint32_t n, uint32_t p, const uin
     __shared__ WgQueue q;
     q_init(q);
     const uint32_t cnt = ctl->wl_cnt[p];
-    const bool small = cnt <= kWaveAppendMax;  // uniform across the grid
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->wl_cnt[p ^ 1u] = 0u;
     uint32_t* rl_cnt = &ctl->rl_cnt[p];
     const uint32_t stride = gridDim.x * kTPB;
@@ -568,23 +699,20 @@ __global__ void __launch_bounds__(kTPB) k_push(uint32_t n, uint32_t p, const uin
         const uint32_t v = valid ? wl[i] : 0u;
         bool first, cand[4];
         int4 nb;
-        push_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, mark, gid, gn, gflow, first, cand, nb,
-                        visits, pushes, sinkf);
+        unsigned long long sdlt;
+        push_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, mark, gid, gn, first, cand, nb, visits,
+                        pushes, sinkf, sdlt);
+        flush_graph_flow<kB>(valid, v, sdlt, gid, gflow);
         // every worklist vertex is relabel-checked exactly once this sub-round
-        if (small) {
-            wave_append(first, v, rl, rl_cnt);
+        q_push(q, first, v);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) wave_append(cand[k], static_cast<uint32_t>(slot_of(nb, k)), rl, rl_cnt);
-        } else {
-            q_push(q, first, v);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) q_push(q, cand[k], static_cast<uint32_t>(slot_of(nb, k)));
-            q_flush(q, rl, rl_cnt);
-        }
+        for (int k = 0; k < 4; ++k) q_push(q, cand[k], static_cast<uint32_t>(slot_of(nb, k)));
+        q_flush_if(q, rl, rl_cnt, 5 * kTPB);
     }
-    flush_counter(&ctl->visits, visits);
-    flush_counter(&ctl->pushes, pushes);
-    flush_counter(&ctl->sink_flow, sinkf);
+    q_flush(q, rl, rl_cnt);
+    flush_stat(ctl, kStVisits, visits);
+    flush_stat(ctl, kStPushes, pushes);
+    flush_stat(ctl, kStSinkFlow, sinkf);
 }
 
 // Relabel phase of sub-round with parity p: recompute heights of every vertex
@@ -599,7 +727,6 @@ __global__ void __launch_bounds__(kTPB) k_relabel(uint32_t n, uint32_t p, const 
     __shared__ WgQueue q;
     q_init(q);
     const uint32_t cnt = ctl->rl_cnt[p];
-    const bool small = cnt <= kWaveAppendMax;
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rl_cnt[p ^ 1u] = 0u;
     uint32_t* wl_cnt = &ctl->wl_cnt[p ^ 1u];
     const uint32_t stride = gridDim.x * kTPB;
@@ -612,14 +739,11 @@ __global__ void __launch_bounds__(kTPB) k_relabel(uint32_t n, uint32_t p, const 
             v = rl[i];
             act = relabel_vertex<kB>(v, n, nbr4, res, term, excess, height, mark, gid, gn, ctl, relabels);
         }
-        if (small) {
-            wave_append(act, v, wl_next, wl_cnt);
-        } else {
-            q_push(q, act, v);
-            q_flush(q, wl_next, wl_cnt);
-        }
+        q_push(q, act, v);
+        q_flush_if(q, wl_next, wl_cnt, kTPB);
     }
-    flush_counter(&ctl->relabels, relabels);
+    q_flush(q, wl_next, wl_cnt);
+    flush_stat(ctl, kStRelabels, relabels);
 }
 
 // LDS-counter append for a single-workgroup kernel: ranks from ballot + mbcnt,
@@ -676,8 +800,10 @@ __global__ void __launch_bounds__(kTPB) k_discharge_tail(uint32_t n, uint32_t p0
             const uint32_t v = valid ? wl[p][i] : 0u;
             bool first, cand[4];
             int4 nb;
-            push_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, mark, gid, gn, gflow, first, cand,
-                            nb, visits, pushes, sinkf);
+            unsigned long long sdlt;
+            push_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, mark, gid, gn, first, cand, nb,
+                            visits, pushes, sinkf, sdlt);
+            flush_graph_flow<kB>(valid, v, sdlt, gid, gflow);
             lds_append(first, v, rl, &s_cnt);
 #pragma unroll
             for (int k = 0; k < 4; ++k) lds_append(cand[k], static_cast<uint32_t>(slot_of(nb, k)), rl, &s_cnt);
@@ -712,10 +838,10 @@ __global__ void __launch_bounds__(kTPB) k_discharge_tail(uint32_t n, uint32_t p0
         __syncthreads();
         ++sub;
     }
-    flush_counter(&ctl->visits, visits);
-    flush_counter(&ctl->pushes, pushes);
-    flush_counter(&ctl->sink_flow, sinkf);
-    flush_counter(&ctl->relabels, relabels_all);
+    flush_stat(ctl, kStVisits, visits);
+    flush_stat(ctl, kStPushes, pushes);
+    flush_stat(ctl, kStSinkFlow, sinkf);
+    flush_stat(ctl, kStRelabels, relabels_all);
     if (threadIdx.x == 0) {
         const uint32_t pf = (p0 + sub) & 1u;
         ctl->wl_cnt[pf] = s_cur;
@@ -919,9 +1045,11 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     unsigned long long* gflow = kB ? reinterpret_cast<unsigned long long*>(base + L.gflow) : nullptr;
 
     HostBuf hctl_buf;
-    ck(hipHostMalloc(&hctl_buf.p, 3 * sizeof(Ctl), hipHostMallocDefault), "pinned ctl");
+    ck(hipHostMalloc(&hctl_buf.p, 5 * sizeof(Ctl), hipHostMallocDefault), "pinned ctl");
     Ctl* hctl = static_cast<Ctl*>(hctl_buf.p);
-    Ctl* snap = hctl + 1;  // two pipelined snapshots
+    Ctl* snap = hctl + 1;   // two pipelined snapshots of the discharge loop
+    Ctl* bsnap = hctl + 3;  // two of the BFS level batches
+    Event bfs_ev[2];
 
     KernelTimer* ktp = nullptr;
     auto read_ctl = [&](const char* where) {
@@ -961,41 +1089,75 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         ck(hipGetLastError(), "globalize");
         nbr4 = nbrg;
     }
-    k_prepare<<<sweep_grid, tpb, 0, s>>>(n, nbr4, d_term, rslot, excess, mark, ctl);
+    k_prepare<<<sweep_grid, tpb, 0, s>>>(n, nbr4, d_term, rslot, excess, mark, wl[0], ctl);
     ck(hipGetLastError(), "prepare");
     check_err("prepare");
 
-    uint64_t sub = 0;  // global sub-round counter; parity selects the worklist
-    // BFS from the sink; leaves the frontier buffers dead (wl[] reused).
+    uint64_t sub = 0;  // sub-rounds run (statistics)
+    // Parity position: the active list is wl[pos & 1].  It advances with every
+    // sub-round and once per active-list filter after a global relabel.
+    uint64_t pos = 0;
+    // BFS from the sink.  Its frontiers ping-pong between the relabel list
+    // (dead between sub-rounds) and the idle worklist, so the active list
+    // wl[pos & 1] survives for the filter that follows.
     auto global_relabel = [&]() {
         ck(hipEventRecord(ev_b0.e, s), "event");
-        k_bfs_seed<<<sweep_grid, tpb, 0, s>>>(n, d_term, height, ctl);
+        const uint32_t p = static_cast<uint32_t>(pos & 1u);
+        uint32_t* bf[2] = {rl, wl[p ^ 1u]};
+        k_bfs_begin<<<1, 1, 0, s>>>(ctl, p);
+        ck(hipGetLastError(), "bfs begin");
+        k_bfs_seed<<<sweep_grid, tpb, 0, s>>>(n, d_term, nbr4, d_res, height, bf[0], ctl);
         ck(hipGetLastError(), "bfs seed");
-        uint32_t d = 1, last = 0xFFFFFFFFu;
+        // Levels run in batches of 8 launches.  Batches are pipelined: batch
+        // b+1 is enqueued before the frontier count after batch b is read, and
+        // a level whose frontier is empty returns at once, so running past the
+        // last level costs a few microseconds instead of a host round trip
+        // per batch.
+        uint32_t d = 2;
         const uint32_t batch = 8;
-        for (;;) {
-            if (d > 1u && last <= kTailMax) {
-                // small frontier: one workgroup runs the levels back to back
+        int next = 0;
+        struct Mark {
+            int slot;
+            uint32_t d;  // level whose frontier count the snapshot holds
+        };
+        auto enqueue_batch = [&]() -> Mark {
+            for (uint32_t b = 0; b < batch; ++b, ++d) {
                 kt.begin(KernelTimer::kLevel);
-                k_bfs_tail<<<1, tpb, 0, s>>>(d, 4096u, wl[0], wl[1], nbr4, rslot, d_res, height, ctl);
+                k_bfs_level<<<grid, tpb, 0, s>>>(n, d, bu_threshold, bf[d & 1u], bf[(d + 1u) & 1u], nbr4, rslot,
+                                                 d_res, height, ctl);
                 kt.end();
+            }
+            ck(hipGetLastError(), "bfs expand");
+            const int sl = next;
+            next ^= 1;
+            ck(hipMemcpyAsync(&bsnap[sl], ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s), "bfs snapshot");
+            ck(hipEventRecord(bfs_ev[sl].e, s), "event");
+            return Mark{sl, d};
+        };
+        Mark prev = enqueue_batch();
+        for (;;) {
+            const Mark cur = enqueue_batch();  // speculative
+            ck(hipEventSynchronize(bfs_ev[prev.slot].e), "bfs snapshot wait");
+            if (ktp) ktp->harvest();
+            const uint32_t last = bsnap[prev.slot].bfs_cnt[prev.d % 3u];
+            if (last == 0u) break;  // the speculative batch ran on empty frontiers
+            if (static_cast<uint64_t>(prev.d) > static_cast<uint64_t>(n) + 1ull)
+                fail(WTV_EINVARIANT, std::string(kPrefix) + "BFS exceeded n+1 levels");
+            if (last <= kTailMax) {
+                // small frontier: one workgroup runs the levels back to back,
+                // from the level the speculative batch reached (it takes the
+                // frontier only while it holds at most kTailMax vertices)
+                k_bfs_tail<<<1, tpb, 0, s>>>(cur.d, 4096u, bf[0], bf[1], nbr4, rslot, d_res, height, ctl);
                 ck(hipGetLastError(), "bfs tail");
                 read_ctl("bfs tail");
                 d = hctl->tail_d;
-            } else {
-                for (uint32_t b = 0; b < batch; ++b, ++d) {
-                    kt.begin(KernelTimer::kLevel);
-                    k_bfs_level<<<grid, tpb, 0, s>>>(n, d, bu_threshold, wl[d & 1u], wl[(d + 1u) & 1u], nbr4,
-                                                     rslot, d_res, height, ctl);
-                    kt.end();
-                }
-                ck(hipGetLastError(), "bfs expand");
-                read_ctl("bfs count");
+                if (hctl->bfs_cnt[d % 3u] == 0u) break;
+                if (static_cast<uint64_t>(d) > static_cast<uint64_t>(n) + 1ull)
+                    fail(WTV_EINVARIANT, std::string(kPrefix) + "BFS exceeded n+1 levels");
+                prev = enqueue_batch();
+                continue;
             }
-            last = hctl->bfs_cnt[d % 3u];
-            if (last == 0u) break;
-            if (static_cast<uint64_t>(d) > static_cast<uint64_t>(n) + 1ull)
-                fail(WTV_EINVARIANT, std::string(kPrefix) + "BFS exceeded n+1 levels");
+            prev = cur;
         }
         ck(hipEventRecord(ev_b1.e, s), "event");
         ck(hipEventSynchronize(ev_b1.e), "event sync");
@@ -1003,17 +1165,18 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         ck(hipEventElapsedTime(&ms, ev_b0.e, ev_b1.e), "event time");
         bfs_ms += ms;
         ++st.global_relabels;
+        if (trace) read_ctl("trace");  // the level count, current
         if (trace)
             std::fprintf(stderr, "[wtv] global relabel %llu: levels=%u %.3f ms\n",
                          static_cast<unsigned long long>(st.global_relabels), hctl->bfs_levels, ms);
     };
     auto build_active = [&]() -> uint32_t {
-        const uint32_t p = static_cast<uint32_t>(sub & 1u);
-        k_set_u32<<<1, 1, 0, s>>>(&ctl->wl_cnt[p], 0u);
-        k_build_active<kB><<<sweep_grid, tpb, 0, s>>>(n, excess, height, wl[p], p, gid, gn, ctl);
-        ck(hipGetLastError(), "build active");
+        const uint32_t p = static_cast<uint32_t>(pos & 1u);
+        k_filter_active<kB><<<grid, tpb, 0, s>>>(n, p, wl[p], wl[p ^ 1u], excess, height, gid, gn, ctl);
+        ck(hipGetLastError(), "filter active");
+        ++pos;
         read_ctl("active count");
-        return hctl->wl_cnt[p];
+        return hctl->wl_cnt[p ^ 1u];
     };
 
     global_relabel();
@@ -1025,7 +1188,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     // Adaptive cadence: a global relabel also runs once the relabel work since
     // the last one exceeds alpha * (active vertices right after it) + 1024.
     const double alpha = prm.relabel_alpha == 0.0 ? 4.0 : prm.relabel_alpha;
-    uint64_t relabels_at_gr = hctl->relabels;
+    uint64_t relabels_at_gr = stat_sum(*hctl, kStRelabels);
     uint64_t active_at_gr = active;
 
     // One macro round = local_steps sub-rounds (push + relabel).  Rounds are
@@ -1034,10 +1197,10 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     // device never idles on the host's decision.
     GraphExec gexec[2];
     auto launch_round = [&]() {
-        const uint32_t p0 = static_cast<uint32_t>(sub & 1u);
-        auto enqueue = [&](uint64_t sub0) {
+        const uint32_t p0 = static_cast<uint32_t>(pos & 1u);
+        auto enqueue = [&](uint64_t pos0) {
             for (int j = 0; j < local_steps; ++j) {
-                const uint32_t p = static_cast<uint32_t>((sub0 + static_cast<uint64_t>(j)) & 1u);
+                const uint32_t p = static_cast<uint32_t>((pos0 + static_cast<uint64_t>(j)) & 1u);
                 kt.begin(KernelTimer::kPush);
                 k_push<kB><<<grid, tpb, 0, s>>>(n, p, wl[p], rl, nbr4, rslot, d_res, d_term, excess, height, mark,
                                                 gid, gn, gflow, ctl);
@@ -1052,7 +1215,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             if (!gexec[p0].exec) {
                 hipGraph_t g = nullptr;
                 ck(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
-                enqueue(sub);
+                enqueue(pos);
                 ck(hipStreamEndCapture(s, &g), "end capture");
                 const hipError_t e = hipGraphInstantiate(&gexec[p0].exec, g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
@@ -1060,14 +1223,15 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             }
             ck(hipGraphLaunch(gexec[p0].exec, s), "graph launch");
         } else {
-            enqueue(sub);
+            enqueue(pos);
         }
         ck(hipGetLastError(), "discharge");
         sub += static_cast<uint64_t>(local_steps);
+        pos += static_cast<uint64_t>(local_steps);
     };
     struct Pending {
         int slot;
-        uint64_t sub_after;
+        uint64_t pos_after;
         Event* ev;
     };
     Event snap_ev[2];
@@ -1077,7 +1241,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         next_slot ^= 1;
         ck(hipMemcpyAsync(&snap[sl], ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s), "snapshot");
         ck(hipEventRecord(snap_ev[sl].e, s), "event");
-        return Pending{sl, sub, &snap_ev[sl]};
+        return Pending{sl, pos, &snap_ev[sl]};
     };
 
     // Small active lists (<= tail_cap) go to the one-workgroup discharge kernel.
@@ -1086,7 +1250,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     auto do_global_relabel = [&]() {
         global_relabel();
         active = build_active();
-        relabels_now = relabels_at_gr = hctl->relabels;
+        relabels_now = relabels_at_gr = stat_sum(*hctl, kStRelabels);
         active_at_gr = active;
     };
     auto work_trigger = [&]() -> bool {
@@ -1122,7 +1286,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             const uint32_t max_sub = static_cast<uint32_t>(
                 std::min<uint64_t>(4096ull, rounds_left * static_cast<uint64_t>(local_steps)));
             kt.begin(KernelTimer::kTail);
-            k_discharge_tail<kB><<<1, tpb, 0, s>>>(n, static_cast<uint32_t>(sub & 1u), tail_cap, max_sub, budget, wl[0],
+            k_discharge_tail<kB><<<1, tpb, 0, s>>>(n, static_cast<uint32_t>(pos & 1u), tail_cap, max_sub, budget, wl[0],
                                                    wl[1], rl, nbr4, rslot, d_res, d_term, excess, height, mark, gid, gn,
                                                    gflow, ctl);
             kt.end();
@@ -1133,10 +1297,11 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             st.tail_sub_rounds += ts;
             const uint64_t before = st.rounds;
             sub += ts;
+            pos += ts;
             st.rounds += std::max<uint64_t>(1ull, (ts + static_cast<uint64_t>(local_steps) - 1u) /
                                                       static_cast<uint64_t>(local_steps));
-            relabels_now = hctl->relabels;
-            active = hctl->wl_cnt[sub & 1u];
+            relabels_now = stat_sum(*hctl, kStRelabels);
+            active = hctl->wl_cnt[pos & 1u];
             const bool by_period = macro_period > 0 && st.rounds / static_cast<uint64_t>(macro_period) !=
                                                            before / static_cast<uint64_t>(macro_period);
             if (active && (by_period || work_trigger())) do_global_relabel();
@@ -1158,12 +1323,12 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             if (ktp) ktp->harvest();
             const Ctl& S = snap[prev.slot];
             check_device_error(S);
-            const uint32_t act_prev = S.wl_cnt[prev.sub_after & 1u];
-            relabels_now = S.relabels;
+            const uint32_t act_prev = S.wl_cnt[prev.pos_after & 1u];
+            relabels_now = stat_sum(S, kStRelabels);
             if (trace)
                 std::fprintf(stderr, "[wtv] round %llu: active=%u relabels=%llu\n",
                              static_cast<unsigned long long>(st.rounds), act_prev,
-                             static_cast<unsigned long long>(S.relabels));
+                             stat_sum(S, kStRelabels));
             if (act_prev == 0u) {
                 if (spec) ck(hipEventSynchronize(cur.ev->e), "drain");
                 active = 0;
@@ -1180,8 +1345,8 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
                 ck(hipEventSynchronize(cur.ev->e), "switch");
                 const Ctl& C2 = snap[cur.slot];
                 check_device_error(C2);
-                active = C2.wl_cnt[cur.sub_after & 1u];
-                relabels_now = C2.relabels;
+                active = C2.wl_cnt[cur.pos_after & 1u];
+                relabels_now = stat_sum(C2, kStRelabels);
                 break;
             }
             prev = cur;
@@ -1219,9 +1384,10 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             << " vertices still active after the final BFS";
         fail(WTV_EINVARIANT, oss.str());
     }
-    if (hctl->sink_flow > static_cast<unsigned long long>(INT64_MAX))
+    const unsigned long long sink_flow = stat_sum(*hctl, kStSinkFlow);
+    if (sink_flow > static_cast<unsigned long long>(INT64_MAX))
         fail(WTV_EINVARIANT, std::string(kPrefix) + "flow exceeds int64");
-    st.raw_flow = static_cast<int64_t>(hctl->sink_flow);
+    st.raw_flow = static_cast<int64_t>(sink_flow);
     if constexpr (kB) {
         std::vector<unsigned long long> f(G);
         ck(hipMemcpyAsync(f.data(), gflow, G * 8u, hipMemcpyDeviceToHost, s), "batch flows D2H");
@@ -1231,15 +1397,16 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             B->h_flows[g] = static_cast<int64_t>(f[g]);
             sum += f[g];
         }
-        if (sum != hctl->sink_flow) fail(WTV_EINVARIANT, std::string(kPrefix) + "per-graph flows do not add up");
+        if (sum != sink_flow) fail(WTV_EINVARIANT, std::string(kPrefix) + "per-graph flows do not add up");
     }
-    st.visits = hctl->visits;
-    st.pushes = hctl->pushes;
-    st.relabels = hctl->relabels;
-    st.bfs_pops = hctl->bfs_pops;
+    st.visits = stat_sum(*hctl, kStVisits);
+    st.pushes = stat_sum(*hctl, kStPushes);
+    st.relabels = stat_sum(*hctl, kStRelabels);
+    st.bfs_pops = stat_sum(*hctl, kStBfsPops);
     st.bfs_levels = hctl->bfs_levels;
     st.bfs_bu_levels = hctl->bu_levels;
-    st.bfs_bu_checks = hctl->bu_checks;
+    st.bfs_bu_checks = stat_sum(*hctl, kStBuChecks);
+    st.bfs_seed_checks = stat_sum(*hctl, kStSeedChecks);
     st.sub_rounds = sub;
     float ms = 0.f;
     ck(hipEventElapsedTime(&ms, ev_t0.e, ev_t1.e), "event time");
