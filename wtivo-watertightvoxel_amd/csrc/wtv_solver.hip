@@ -318,7 +318,7 @@ static_assert(kSeedUnroll * kTPB <= kQCap, "queue holds one append per vertex pe
 __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __restrict__ term,
                                                    const int4* __restrict__ nbr4, const uint32_t* __restrict__ res,
                                                    uint32_t* __restrict__ height, uint32_t* __restrict__ fout,
-                                                   Ctl* ctl) {
+                                                   uint32_t first, Ctl* ctl) {
     __shared__ WgQueue q;
     q_init(q);
     const uint32_t span = kTPB * kSeedUnroll;
@@ -353,7 +353,11 @@ __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __
         }
 #pragma unroll
         for (int j = 0; j < kSeedUnroll; ++j) {
-            if (static_cast<uint64_t>(base) + static_cast<uint64_t>(j) * kTPB + threadIdx.x < n)
+            // a vertex with residual sink capacity keeps height 1 once set:
+            // valid labels give it h <= 1, and term < 0 never returns once
+            // the sink arc is saturated, so later relabels skip those writes
+            if (static_cast<uint64_t>(base) + static_cast<uint64_t>(j) * kTPB + threadIdx.x < n &&
+                (t[j] >= 0 || first))
                 height[v[j]] = t[j] < 0 ? 1u : (found[j] ? 2u : kSent);
             q_push(q, found[j], v[j]);
         }
@@ -462,7 +466,10 @@ __global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint
 // the frontier of the level it stops at in f[d & 1] with its count in the
 // counter ring, exactly as a sequence of k_bfs_level launches would.
 constexpr uint32_t kTailMax = 1024;
-__global__ void __launch_bounds__(kTPB) k_bfs_tail(uint32_t d0, uint32_t max_levels, uint32_t* f0, uint32_t* f1,
+// The one-workgroup kernels run 1024 threads: a list of up to 1024 items
+// takes one pass per level or phase.
+constexpr int kTailTPB = 1024;
+__global__ void __launch_bounds__(kTailTPB) k_bfs_tail(uint32_t d0, uint32_t max_levels, uint32_t* f0, uint32_t* f1,
                                                    const int4* __restrict__ nbr4, const uint8_t* __restrict__ rslot,
                                                    const uint32_t* __restrict__ res, uint32_t* height, Ctl* ctl) {
     __shared__ uint32_t s_cnt;
@@ -475,7 +482,7 @@ __global__ void __launch_bounds__(kTPB) k_bfs_tail(uint32_t d0, uint32_t max_lev
         uint32_t* fout = f[(d + 1u) & 1u];
         if (threadIdx.x == 0) s_cnt = 0u;
         __syncthreads();
-        for (uint32_t base = 0; base < cur; base += kTPB) {
+        for (uint32_t base = 0; base < cur; base += kTailTPB) {
             const uint32_t i = base + threadIdx.x;
             const bool valid = i < cur;
             const uint32_t u = valid ? fin[i] : 0u;
@@ -766,7 +773,7 @@ __device__ __forceinline__ void lds_append(bool pred, uint32_t val, uint32_t* li
 // height increases were made (the host then runs a global relabel).  Leaves
 // the counters exactly as the same number of k_push/k_relabel pairs would.
 template <bool kB>
-__global__ void __launch_bounds__(kTPB) k_discharge_tail(uint32_t n, uint32_t p0, uint32_t cap, uint32_t max_sub,
+__global__ void __launch_bounds__(kTailTPB) k_discharge_tail(uint32_t n, uint32_t p0, uint32_t cap, uint32_t max_sub,
                                                          unsigned long long rel_budget, uint32_t* wl0, uint32_t* wl1,
                                                          uint32_t* rl, const int4* __restrict__ nbr4,
                                                          const uint8_t* __restrict__ rslot, uint32_t* res,
@@ -778,7 +785,7 @@ __global__ void __launch_bounds__(kTPB) k_discharge_tail(uint32_t n, uint32_t p0
     __shared__ uint32_t s_cnt;
     __shared__ uint32_t s_cur;
     __shared__ unsigned long long s_rel;
-    __shared__ unsigned long long s_wrel[kTPB / 64];
+    __shared__ unsigned long long s_wrel[kTailTPB / 64];
     uint32_t* wl[2] = {wl0, wl1};
     if (threadIdx.x == 0) {
         s_cur = ctl->wl_cnt[p0];
@@ -794,7 +801,7 @@ __global__ void __launch_bounds__(kTPB) k_discharge_tail(uint32_t n, uint32_t p0
         // push phase
         if (threadIdx.x == 0) s_cnt = 0u;
         __syncthreads();
-        for (uint32_t base = 0; base < cur; base += kTPB) {
+        for (uint32_t base = 0; base < cur; base += kTailTPB) {
             const uint32_t i = base + threadIdx.x;
             const bool valid = i < cur;
             const uint32_t v = valid ? wl[p][i] : 0u;
@@ -815,7 +822,7 @@ __global__ void __launch_bounds__(kTPB) k_discharge_tail(uint32_t n, uint32_t p0
         if (threadIdx.x == 0) s_cnt = 0u;
         __syncthreads();
         unsigned long long rel = 0;
-        for (uint32_t base = 0; base < rcnt; base += kTPB) {
+        for (uint32_t base = 0; base < rcnt; base += kTailTPB) {
             const uint32_t i = base + threadIdx.x;
             bool act = false;
             uint32_t v = 0;
@@ -831,7 +838,7 @@ __global__ void __launch_bounds__(kTPB) k_discharge_tail(uint32_t n, uint32_t p0
         __syncthreads();
         if (threadIdx.x == 0) {
             unsigned long long t = 0;
-            for (int w = 0; w < kTPB / 64; ++w) t += s_wrel[w];
+            for (int w = 0; w < kTailTPB / 64; ++w) t += s_wrel[w];
             s_rel += t;
             s_cur = s_cnt;
         }
@@ -1106,7 +1113,8 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         uint32_t* bf[2] = {rl, wl[p ^ 1u]};
         k_bfs_begin<<<1, 1, 0, s>>>(ctl, p);
         ck(hipGetLastError(), "bfs begin");
-        k_bfs_seed<<<sweep_grid, tpb, 0, s>>>(n, d_term, nbr4, d_res, height, bf[0], ctl);
+        k_bfs_seed<<<sweep_grid, tpb, 0, s>>>(n, d_term, nbr4, d_res, height, bf[0], st.global_relabels == 0 ? 1u : 0u,
+                                              ctl);
         ck(hipGetLastError(), "bfs seed");
         // Levels run in batches of 8 launches.  Batches are pipelined: batch
         // b+1 is enqueued before the frontier count after batch b is read, and
@@ -1147,7 +1155,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
                 // small frontier: one workgroup runs the levels back to back,
                 // from the level the speculative batch reached (it takes the
                 // frontier only while it holds at most kTailMax vertices)
-                k_bfs_tail<<<1, tpb, 0, s>>>(cur.d, 4096u, bf[0], bf[1], nbr4, rslot, d_res, height, ctl);
+                k_bfs_tail<<<1, kTailTPB, 0, s>>>(cur.d, 4096u, bf[0], bf[1], nbr4, rslot, d_res, height, ctl);
                 ck(hipGetLastError(), "bfs tail");
                 read_ctl("bfs tail");
                 d = hctl->tail_d;
@@ -1286,7 +1294,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             const uint32_t max_sub = static_cast<uint32_t>(
                 std::min<uint64_t>(4096ull, rounds_left * static_cast<uint64_t>(local_steps)));
             kt.begin(KernelTimer::kTail);
-            k_discharge_tail<kB><<<1, tpb, 0, s>>>(n, static_cast<uint32_t>(pos & 1u), tail_cap, max_sub, budget, wl[0],
+            k_discharge_tail<kB><<<1, kTailTPB, 0, s>>>(n, static_cast<uint32_t>(pos & 1u), tail_cap, max_sub, budget, wl[0],
                                                    wl[1], rl, nbr4, rslot, d_res, d_term, excess, height, mark, gid, gn,
                                                    gflow, ctl);
             kt.end();
