@@ -162,11 +162,12 @@ def main():
     def bfs_bytes(x):
         return 61 * x["bfs_pops"] + 4 * n * x["bfs_bu_levels"] + 48 * x["bfs_bu_checks"]
 
-    # Whole solve: discharge + BFS levels + per relabel the level-1/2 sweep
-    # (term read 4 + height write 4 per vertex, 48 B per term >= 0 vertex
-    # examined) + 2 B/vertex for the partition.
+    # Whole solve: discharge + BFS levels + the level-1/2 sweep of every relabel
+    # (term read 4 B per vertex; height write 4 B per vertex in the first, then
+    # only for the term >= 0 vertices it examines, each costing 48 B of nbr,
+    # res and neighbour terms + 4 B of height) + 2 B/vertex for the partition.
     bytes_alg = (69 * s0["visits"] + 32 * s0["pushes"] + 4 * s0["relabels"] + bfs_bytes(s0)
-                 + 8 * n * s0["global_relabels"] + 48 * s0["bfs_seed_checks"] + 2 * n)
+                 + 4 * n * s0["global_relabels"] + 4 * n + 52 * s0["bfs_seed_checks"] + 2 * n)
     t_solve_s = s0["t_solve_ms"] / 1e3
     roofline = None
     kernels = {}
@@ -311,6 +312,21 @@ def atomic_probe(torch, n=1 << 26, reps=3):
     g.manual_seed(7)
     idx_seq = torch.arange(n, dtype=torch.int64, device="cuda")
     idx_rnd = torch.randperm(n, device="cuda", generator=g)
+    # one address: every element adds to dst[0] (device-scope atomics on one
+    # address serialise, so this is the rate a shared counter sustains)
+    m = 1 << 20
+    one = torch.zeros(1, dtype=torch.int32, device="cuda")
+    idx0 = torch.zeros(m, dtype=torch.int64, device="cuda")
+    src1 = torch.ones(m, dtype=torch.int32, device="cuda")
+    one.index_add_(0, idx0, src1)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    one.index_add_(0, idx0, src1)
+    e1.record()
+    torch.cuda.synchronize()
+    out["u32_one_address_atomics_per_s"] = round(m / (e0.elapsed_time(e1) * 1e-3), 1)
+    assert int(one[0]) == 2 * m
     for dt, name in ((torch.int32, "u32"), (torch.int64, "i64")):
         dst = torch.zeros(n, dtype=dt, device="cuda")
         src = torch.ones(n, dtype=dt, device="cuda")

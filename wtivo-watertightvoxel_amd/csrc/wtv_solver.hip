@@ -408,6 +408,11 @@ __global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint
                                                     const int4* __restrict__ nbr4, const uint8_t* __restrict__ rslot,
                                                     const uint32_t* __restrict__ res, uint32_t* height, Ctl* ctl) {
     __shared__ WgQueue q;
+    // The first frontier entry of this thread is loaded before the count is
+    // known (the buffer holds n entries; entries past the count are ignored),
+    // so the two loads overlap instead of forming two round trips.
+    const uint32_t i0 = blockIdx.x * kTPB + threadIdx.x;
+    const uint32_t u0 = i0 < n ? fin[i0] : 0u;
     q_init(q);
     const uint32_t cnt = ctl->bfs_cnt[d % 3u];
     uint32_t* out_cnt = &ctl->bfs_cnt[(d + 1u) % 3u];
@@ -416,6 +421,8 @@ __global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint
         ctl->bfs_cnt[(d + 2u) % 3u] = 0u;
         if (cnt) ctl->bfs_levels += 1u;
         if (bottom_up) ctl->bu_levels += 1u;
+        // every frontier entry is expanded once: one add per level
+        else if (cnt) atom_add(&ctl->stat[kStBfsPops][d % kStatSlots], static_cast<unsigned long long>(cnt));
     }
     const uint32_t stride = gridDim.x * kTPB;
     unsigned long long work = 0;
@@ -446,8 +453,7 @@ __global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint
         for (uint32_t base = blockIdx.x * kTPB; base < cnt; base += stride) {
             const uint32_t i = base + threadIdx.x;
             const bool valid = i < cnt;
-            const uint32_t u = valid ? fin[i] : 0u;
-            work += valid;
+            const uint32_t u = !valid ? 0u : (base == blockIdx.x * kTPB ? u0 : fin[i]);
             int4 nb;
             bool claim[4];
             td_claims(valid, u, d, nbr4, rslot, res, height, nb, claim);
@@ -456,7 +462,6 @@ __global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint
             q_flush_if(q, fout, out_cnt, 4 * kTPB);
         }
         q_flush(q, fout, out_cnt);
-        flush_stat(ctl, kStBfsPops, work);
     }
 }
 
