@@ -42,12 +42,24 @@ inline void ckg(hipError_t e, const char* where) {
     fail(WTV_EDEVICE, std::string(kPrefix) + "HIP at " + where + ": " + hipGetErrorString(e));
 }
 
+// Counters are spread over kSlots addresses (slot = workgroup index mod
+// kSlots): device-scope atomics on one address serialise at ~12 ns each on
+// MI355X, and a 78M-tet build has ~19k workgroups.  The host sums the slots.
+constexpr int kSlots = 64;
 struct BuildCtl {
-    unsigned long long free_n, fixed_source, fixed_sink, full_refs, free_refs, terminal_edges;
-    long long base_flow;
+    unsigned long long free_n[kSlots], fixed_source[kSlots], fixed_sink[kSlots], full_refs[kSlots],
+        free_refs[kSlots], terminal_edges[kSlots];
+    long long base_flow[kSlots];
     int32_t bad_range;  // neighbour out of range (classify)
     int32_t bad_build;  // 1 hull face on a free tet, 3 bad class, 4 term range, 5 bad vertex, 6 overflow
 };
+struct BuildSums {
+    unsigned long long free_n = 0, fixed_source = 0, fixed_sink = 0, full_refs = 0, free_refs = 0,
+                       terminal_edges = 0;
+    long long base_flow = 0;
+    int32_t bad_range = 0, bad_build = 0;
+};
+__device__ __forceinline__ uint32_t cslot() { return blockIdx.x % kSlots; }
 
 __device__ __forceinline__ unsigned long long wsum(unsigned long long x) {
 #pragma unroll
@@ -116,9 +128,9 @@ __global__ void __launch_bounds__(kB) k_classify(uint32_t N, const int4* __restr
         cls[i] = c;
     }
     if (bad) flag(&ctl->bad_range, 1);
-    wg_add(&ctl->fixed_source, ns, lds);
-    wg_add(&ctl->fixed_sink, nk, lds);
-    wg_add(&ctl->full_refs, refs, lds);
+    wg_add(&ctl->fixed_source[cslot()], ns, lds);
+    wg_add(&ctl->fixed_sink[cslot()], nk, lds);
+    wg_add(&ctl->full_refs[cslot()], refs, lds);
     // per-tile free count (also summed into free_n)
     unsigned long long x = wsum(nf);
     if (__lane_id() == 0) lds[threadIdx.x >> 6] = x;
@@ -127,7 +139,7 @@ __global__ void __launch_bounds__(kB) k_classify(uint32_t N, const int4* __restr
         unsigned long long t = 0;
         for (int i = 0; i < kB / 64; ++i) t += lds[i];
         tile_free[blockIdx.x] = static_cast<uint32_t>(t);
-        if (t) add_u64(&ctl->free_n, t);
+        if (t) add_u64(&ctl->free_n[cslot()], t);
     }
 }
 
@@ -265,9 +277,9 @@ __global__ void __launch_bounds__(kB) k_build(uint32_t N, const double* __restri
         term_r[rr] = static_cast<int32_t>(diff);
     }
     if (bad) flag(&ctl->bad_build, bad);
-    wg_add(&ctl->free_refs, refs, lds_u);
-    wg_add(&ctl->terminal_edges, terms, lds_u);
-    wg_add(&ctl->base_flow, base, lds_i);
+    wg_add(&ctl->free_refs[cslot()], refs, lds_u);
+    wg_add(&ctl->terminal_edges[cslot()], terms, lds_u);
+    wg_add(&ctl->base_flow[cslot()], base, lds_i);
 }
 
 __global__ void __launch_bounds__(kB) k_scatter(uint32_t n, const uint32_t* __restrict__ free_old,
@@ -322,7 +334,7 @@ __global__ void __launch_bounds__(kB) k_surf_count(uint32_t N, const int4* __res
         unsigned long long t = 0;
         for (int i = 0; i < kB / 64; ++i) t += lds[i];
         tile_cnt[blockIdx.x] = static_cast<uint32_t>(t);
-        if (t) add_u64(&ctl->free_n, t);  // total faces
+        if (t) add_u64(&ctl->free_n[cslot()], t);  // total faces
     }
 }
 
@@ -477,6 +489,27 @@ struct Dev {
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Copies the control block to the host and sums the counter slots.
+BuildSums read_build_ctl(const BuildCtl* d, hipStream_t s, const char* where) {
+    std::vector<BuildCtl> raw(1);
+    ckg(hipMemcpyAsync(raw.data(), d, sizeof(BuildCtl), hipMemcpyDeviceToHost, s), where);
+    ckg(hipStreamSynchronize(s), where);
+    const BuildCtl& r = raw[0];
+    BuildSums h;
+    for (int i = 0; i < kSlots; ++i) {
+        h.free_n += r.free_n[i];
+        h.fixed_source += r.fixed_source[i];
+        h.fixed_sink += r.fixed_sink[i];
+        h.full_refs += r.full_refs[i];
+        h.free_refs += r.free_refs[i];
+        h.terminal_edges += r.terminal_edges[i];
+        h.base_flow += r.base_flow[i];
+    }
+    h.bad_range = r.bad_range;
+    h.bad_build = r.bad_build;
+    return h;
+}
+
 }  // namespace
 
 // Builds the reduced graph on the device.  Outputs are caller buffers sized
@@ -498,12 +531,11 @@ bool build_reduced(const double* V, size_t nv, const int32_t* dT, const int32_t*
     const int4* nbr4 = reinterpret_cast<const int4*>(dN);
     const int4* tets4 = reinterpret_cast<const int4*>(dT);
     BuildCtl* ctl = dCtl.as<BuildCtl>();
-    BuildCtl h{};
+    BuildSums h;
     // classify + reduced ids (wtivo_gpupr_bindings.cpp:139-188)
     k_classify<<<tiles, kB, 0, s>>>(n32, nbr4, d_lab, dCls.as<int32_t>(), dTiles.as<uint32_t>(), ctl);
     ckg(hipGetLastError(), "classify");
-    ckg(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s), "classify D2H");
-    ckg(hipStreamSynchronize(s), "classify sync");
+    h = read_build_ctl(ctl, s, "classify D2H");
     if (h.bad_range) fail(WTV_EINVAL, "neighbor out of range");
     if (h.full_refs & 1ull) fail(WTV_EINVAL, "neighbor reference count not even");
     info.free_n = h.free_n;
@@ -521,8 +553,7 @@ bool build_reduced(const double* V, size_t nv, const int32_t* dT, const int32_t*
     k_build<<<grid, kB, 0, s>>>(n32, dV.as<double>(), nv, tets4, nbr4, d_lab, dCls.as<int32_t>(), fill_T,
                                 reinterpret_cast<int4*>(gN), reinterpret_cast<uint4*>(gR), gT, ctl);
     ckg(hipGetLastError(), "build");
-    ckg(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s), "build D2H");
-    ckg(hipStreamSynchronize(s), "build sync");
+    h = read_build_ctl(ctl, s, "build D2H");
     // the reference's build lambda turns every failure, the capacity
     // overflow_error included, into one runtime_error (:292,296-297)
     if (h.bad_build == 6) fail(WTV_EINVAL, "invalid graph reduction/capacity state (capacity outside safe 30-bit range)");
@@ -654,13 +685,12 @@ void surface_extract(const uint8_t* lp, const double* V, size_t nv, const int32_
     ckg(hipMemcpyAsync(dLab.p, lp, N, hipMemcpyHostToDevice, s), "labels H2D");
     ckg(hipMemsetAsync(dCtl.p, 0, sizeof(BuildCtl), s), "ctl clear");
     BuildCtl* ctl = dCtl.as<BuildCtl>();
-    BuildCtl h{};
+    BuildSums h;
     const int4* nbr4 = reinterpret_cast<const int4*>(dN);
     const int4* tets4 = reinterpret_cast<const int4*>(dT);
     k_surf_count<<<tiles, kB, 0, s>>>(n32, nbr4, dLab.as<uint8_t>(), dTiles.as<uint32_t>(), ctl);
     ckg(hipGetLastError(), "surface count");
-    ckg(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s), "surface D2H");
-    ckg(hipStreamSynchronize(s), "surface count sync");
+    h = read_build_ctl(ctl, s, "surface D2H");
     if (h.bad_range) fail(WTV_EINVAL, "surface neighbor out of range");
     const uint64_t F = h.free_n;
     if (F * 3u > 0x7FFFFFFFull) fail(WTV_EINVAL, "surface too large");
@@ -676,8 +706,7 @@ void surface_extract(const uint8_t* lp, const double* V, size_t nv, const int32_
     k_surf_emit<<<tiles, kB, 0, s>>>(n32, tets4, nbr4, dLab.as<uint8_t>(), dV.as<double>(), nv, dTiles.as<uint32_t>(),
                                      dRaw.as<int32_t>(), dUsed.as<uint8_t>(), ctl);
     ckg(hipGetLastError(), "surface emit");
-    ckg(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s), "surface D2H");
-    ckg(hipStreamSynchronize(s), "surface emit sync");
+    h = read_build_ctl(ctl, s, "surface D2H");
     if (h.bad_build) fail(WTV_EINVAL, "surface invalid tet vertex index");
     const uint32_t nv32 = static_cast<uint32_t>(nv);
     k_flag_count<<<vtiles, kB, 0, s>>>(nv32, dUsed.as<uint8_t>(), dVt.as<uint32_t>());
