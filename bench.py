@@ -312,8 +312,8 @@ def atomic_probe(torch, n=1 << 26, reps=3):
     g.manual_seed(7)
     idx_seq = torch.arange(n, dtype=torch.int64, device="cuda")
     idx_rnd = torch.randperm(n, device="cuda", generator=g)
-    # one address: every element adds to dst[0] (device-scope atomics on one
-    # address serialise, so this is the rate a shared counter sustains)
+    # one address: every element adds to dst[0]; device-scope atomics on one
+    # address serialise, so this is the rate a shared counter sustains
     m = 1 << 20
     one = torch.zeros(1, dtype=torch.int32, device="cuda")
     idx0 = torch.zeros(m, dtype=torch.int64, device="cuda")
@@ -325,7 +325,9 @@ def atomic_probe(torch, n=1 << 26, reps=3):
     one.index_add_(0, idx0, src1)
     e1.record()
     torch.cuda.synchronize()
-    out["u32_one_address_atomics_per_s"] = round(m / (e0.elapsed_time(e1) * 1e-3), 1)
+    # lanes of a wave that hit one address are combined into one request
+    # (rocprofv3 TCC_ATOMIC_sum reads m/64), so the serialised rate is per wave
+    out["one_address_requests_per_s"] = round(m / 64 / (e0.elapsed_time(e1) * 1e-3), 1)
     assert int(one[0]) == 2 * m
     for dt, name in ((torch.int32, "u32"), (torch.int64, "i64")):
         dst = torch.zeros(n, dtype=dt, device="cuda")
