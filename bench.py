@@ -59,6 +59,8 @@ def main():
     ap.add_argument("--serve-load", type=float, default=0.8, help="offered load vs looped-solve capacity")
     ap.add_argument("--delaunay-points", type=int, default=400_000,
                     help="Delaunay leg: genus-3 point-cloud size (0 = skip)")
+    ap.add_argument("--no-cut-leg", action="store_true",
+                    help="skip the graph_cut_fast leg (reference capacity model, full drop-in call)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy three times (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     ap.add_argument("--atomic-probe", action="store_true",
@@ -230,6 +232,15 @@ def main():
                               "frac_of_probe_u32_random": round(push_rate / probe["u32_random_atomics_per_s"], 4)},
                    "counters": counters}
 
+    ceiling = copy_ceiling(torch)
+    if roofline:
+        roofline["copy_ceiling"] = ceiling
+        roofline["frac_of_copy"] = round(roofline["achieved"] / ceiling, 5)
+
+    cutleg = None
+    if not a.no_cut_leg:
+        cutleg = cut_leg(a, torch, np, W, S, dev, stream)
+
     delaunay = None
     if a.delaunay_points > 0:
         delaunay = delaunay_leg(a, torch, np, W, S, dev, stream)
@@ -270,11 +281,14 @@ def main():
                                         "relabels", "bfs_pops", "bfs_levels", "bfs_bu_levels",
                                         "bfs_bu_checks", "bfs_seed_checks", "tail_sub_rounds")},
         "solve_roofline": {"bytes_alg": bytes_alg, "achieved_GBs": round(bytes_alg / t_solve_s / 1e9, 2),
-                           "frac": round(bytes_alg / t_solve_s / 1e9 / HBM_PEAK_GBS, 5)},
+                           "frac": round(bytes_alg / t_solve_s / 1e9 / HBM_PEAK_GBS, 5),
+                           "copy_ceiling_GBs": ceiling,
+                           "frac_of_copy": round(bytes_alg / t_solve_s / 1e9 / ceiling, 5)},
         "roofline": roofline,
         "kernels_ms": {kk: round(vv[0], 3) for kk, vv in kernels.items()} if kernels else None,
         "cpu_baseline": cpu,
         "atomics": atomics,
+        "graph_cut_fast": cutleg,
         "delaunay": delaunay,
         "batched": batched,
         "serving": serving,
@@ -344,6 +358,102 @@ def atomic_probe(torch, n=1 << 26, reps=3):
             ms = e0.elapsed_time(e1) / reps
             out[f"{name}_{kind}_atomics_per_s"] = round(n / (ms * 1e-3), 1)
         assert int(dst[:16].min()) == 2 * reps + 2  # every address got every add
+    return out
+
+
+def copy_ceiling(torch, nbytes=1 << 30, reps=3):
+    """Measured device-copy rate (read + write bytes per second) of a 1 GiB copy."""
+    src = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    return round(gbs, 1)
+
+
+def cut_leg(a, torch, np, W, S, dev, stream):
+    """The drop-in entry end to end: wtv_graph_cut_fast (graph_cut_fast's C-ABI)
+    on the Kuhn 235^3 mesh with the reference's own capacity model: labels 0
+    within one cell of the genus-3 surface, fill_T = 10.  One call builds the
+    reduced graph on the device (capacity_exact_formula), solves it and
+    scatters the labels; it is timed wall-clock.  Checks: the returned labels
+    cut the device-built reduced graph at exactly the returned flow, and that
+    graph equals the CPU restatement's build array for array (timed as the
+    CPU baseline of the build, 1 thread)."""
+    import ctypes as C
+    t = time.time()
+    k = a.k
+    V, T, Nb = S.kuhn_mesh_t(k, seed=a.seed, device=dev)
+    shape = S.genus3(a.seed)
+    lab = (S.tet_sdf_t(V, T, shape).abs() > 1.0 / k).to(torch.uint8)
+    Vh = np.ascontiguousarray(V.cpu().numpy(), dtype=np.float64)
+    labh = np.ascontiguousarray(lab.cpu().numpy())
+    del V, lab
+    N, nv = int(T.shape[0]), int(Vh.shape[0])
+    out_lab = np.empty(N, np.uint8)
+    res = W.WtvCutResult()
+    L = W.lib()
+    gen_s = time.time() - t
+
+    def call():
+        W.check(L.wtv_graph_cut_fast(Vh.ctypes.data, nv, T.data_ptr(), Nb.data_ptr(), N, labh.ctypes.data, 10.0, 16,
+                                     a.period, 0, a.local_steps, out_lab.ctypes.data, C.byref(res), stream))
+
+    call()
+    torch.cuda.synchronize()
+    walls = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        call()
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+    st = W.WtvStats()
+    L.wtv_last_stats(C.byref(st))
+    # certificate on the device-built reduced graph
+    rn = torch.empty((N, 4), dtype=torch.int32, device=dev)
+    rr = torch.empty((N, 4), dtype=torch.int32, device=dev)
+    rt = torch.empty(N, dtype=torch.int32, device=dev)
+    fo = torch.empty(N, dtype=torch.int32, device=dev)
+    info = W.WtvBuildInfo()
+    W.check(L.wtv_build_reduced_graph(Vh.ctypes.data, nv, T.data_ptr(), Nb.data_ptr(), N, labh.ctypes.data, 10.0,
+                                      rn.data_ptr(), rr.data_ptr(), rt.data_ptr(), fo.data_ptr(), C.byref(info),
+                                      stream))
+    torch.cuda.synchronize()
+    F = int(info.free_n)
+    part = torch.from_numpy(out_lab).to(dev)[fo[:F].long()]
+    cut, _ = cut_capacity(torch, rn[:F], rr[:F], rt[:F], part)
+    assert cut == res.raw_flow, f"cut leg: cut capacity {cut} != flow {res.raw_flow}"
+    assert res.adjusted_flow == res.raw_flow + info.base_flow
+    out = {"workload": f"graph_cut_fast (C-ABI) on Kuhn {k}^3, reference capacity model: labels 0 within one cell "
+                       "of the genus-3 surface, fill_T 10; build + solve + label scatter in one call",
+           "tets": N, "free_nodes": F, "wall_s": round(min(walls), 4),
+           "solve_s": round(res.solve_seconds, 4), "build_upload_s": round(res.build_upload_seconds, 4),
+           "native_total_s": round(res.native_total_seconds, 4), "t_solve_ms_device": round(st.t_solve_ms, 3),
+           "tets_per_s": round(N / min(walls), 1), "raw_flow": res.raw_flow, "adjusted_flow": res.adjusted_flow,
+           "global_relabels": res.global_relabels, "certificate": "flow == integer cut capacity of the built graph",
+           "gen_seconds": round(gen_s, 1)}
+    if not a.no_cpu:
+        import oracle
+        Th = T.cpu().numpy()
+        Nh = Nb.cpu().numpy()
+        tc = time.perf_counter()
+        r = oracle.reduce(Vh, Th, Nh, labh, 10.0)
+        cpu_s = time.perf_counter() - tc
+        assert len(r.term) == F
+        assert np.array_equal(r.nbr, rn[:F].cpu().numpy()), "cut leg: reduced neighbours differ from the CPU build"
+        assert np.array_equal(r.res.view(np.int32), rr[:F].cpu().numpy()), "cut leg: capacities differ"
+        assert np.array_equal(r.term, rt[:F].cpu().numpy()), "cut leg: terminals differ"
+        out["cpu_build_s"] = round(cpu_s, 3)
+        out["cpu_build"] = "oracle/wtv_oracle.c restatement of the reduced-graph build, 1 thread; arrays equal"
+    log(f"[bench] graph_cut_fast: {N} tets, {F} free, wall {min(walls) * 1e3:.1f} ms "
+        f"(solve {res.solve_seconds * 1e3:.1f} ms, build {res.build_upload_seconds * 1e3:.1f} ms)")
     return out
 
 

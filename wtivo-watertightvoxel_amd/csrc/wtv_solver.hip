@@ -471,6 +471,7 @@ __global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint
 // the frontier of the level it stops at in f[d & 1] with its count in the
 // counter ring, exactly as a sequence of k_bfs_level launches would.
 constexpr uint32_t kTailMax = 1024;
+constexpr double kRelabelCost = 1.0;
 // The one-workgroup kernels run 1024 threads: a list of up to 1024 items
 // takes one pass per level or phase.
 constexpr int kTailTPB = 1024;
@@ -1080,6 +1081,13 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
 
     Event ev_t0, ev_t1, ev_b0, ev_b1;
     double bfs_ms = 0.0;
+    // Cost guard of the adaptive relabel trigger: a global relabel may run
+    // once the time since the last one reaches kRelabelCost times that one's
+    // duration, so relabels take at most about 1/(1+kRelabelCost) of a solve
+    // (a relabel costs ~1 ms on a soft-seed 78M graph, but ~15 ms where the
+    // BFS sweeps most of the graph, as in the reference capacity model).
+    double last_gr_ms = 0.0;
+    auto last_gr_end = Clock::now();
     KernelTimer kt;
     kt.on = prm.profile != 0;
     kt.s = s;
@@ -1177,6 +1185,8 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         float ms = 0.f;
         ck(hipEventElapsedTime(&ms, ev_b0.e, ev_b1.e), "event time");
         bfs_ms += ms;
+        last_gr_ms = ms;
+        last_gr_end = Clock::now();
         ++st.global_relabels;
         if (trace) read_ctl("trace");  // the level count, current
         if (trace)
@@ -1266,9 +1276,14 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         relabels_now = relabels_at_gr = stat_sum(*hctl, kStRelabels);
         active_at_gr = active;
     };
+    auto affordable = [&]() -> bool {
+        return std::chrono::duration<double, std::milli>(Clock::now() - last_gr_end).count() >=
+               kRelabelCost * last_gr_ms;
+    };
     auto work_trigger = [&]() -> bool {
         return alpha > 0.0 &&
-               static_cast<double>(relabels_now - relabels_at_gr) > alpha * static_cast<double>(active_at_gr) + 1024.0;
+               static_cast<double>(relabels_now - relabels_at_gr) > alpha * static_cast<double>(active_at_gr) + 1024.0 &&
+               affordable();
     };
     auto check_device_error = [&](const Ctl& c) {
         if (c.error) {
@@ -1287,11 +1302,14 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             // ---- one workgroup, sub-rounds back to back
             unsigned long long budget = ~0ull;
             if (alpha > 0.0) {
-                const double left = alpha * static_cast<double>(active_at_gr) + 1024.0 -
-                                    static_cast<double>(relabels_now - relabels_at_gr);
+                double left = alpha * static_cast<double>(active_at_gr) + 1024.0 -
+                              static_cast<double>(relabels_now - relabels_at_gr);
                 if (left <= 0.0) {
-                    do_global_relabel();
-                    continue;
+                    if (affordable()) {
+                        do_global_relabel();
+                        continue;
+                    }
+                    left = alpha * static_cast<double>(active_at_gr) + 1024.0;  // one more interval first
                 }
                 budget = static_cast<unsigned long long>(left) + 1ull;
             }
