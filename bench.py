@@ -19,6 +19,16 @@ the integer capacity of the returned cut (max-flow/min-cut certificate);
 and (unless --no-cpu) the CPU oracle's Boykov-Kolmogorov solve of the same
 graph returns the same flow and labels.
 
+Further legs, each reported in its own object of the line:
+  * graph_cut_fast: the drop-in call (build + solve + scatter) on the same mesh
+    with the reference capacity model, certificate-checked, with the CPU
+    restatement of the build timed and compared array for array;
+  * delaunay: the Qhull mesh of a 400k-point genus-3 cloud, checked against BK;
+  * batched: 256 uneven graphs, looped vs one autotuned batched solve;
+  * serving: Poisson-arriving requests into the native server, p50/p99;
+  * atomics: probe rates (scattered, coalesced, one address) and rocprof counters;
+  * roofline: the dominant kernel against 8 TB/s and the measured copy ceiling.
+
 Prints one JSON line (see the keys below).
 """
 from __future__ import annotations
