@@ -306,6 +306,27 @@ def test_batch_soft_graphs_vs_bk(orc, torch_cuda):
         assert np.array_equal(part[int(off[i]):int(off[i + 1])], b.partition)
 
 
+def test_batch_delaunay_and_kuhn_graphs_vs_bk(orc, torch_cuda):
+    """Irregular (Delaunay) and regular (Kuhn) graphs of uneven size in one batch:
+    every graph's flow and labels equal BK's on that graph alone."""
+    graphs = []
+    for i, npts in enumerate((3_000, 20_000, 45_000)):
+        P, shape, res = S.genus3_point_cloud(npts, 70 + i)
+        m = S.delaunay_mesh(P, morton=bool(i % 2))
+        lab = S.labels_from_sdf(S.tet_centroids_sdf(m, shape), 0.02, seed=71 + i)
+        g = S.soft_graph(m, lab, shape, float(5 + 10 * i), 1.0 / res)
+        graphs.append((g.nbr, g.res, g.term))
+    _, _, g = _c0_soft(20, 72)
+    graphs.append((g.nbr, g.res, g.term))
+    off, flows, part, r_out, t_out, st = gpu_batch(torch_cuda, graphs)
+    for i, (nbr, res, term) in enumerate(graphs):
+        a, b = int(off[i]), int(off[i + 1])
+        sol = orc.bk_solve(nbr, res, term)
+        assert flows[i] == sol.raw_flow, i
+        assert np.array_equal(part[a:b], sol.partition), i
+        assert orc.certificate(nbr, res, term, r_out[a:b], t_out[a:b], part[a:b], int(flows[i])) == 0
+
+
 def test_batch_rejects_bad_offsets_and_ids(torch_cuda):
     nbr, res, term = S.random_graph(40, 9)
     dn = torch_cuda.from_numpy(nbr).cuda()
@@ -419,6 +440,58 @@ def test_graph_cut_fast_device_build_large(orc, torch_cuda):
     out = G.graph_cut_fast(m.V, torch_cuda.from_numpy(m.tets).cuda(), torch_cuda.from_numpy(m.nbrs).cuda(), lab, 10.0)
     assert out[1] == sol.raw_flow + r.info["base_flow"] and out[2] == sol.raw_flow
     assert np.array_equal(out[0], want)
+
+
+def _reference_model_cut(orc, torch, V, tets, nbrs, lab):
+    """graph_cut_fast through the pybind module and the device build through
+    the C-ABI, both against the oracle's build + BK + scatter."""
+    import ctypes as C
+    import wtivo_gpupr as G
+    r = orc.reduce(V, tets, nbrs, lab, 10.0)
+    sol = orc.bk_solve(r.nbr, r.res, r.term)
+    want = orc.scatter(lab, r.free_old, sol.partition)
+    dT = torch.from_numpy(np.ascontiguousarray(tets, np.int32)).cuda()
+    dN = torch.from_numpy(np.ascontiguousarray(nbrs, np.int32)).cuda()
+    out = G.graph_cut_fast(V, dT, dN, lab, 10.0)
+    assert out[2] == sol.raw_flow and out[1] == sol.raw_flow + r.info["base_flow"]
+    assert np.array_equal(out[0], want)
+    N = tets.shape[0]
+    rn = torch.empty((N, 4), dtype=torch.int32, device="cuda")
+    rr = torch.empty_like(rn)
+    rt = torch.empty(N, dtype=torch.int32, device="cuda")
+    fo = torch.empty_like(rt)
+    info = W.WtvBuildInfo()
+    Vc = np.ascontiguousarray(V, np.float64)
+    lc = np.ascontiguousarray(lab, np.uint8)
+    W.check(W.lib().wtv_build_reduced_graph(Vc.ctypes.data, Vc.shape[0], dT.data_ptr(), dN.data_ptr(), N,
+                                            lc.ctypes.data, 10.0, rn.data_ptr(), rr.data_ptr(), rt.data_ptr(),
+                                            fo.data_ptr(), C.byref(info), torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    F = int(info.free_n)
+    assert F == len(r.term)
+    assert np.array_equal(rn[:F].cpu().numpy(), r.nbr)
+    assert np.array_equal(rr[:F].cpu().numpy().view(np.uint32), r.res)
+    assert np.array_equal(rt[:F].cpu().numpy(), r.term)
+    assert np.array_equal(fo[:F].cpu().numpy().view(np.uint32), np.asarray(r.free_old, np.uint32))
+
+
+def test_reference_model_cut_on_delaunay(orc, torch_cuda):
+    """The reference capacity model on an irregular mesh: labels 0 within one
+    voxel of the genus-3 surface of a Delaunay point cloud."""
+    P, shape, res = S.genus3_point_cloud(40_000, 81)
+    m = S.delaunay_mesh(P)
+    lab = S.band_labels(S.tet_centroids_sdf(m, shape), 1.0 / res)
+    _reference_model_cut(orc, torch_cuda, m.V, m.tets, m.nbrs, lab)
+
+
+def test_reference_model_cut_kuhn96(orc, torch_cuda):
+    """The reference capacity model at 5.3M tets (the bench's drop-in leg at a
+    size BK finishes in seconds)."""
+    k = 96
+    m = S.kuhn_mesh(k, seed=20260001)
+    shape = S.genus3(20260001)
+    lab = S.band_labels(S.tet_centroids_sdf(m, shape), 1.0 / k)
+    _reference_model_cut(orc, torch_cuda, m.V, m.tets, m.nbrs, lab)
 
 
 def test_config4_integer_stress_full_size(orc, torch_cuda):

@@ -23,10 +23,11 @@
 // (h(v) <= h(u) + 1 on every residual arc) with any number of threads, which
 // is what makes the final BFS partition the unique minimal sink set.
 //
-// Worklist appends are wave64-aggregated (ballot + mbcnt prefix, one atomic
-// per wave).  Counters live in device memory and are zeroed by the kernel
-// before the one that appends to them, so the host never waits per round:
-// it reads the active count once per macro round (local_steps sub-rounds).
+// List appends are staged in LDS (ballot + mbcnt ranks, one LDS atomic per
+// wave) and reach the global list with one device atomic per workgroup flush.
+// Counters live in device memory and are zeroed by the kernel before the one
+// that appends to them, so the host never waits per round: it reads the
+// active count once per macro round (local_steps sub-rounds).
 //
 // Global relabel = level-synchronous BFS from the sink over residual arcs
 // (gpu_push_relabel_fast.cu:85-147,360-402), with reverse slots precomputed
@@ -1193,7 +1194,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             std::fprintf(stderr, "[wtv] global relabel %llu: levels=%u %.3f ms\n",
                          static_cast<unsigned long long>(st.global_relabels), hctl->bfs_levels, ms);
     };
-    auto build_active = [&]() -> uint32_t {
+    auto filter_active = [&]() -> uint32_t {
         const uint32_t p = static_cast<uint32_t>(pos & 1u);
         k_filter_active<kB><<<grid, tpb, 0, s>>>(n, p, wl[p], wl[p ^ 1u], excess, height, gid, gn, ctl);
         ck(hipGetLastError(), "filter active");
@@ -1204,7 +1205,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
 
     global_relabel();
     check_err("global relabel");
-    uint32_t active = build_active();
+    uint32_t active = filter_active();
 
     int macro_period = 0;
     if (period > 0) macro_period = std::max(1, (period + local_steps - 1) / local_steps);
@@ -1272,7 +1273,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     uint64_t relabels_now = relabels_at_gr;
     auto do_global_relabel = [&]() {
         global_relabel();
-        active = build_active();
+        active = filter_active();
         relabels_now = relabels_at_gr = stat_sum(*hctl, kStRelabels);
         active_at_gr = active;
     };
