@@ -87,7 +87,9 @@ typedef struct wtv_params {
     int32_t global_relabel_period; /* reference default 1024 */
     int32_t local_steps;           /* reference default 8    */
     uint64_t max_rounds;           /* reference default 2e6  */
-    int32_t wg_size;               /* 0 = default (256)      */
+    int32_t wg_size;               /* workgroup size of the list kernels (push, relabel,
+                                      BFS level, active filter): 0 (= 256), 256, 512 or
+                                      1024; other values fail with WTV_EINVAL */
     int32_t grid_wgs;              /* 0 = default            */
     double time_limit_s;           /* 0 = 240 s (reference)  */
     int32_t profile;               /* 1 = time every kernel with HIP events */

@@ -5,7 +5,7 @@ import wtv_autotune as AT
 
 
 def test_coordinate_descent_finds_separable_minimum():
-    best_vals = {"relabel_alpha": 16.0, "local_steps": 32, "tail_cap": 512, "grid_wgs": 2048}
+    best_vals = {"relabel_alpha": 16.0, "local_steps": 32, "tail_cap": 512, "grid_wgs": 2048, "wg_size": 1024}
 
     def run(p):
         cost = sum(abs(AT.SPACE[k].index(p[k]) - AT.SPACE[k].index(best_vals[k])) for k in p)

@@ -148,11 +148,34 @@ def test_stress_integer_shell(orc, torch_cuda):
 
 @pytest.mark.parametrize("kw", [dict(local_steps=0), dict(local_steps=100), dict(global_relabel_period=-5),
                                 dict(global_relabel_period=1, local_steps=1), dict(grid_wgs=7),
-                                dict(max_rounds=0)])
+                                dict(max_rounds=0), dict(wg_size=512), dict(wg_size=1024),
+                                dict(wg_size=1024, grid_wgs=3, local_steps=2)])
 def test_clamped_parameters_same_answer(orc, torch_cuda, kw):
     nbr, res, term = S.random_graph(2000, 77, cap_max=30)
     a = orc.bk_solve(nbr, res, term)
     assert_same(orc, nbr, res, term, gpu_solve(torch_cuda, nbr, res, term, **kw), a.raw_flow, a.partition)
+
+
+@pytest.mark.parametrize("wg", [512, 1024])
+def test_workgroup_size_same_answer_large(orc, torch_cuda, wg):
+    """Every list kernel at 512 and 1024 threads per workgroup, in single and
+    batched mode, on graphs large enough for the queue flushes to trigger."""
+    _, _, g = _c0_soft(48, 90 + wg)
+    a = orc.bk_solve(g.nbr, g.res, g.term)
+    assert_same(orc, g.nbr, g.res, g.term, gpu_solve(torch_cuda, g.nbr, g.res, g.term, wg_size=wg, tail_cap=-1),
+                a.raw_flow, a.partition)
+    graphs = [(g.nbr, g.res, g.term), S.random_graph(3000, wg, cap_max=40)]
+    off, flows, part, _, _, _ = gpu_batch(torch_cuda, graphs, wg_size=wg)
+    for i, (nbr, res, term) in enumerate(graphs):
+        sol = orc.bk_solve(nbr, res, term)
+        assert flows[i] == sol.raw_flow and np.array_equal(part[int(off[i]):int(off[i + 1])], sol.partition)
+
+
+def test_workgroup_size_rejected(torch_cuda):
+    nbr, res, term = S.random_graph(50, 3)
+    for wg in (64, 300, 2048):
+        with pytest.raises(W.WtvError, match="wg_size"):
+            gpu_solve(torch_cuda, nbr, res, term, wg_size=wg)
 
 
 def test_deterministic_labels_and_flow(torch_cuda):

@@ -45,6 +45,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <sstream>
+#include <type_traits>
 #include <vector>
 
 #include "wtv_internal.h"
@@ -132,7 +133,7 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
 __device__ __forceinline__ void flush_stat(Ctl* ctl, int f, unsigned long long x) {
     x = wave_sum(x);
     if (__lane_id() == 0 && x) {
-        const uint32_t w = blockIdx.x * (kTPB / 64u) + (threadIdx.x >> 6);
+        const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
         atom_add(&ctl->stat[f][w % kStatSlots], x);
     }
 }
@@ -144,17 +145,19 @@ __device__ __forceinline__ void flush_stat(Ctl* ctl, int f, unsigned long long x
 // flushed lazily -- when the next iteration could overflow it, and at kernel
 // end -- because device-scope atomics on one address serialise at ~12 ns each
 // (~83M/s) on MI355X: one flush per iteration of a 78M-vertex sweep would
-// cost milliseconds.  16 KiB of LDS per workgroup.
-constexpr int kQCap = 4096;
-static_assert(kQCap >= 5 * kTPB, "one iteration appends at most 5 items per thread");
+// cost milliseconds.  At least 16 KiB of LDS per workgroup; the capacity holds
+// five appends per thread (20 KiB at 1024 threads).
+template <int TPB>
 struct WgQueue {
+    static constexpr int kCap = 5 * TPB > 4096 ? 5 * TPB : 4096;
     uint32_t cnt;
     uint32_t base;
-    uint32_t buf[kQCap];
+    uint32_t buf[kCap];
 };
 
 // All active lanes of the wave must call this (pred may be false).
-__device__ __forceinline__ void q_push(WgQueue& q, bool pred, uint32_t val) {
+template <class Q>
+__device__ __forceinline__ void q_push(Q& q, bool pred, uint32_t val) {
     const uint64_t m = __ballot(pred);
     if (m == 0) return;
     const uint32_t rank = lane_rank(m);
@@ -166,7 +169,8 @@ __device__ __forceinline__ void q_push(WgQueue& q, bool pred, uint32_t val) {
 }
 
 // Every thread of the workgroup must call this the same number of times.
-__device__ __forceinline__ void q_flush(WgQueue& q, uint32_t* list, uint32_t* gcnt) {
+template <class Q>
+__device__ __forceinline__ void q_flush(Q& q, uint32_t* list, uint32_t* gcnt) {
     __syncthreads();
     const uint32_t c = q.cnt;
     __syncthreads();
@@ -176,7 +180,7 @@ __device__ __forceinline__ void q_flush(WgQueue& q, uint32_t* list, uint32_t* gc
     }
     __syncthreads();
     const uint32_t b = q.base;
-    for (uint32_t i = threadIdx.x; i < c; i += kTPB) list[b + i] = q.buf[i];
+    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) list[b + i] = q.buf[i];
     __syncthreads();
 }
 
@@ -184,23 +188,25 @@ __device__ __forceinline__ void q_flush(WgQueue& q, uint32_t* list, uint32_t* gc
 // iteration (at most `room` appends) could overflow.  Uniform: every thread
 // reads the count between two barriers.  Every thread of the workgroup must
 // call this the same number of times, and call q_flush once at the end.
-__device__ __forceinline__ void q_flush_if(WgQueue& q, uint32_t* list, uint32_t* gcnt, uint32_t room) {
+template <class Q>
+__device__ __forceinline__ void q_flush_if(Q& q, uint32_t* list, uint32_t* gcnt, uint32_t room) {
     __syncthreads();
     const uint32_t c = q.cnt;
     __syncthreads();
-    if (c + room > static_cast<uint32_t>(kQCap)) {
+    if (c + room > static_cast<uint32_t>(Q::kCap)) {
         if (threadIdx.x == 0) {
             q.base = atom_add(gcnt, c);
             q.cnt = 0u;
         }
         __syncthreads();
         const uint32_t b = q.base;
-        for (uint32_t i = threadIdx.x; i < c; i += kTPB) list[b + i] = q.buf[i];
+        for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) list[b + i] = q.buf[i];
         __syncthreads();
     }
 }
 
-__device__ __forceinline__ void q_init(WgQueue& q) {
+template <class Q>
+__device__ __forceinline__ void q_init(Q& q) {
     if (threadIdx.x == 0) q.cnt = 0u;
     __syncthreads();
 }
@@ -257,7 +263,7 @@ __global__ void __launch_bounds__(kTPB) k_prepare(uint32_t n, const int4* __rest
                                                   const int32_t* __restrict__ term, uint8_t* __restrict__ rslot,
                                                   long long* __restrict__ excess, uint32_t* __restrict__ mark,
                                                   uint32_t* __restrict__ wl0, Ctl* ctl) {
-    __shared__ WgQueue q;
+    __shared__ WgQueue<kTPB> q;
     q_init(q);
     const uint32_t stride = gridDim.x * kTPB;
     // uniform trip count per workgroup: every thread reaches each queue flush
@@ -315,12 +321,12 @@ __global__ void k_bfs_begin(Ctl* ctl, uint32_t p) {
 // flushes once per kSeedUnroll * kTPB vertices.  Level 2 goes to `fout` with
 // its count in bfs_cnt[2] (zeroed by k_bfs_begin).
 constexpr int kSeedUnroll = 4;
-static_assert(kSeedUnroll * kTPB <= kQCap, "queue holds one append per vertex per iteration");
+static_assert(kSeedUnroll * kTPB <= WgQueue<kTPB>::kCap, "queue holds one append per vertex per iteration");
 __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __restrict__ term,
                                                    const int4* __restrict__ nbr4, const uint32_t* __restrict__ res,
                                                    uint32_t* __restrict__ height, uint32_t* __restrict__ fout,
                                                    uint32_t first, Ctl* ctl) {
-    __shared__ WgQueue q;
+    __shared__ WgQueue<kTPB> q;
     q_init(q);
     const uint32_t span = kTPB * kSeedUnroll;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * span;
@@ -404,15 +410,16 @@ __device__ __forceinline__ void td_claims(bool valid, uint32_t u, uint32_t d, co
 // all vertices when the frontier is large: each unreached v checks its own
 // arcs v->u for a residual arc into level d, needing neither the reverse slot
 // nor a CAS.
-__global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint32_t bu_threshold,
+template <int TPB>
+__global__ void __launch_bounds__(TPB) k_bfs_level(uint32_t n, uint32_t d, uint32_t bu_threshold,
                                                     const uint32_t* __restrict__ fin, uint32_t* __restrict__ fout,
                                                     const int4* __restrict__ nbr4, const uint8_t* __restrict__ rslot,
                                                     const uint32_t* __restrict__ res, uint32_t* height, Ctl* ctl) {
-    __shared__ WgQueue q;
+    __shared__ WgQueue<TPB> q;
     // The first frontier entry of this thread is loaded before the count is
     // known (the buffer holds n entries; entries past the count are ignored),
     // so the two loads overlap instead of forming two round trips.
-    const uint32_t i0 = blockIdx.x * kTPB + threadIdx.x;
+    const uint32_t i0 = blockIdx.x * TPB + threadIdx.x;
     const uint32_t u0 = i0 < n ? fin[i0] : 0u;
     q_init(q);
     const uint32_t cnt = ctl->bfs_cnt[d % 3u];
@@ -425,10 +432,10 @@ __global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint
         // every frontier entry is expanded once: one add per level
         else if (cnt) atom_add(&ctl->stat[kStBfsPops][d % kStatSlots], static_cast<unsigned long long>(cnt));
     }
-    const uint32_t stride = gridDim.x * kTPB;
+    const uint32_t stride = gridDim.x * TPB;
     unsigned long long work = 0;
     if (bottom_up) {
-        for (uint32_t base = blockIdx.x * kTPB; base < n; base += stride) {
+        for (uint32_t base = blockIdx.x * TPB; base < n; base += stride) {
             const uint32_t v = base + threadIdx.x;
             bool found = false;
             if (v < n && height[v] == kSent) {
@@ -446,21 +453,21 @@ __global__ void __launch_bounds__(kTPB) k_bfs_level(uint32_t n, uint32_t d, uint
                 if (found) height[v] = d + 1u;
             }
             q_push(q, found, v);
-            q_flush_if(q, fout, out_cnt, kTPB);
+            q_flush_if(q, fout, out_cnt, TPB);
         }
         q_flush(q, fout, out_cnt);
         flush_stat(ctl, kStBuChecks, work);
     } else {
-        for (uint32_t base = blockIdx.x * kTPB; base < cnt; base += stride) {
+        for (uint32_t base = blockIdx.x * TPB; base < cnt; base += stride) {
             const uint32_t i = base + threadIdx.x;
             const bool valid = i < cnt;
-            const uint32_t u = !valid ? 0u : (base == blockIdx.x * kTPB ? u0 : fin[i]);
+            const uint32_t u = !valid ? 0u : (base == blockIdx.x * TPB ? u0 : fin[i]);
             int4 nb;
             bool claim[4];
             td_claims(valid, u, d, nbr4, rslot, res, height, nb, claim);
 #pragma unroll
             for (int k = 0; k < 4; ++k) q_push(q, claim[k], static_cast<uint32_t>(slot_of(nb, k)));
-            q_flush_if(q, fout, out_cnt, 4 * kTPB);
+            q_flush_if(q, fout, out_cnt, 4 * TPB);
         }
         q_flush(q, fout, out_cnt);
     }
@@ -531,19 +538,19 @@ __global__ void __launch_bounds__(kTailTPB) k_bfs_tail(uint32_t d0, uint32_t max
 // filtered by the new heights.  Entries are distinct, so survivors are too.
 // Unreached vertices keep the sentinel height, which every reader treats as
 // the reference's n+1 (gpu_push_relabel_fast.cu:141-147).
-template <bool kB>
-__global__ void __launch_bounds__(kTPB) k_filter_active(uint32_t n, uint32_t p, const uint32_t* __restrict__ src,
+template <bool kB, int TPB>
+__global__ void __launch_bounds__(TPB) k_filter_active(uint32_t n, uint32_t p, const uint32_t* __restrict__ src,
                                                         uint32_t* __restrict__ dst,
                                                         const long long* __restrict__ excess,
                                                         const uint32_t* __restrict__ height,
                                                         const uint32_t* __restrict__ gid,
                                                         const uint32_t* __restrict__ gn, Ctl* ctl) {
-    __shared__ WgQueue q;
+    __shared__ WgQueue<TPB> q;
     q_init(q);
     const uint32_t cnt = ctl->wl_cnt[p];
     uint32_t* out_cnt = &ctl->wl_cnt[p ^ 1u];
-    const uint32_t stride = gridDim.x * kTPB;
-    for (uint32_t base = blockIdx.x * kTPB; base < cnt; base += stride) {
+    const uint32_t stride = gridDim.x * TPB;
+    for (uint32_t base = blockIdx.x * TPB; base < cnt; base += stride) {
         const uint32_t i = base + threadIdx.x;
         bool a = false;
         uint32_t v = 0;
@@ -552,7 +559,7 @@ __global__ void __launch_bounds__(kTPB) k_filter_active(uint32_t n, uint32_t p, 
             a = height[v] < src_h<kB>(v, n, gid, gn) && excess[v] > 0;
         }
         q_push(q, a, v);
-        q_flush_if(q, dst, out_cnt, kTPB);
+        q_flush_if(q, dst, out_cnt, TPB);
     }
     q_flush(q, dst, out_cnt);
 }
@@ -692,22 +699,22 @@ __device__ __forceinline__ bool relabel_vertex(uint32_t v, uint32_t n, const int
 }
 
 // Push phase of sub-round with parity p (heights read-only).
-template <bool kB>
-__global__ void __launch_bounds__(kTPB) k_push(uint32_t n, uint32_t p, const uint32_t* __restrict__ wl,
+template <bool kB, int TPB>
+__global__ void __launch_bounds__(TPB) k_push(uint32_t n, uint32_t p, const uint32_t* __restrict__ wl,
                                                uint32_t* __restrict__ rl, const int4* __restrict__ nbr4,
                                                const uint8_t* __restrict__ rslot, uint32_t* res,
                                                int32_t* __restrict__ term, long long* excess,
                                                const uint32_t* __restrict__ height, uint32_t* mark,
                                                const uint32_t* __restrict__ gid, const uint32_t* __restrict__ gn,
                                                unsigned long long* __restrict__ gflow, Ctl* ctl) {
-    __shared__ WgQueue q;
+    __shared__ WgQueue<TPB> q;
     q_init(q);
     const uint32_t cnt = ctl->wl_cnt[p];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->wl_cnt[p ^ 1u] = 0u;
     uint32_t* rl_cnt = &ctl->rl_cnt[p];
-    const uint32_t stride = gridDim.x * kTPB;
+    const uint32_t stride = gridDim.x * TPB;
     unsigned long long visits = 0, pushes = 0, sinkf = 0;
-    for (uint32_t base = blockIdx.x * kTPB; base < cnt; base += stride) {
+    for (uint32_t base = blockIdx.x * TPB; base < cnt; base += stride) {
         const uint32_t i = base + threadIdx.x;
         const bool valid = i < cnt;
         const uint32_t v = valid ? wl[i] : 0u;
@@ -721,7 +728,7 @@ __global__ void __launch_bounds__(kTPB) k_push(uint32_t n, uint32_t p, const uin
         q_push(q, first, v);
 #pragma unroll
         for (int k = 0; k < 4; ++k) q_push(q, cand[k], static_cast<uint32_t>(slot_of(nb, k)));
-        q_flush_if(q, rl, rl_cnt, 5 * kTPB);
+        q_flush_if(q, rl, rl_cnt, 5 * TPB);
     }
     q_flush(q, rl, rl_cnt);
     flush_stat(ctl, kStVisits, visits);
@@ -731,21 +738,21 @@ __global__ void __launch_bounds__(kTPB) k_push(uint32_t n, uint32_t p, const uin
 
 // Relabel phase of sub-round with parity p: recompute heights of every vertex
 // that was active or received flow, and rebuild the active list.
-template <bool kB>
-__global__ void __launch_bounds__(kTPB) k_relabel(uint32_t n, uint32_t p, const uint32_t* __restrict__ rl,
+template <bool kB, int TPB>
+__global__ void __launch_bounds__(TPB) k_relabel(uint32_t n, uint32_t p, const uint32_t* __restrict__ rl,
                                                   uint32_t* __restrict__ wl_next, const int4* __restrict__ nbr4,
                                                   const uint32_t* __restrict__ res, const int32_t* __restrict__ term,
                                                   const long long* __restrict__ excess, uint32_t* height,
                                                   uint32_t* __restrict__ mark, const uint32_t* __restrict__ gid,
                                                   const uint32_t* __restrict__ gn, Ctl* ctl) {
-    __shared__ WgQueue q;
+    __shared__ WgQueue<TPB> q;
     q_init(q);
     const uint32_t cnt = ctl->rl_cnt[p];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rl_cnt[p ^ 1u] = 0u;
     uint32_t* wl_cnt = &ctl->wl_cnt[p ^ 1u];
-    const uint32_t stride = gridDim.x * kTPB;
+    const uint32_t stride = gridDim.x * TPB;
     unsigned long long relabels = 0;
-    for (uint32_t base = blockIdx.x * kTPB; base < cnt; base += stride) {
+    for (uint32_t base = blockIdx.x * TPB; base < cnt; base += stride) {
         const uint32_t i = base + threadIdx.x;
         bool act = false;
         uint32_t v = 0;
@@ -754,7 +761,7 @@ __global__ void __launch_bounds__(kTPB) k_relabel(uint32_t n, uint32_t p, const 
             act = relabel_vertex<kB>(v, n, nbr4, res, term, excess, height, mark, gid, gn, ctl, relabels);
         }
         q_push(q, act, v);
-        q_flush_if(q, wl_next, wl_cnt, kTPB);
+        q_flush_if(q, wl_next, wl_cnt, TPB);
     }
     q_flush(q, wl_next, wl_cnt);
     flush_stat(ctl, kStRelabels, relabels);
@@ -1002,6 +1009,15 @@ Layout layout_for(uint32_t n, uint32_t G) {
     return L;
 }
 
+// Calls f with std::integral_constant<int, wg> for the supported list-kernel
+// workgroup sizes (checked by the caller).
+template <class F>
+void launch_wg(int wg, F&& f) {
+    if (wg == 1024) f(std::integral_constant<int, 1024>{});
+    else if (wg == 512) f(std::integral_constant<int, 512>{});
+    else f(std::integral_constant<int, 256>{});
+}
+
 struct BatchArgs {
     uint32_t G;
     const uint64_t* h_off;  // G+1 node offsets (host)
@@ -1024,8 +1040,12 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     int period = prm.global_relabel_period < 0 ? 0 : prm.global_relabel_period;
     uint64_t max_rounds = prm.max_rounds ? prm.max_rounds : 2000000ull;
     const double time_limit = prm.time_limit_s > 0 ? prm.time_limit_s : 240.0;
-    const int tpb = kTPB;
+    const int tpb = kTPB;  // sweeps over all vertices
     int grid = prm.grid_wgs > 0 ? prm.grid_wgs : 1024;
+    // workgroup size of the list kernels (push, relabel, BFS level, filter)
+    const int wg = prm.wg_size == 0 ? 256 : prm.wg_size;
+    if (wg != 256 && wg != 512 && wg != 1024)
+        fail(WTV_EINVAL, std::string(kPrefix) + "wg_size must be 0 (256), 256, 512 or 1024");
     const int sweep_grid = static_cast<int>(std::min<uint64_t>((static_cast<uint64_t>(n) + tpb - 1) / tpb, 8192));
     // a level whose frontier holds more than n/32 vertices expands bottom-up
     const uint32_t bu_threshold = std::max<uint32_t>(n / 32u, 4096u);
@@ -1145,8 +1165,11 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         auto enqueue_batch = [&]() -> Mark {
             for (uint32_t b = 0; b < batch; ++b, ++d) {
                 kt.begin(KernelTimer::kLevel);
-                k_bfs_level<<<grid, tpb, 0, s>>>(n, d, bu_threshold, bf[d & 1u], bf[(d + 1u) & 1u], nbr4, rslot,
-                                                 d_res, height, ctl);
+                launch_wg(wg, [&](auto t) {
+                    constexpr int T = decltype(t)::value;
+                    k_bfs_level<T><<<grid, T, 0, s>>>(n, d, bu_threshold, bf[d & 1u], bf[(d + 1u) & 1u], nbr4,
+                                                      rslot, d_res, height, ctl);
+                });
                 kt.end();
             }
             ck(hipGetLastError(), "bfs expand");
@@ -1196,7 +1219,10 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     };
     auto filter_active = [&]() -> uint32_t {
         const uint32_t p = static_cast<uint32_t>(pos & 1u);
-        k_filter_active<kB><<<grid, tpb, 0, s>>>(n, p, wl[p], wl[p ^ 1u], excess, height, gid, gn, ctl);
+        launch_wg(wg, [&](auto t) {
+            constexpr int T = decltype(t)::value;
+            k_filter_active<kB, T><<<grid, T, 0, s>>>(n, p, wl[p], wl[p ^ 1u], excess, height, gid, gn, ctl);
+        });
         ck(hipGetLastError(), "filter active");
         ++pos;
         read_ctl("active count");
@@ -1226,12 +1252,18 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             for (int j = 0; j < local_steps; ++j) {
                 const uint32_t p = static_cast<uint32_t>((pos0 + static_cast<uint64_t>(j)) & 1u);
                 kt.begin(KernelTimer::kPush);
-                k_push<kB><<<grid, tpb, 0, s>>>(n, p, wl[p], rl, nbr4, rslot, d_res, d_term, excess, height, mark,
-                                                gid, gn, gflow, ctl);
+                launch_wg(wg, [&](auto t) {
+                    constexpr int T = decltype(t)::value;
+                    k_push<kB, T><<<grid, T, 0, s>>>(n, p, wl[p], rl, nbr4, rslot, d_res, d_term, excess, height,
+                                                     mark, gid, gn, gflow, ctl);
+                });
                 kt.end();
                 kt.begin(KernelTimer::kRelabel);
-                k_relabel<kB><<<grid, tpb, 0, s>>>(n, p, rl, wl[p ^ 1u], nbr4, d_res, d_term, excess, height, mark,
-                                                   gid, gn, ctl);
+                launch_wg(wg, [&](auto t) {
+                    constexpr int T = decltype(t)::value;
+                    k_relabel<kB, T><<<grid, T, 0, s>>>(n, p, rl, wl[p ^ 1u], nbr4, d_res, d_term, excess, height,
+                                                        mark, gid, gn, ctl);
+                });
                 kt.end();
             }
         };

@@ -7,6 +7,7 @@ the schedule but never the answer:
                       host checks, and the replayed graph's length)
   * tail_cap       -- active-list size handed to the one-workgroup discharge
   * grid_wgs       -- workgroups per discharge / BFS launch
+  * wg_size        -- threads per workgroup of those launches
 Every candidate must reproduce the reference flows exactly (flow and labels do
 not depend on the schedule; a mismatch raises).  Results can be cached in a
 JSON file keyed by a workload signature.
@@ -17,12 +18,13 @@ import json
 import os
 import time
 
-DEFAULT = {"relabel_alpha": 4.0, "local_steps": 8, "tail_cap": 2048, "grid_wgs": 1024}
+DEFAULT = {"relabel_alpha": 4.0, "local_steps": 8, "tail_cap": 2048, "grid_wgs": 1024, "wg_size": 256}
 SPACE = {
     "relabel_alpha": [2.0, 4.0, 8.0, 16.0, 32.0],
     "local_steps": [4, 8, 16, 32],
     "tail_cap": [-1, 512, 2048, 8192],
     "grid_wgs": [512, 1024, 2048],
+    "wg_size": [256, 512, 1024],
 }
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_params.json")
 
