@@ -100,8 +100,9 @@ typedef struct wtv_params {
     double relabel_alpha;          /* also relabel globally once the height
                                       increases since the last global relabel
                                       exceed alpha * (active vertices after it)
-                                      + 1024; 0 = default 4.0, < 0 = only the
-                                      reference period                        */
+                                      + 1024 and the time since it has reached
+                                      its duration; 0 = default 4.0, < 0 = only
+                                      the reference period                    */
 } wtv_params;
 
 /* Fill *p with the reference defaults. */
