@@ -399,6 +399,36 @@ def test_server_results_equal_alone_and_in_the_mix(orc, torch_cuda):
                                w.partition, r.raw_flow) == 0
 
 
+def test_server_isolates_a_failed_request(orc, torch_cuda):
+    """A request whose graph is invalid (a missing reverse arc) fails alone: the
+    batch it was admitted with is re-solved request by request, so every other
+    request still returns its lone-solve flow and labels."""
+    _, _, big = _c0_soft(24, 11)
+    good = [S.random_graph(int(200 + 50 * i), 960 + i, cap_max=40) for i in range(5)]
+    bad = S.random_graph(300, 3)
+    nbr = bad[0].copy()
+    v = int(np.argwhere(nbr >= 0)[0][0])
+    u = int(nbr[v, int(np.argwhere(nbr[v] >= 0)[0][0])])
+    nbr[u, nbr[u] == v] = -1  # drop the reverse arc
+    bad = (nbr, bad[1], bad[2])
+    graphs = [(big.nbr, big.res, big.term)] + good[:2] + [bad] + good[2:]
+    dev = [_dev_graph(torch_cuda, *g) for g in graphs]
+    torch_cuda.cuda.synchronize()
+    # one slot: the first (large) request runs while the rest queue into one batch
+    with W.Server(slots=1, max_batch=16) as srv:
+        tickets = [srv.submit(g[2].shape[0], d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), d[3].data_ptr())
+                   for g, d in zip(graphs, dev)]
+        res = [srv.wait(t) for t in tickets]
+    assert res[3].status == W.WTV_EINVARIANT
+    assert max(r.batch_size for r in res) > 1, "nothing was batched"
+    for i, (g, d, r) in enumerate(zip(graphs, dev, res)):
+        if i == 3:
+            continue
+        w = orc.bk_solve(*g)
+        assert r.status == 0 and r.raw_flow == w.raw_flow, i
+        assert np.array_equal(d[3].cpu().numpy()[:g[2].shape[0]], w.partition), i
+
+
 def test_server_admission_by_footprint(orc, torch_cuda):
     graphs = [S.random_graph(3000, 950 + i, cap_max=40) for i in range(4)]
     fp = W.lib().wtv_request_footprint(3000)

@@ -48,6 +48,13 @@ struct DevMem {
     ~DevMem() { if (p) (void)hipFree(p); }
 };
 
+// Waits for the stream on scope exit, also when an exception unwinds: the
+// gather buffers declared before it must outlive every copy queued on it.
+struct SyncOnExit {
+    hipStream_t s;
+    ~SyncOnExit() { (void)hipStreamSynchronize(s); }
+};
+
 }  // namespace
 
 uint64_t request_footprint(uint32_t n) {
@@ -146,13 +153,22 @@ class Server {
             }
             const double start = now();
             std::vector<int64_t> flows(batch.size(), 0);
-            int status = WTV_OK;
-            try {
-                run(batch, flows, s);
-            } catch (const Error& e) {
-                status = e.code;
-            } catch (...) {
-                status = WTV_EINVAL;
+            std::vector<int> status(batch.size(), WTV_OK);
+            const int st = attempt([&]() { run(batch, flows, s); });
+            if (st != WTV_OK) {
+                if (batch.size() == 1) {
+                    status[0] = st;
+                } else {
+                    // a failed batch leaves every request untouched (results are
+                    // scattered back only on success): solve each alone, so only
+                    // the request at fault reports the error
+                    for (size_t i = 0; i < batch.size(); ++i) {
+                        std::vector<std::shared_ptr<Job>> one{batch[i]};
+                        std::vector<int64_t> f1(1, 0);
+                        status[i] = attempt([&]() { run(one, f1, s); });
+                        flows[i] = f1[0];
+                    }
+                }
             }
             const double end = now();
             {
@@ -160,7 +176,7 @@ class Server {
                 for (size_t i = 0; i < batch.size(); ++i) {
                     auto& j = batch[i];
                     used_bytes_ -= j->footprint;
-                    j->res.status = status;
+                    j->res.status = status[i];
                     j->res.batch_size = static_cast<int32_t>(batch.size());
                     j->res.slot = slot;
                     j->res.raw_flow = flows[i];
@@ -177,6 +193,18 @@ class Server {
         if (s) (void)hipStreamDestroy(s);
     }
 
+    template <class F>
+    static int attempt(F&& f) {
+        try {
+            f();
+            return WTV_OK;
+        } catch (const Error& e) {
+            return e.code;
+        } catch (...) {
+            return WTV_EINVAL;
+        }
+    }
+
     void run(const std::vector<std::shared_ptr<Job>>& batch, std::vector<int64_t>& flows, hipStream_t s) {
         wtv_stats st{};
         if (batch.size() == 1) {
@@ -190,6 +218,7 @@ class Server {
         for (uint32_t g = 0; g < G; ++g) off[g + 1] = off[g] + batch[g]->req.n;
         const size_t N = off[G];
         DevMem nbr, res, term, part;
+        SyncOnExit sync{s};
         cks(hipMalloc(&nbr.p, std::max<size_t>(N, 1) * 16u), "gather nbr");
         cks(hipMalloc(&res.p, std::max<size_t>(N, 1) * 16u), "gather res");
         cks(hipMalloc(&term.p, std::max<size_t>(N, 1) * 4u), "gather term");
