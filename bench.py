@@ -516,6 +516,13 @@ def delaunay_leg(a, torch, np, W, S, dev, stream):
         assert b.raw_flow == flows[0] and np.array_equal(b.partition, part.cpu().numpy()), \
             "Delaunay leg: CPU oracle differs"
         out["cpu_bk_s"] = round(cpu_s, 3)
+        # the restated reference push-relabel (same Phase I, global relabel and
+        # final BFS), sequential on one core
+        tc = time.perf_counter()
+        pr = oracle.pr_solve(nbr_h, res0.cpu().numpy().view(np.uint32), term0.cpu().numpy())
+        out["cpu_pr_restated_s"] = round(time.perf_counter() - tc, 3)
+        assert pr.raw_flow == flows[0] and np.array_equal(pr.partition, part.cpu().numpy()), \
+            "Delaunay leg: restated push-relabel differs"
         out["identical_to_cpu"] = True
     log(f"[bench] delaunay: {N} tets, {t_s * 1e3:.2f} ms/solve, gen {gen_s:.1f}s")
     return out
