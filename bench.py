@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--serve-load", type=float, default=0.8, help="offered load vs looped-solve capacity")
     ap.add_argument("--delaunay-points", type=int, default=400_000,
                     help="Delaunay leg: genus-3 point-cloud size (0 = skip)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuned re-runs of the single-graph legs")
     ap.add_argument("--no-cut-leg", action="store_true",
                     help="skip the graph_cut_fast leg (reference capacity model, full drop-in call)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -152,6 +154,10 @@ def main():
             f"sub={s['sub_rounds']} relabels(global)={s['global_relabels']} flow={s['raw_flow']}")
     assert len(set(flows)) == 1, f"flows differ between steps: {flows}"
     raw_flow = flows[0]
+    tuned = None
+    if not a.no_tune:
+        tuned = autotune_single(a, torch, W, N, nbr, res, term, res0, term0, part, stream, a.steps, raw_flow)
+        log(f"[bench] autotuned {tuned['params']}: {tuned['s_per_solve'] * 1e3:.1f} ms/solve")
 
     # -------------------------------------------------- certificate (GPU)
     cut, energy = cut_capacity(torch, nbr, res0, term0, part, g["cap_f32"], g["term_f32"])
@@ -283,6 +289,7 @@ def main():
                    "global_relabel_period": a.period, "local_steps": a.local_steps, "relabel_alpha": a.alpha,
                    "stand_in": "Kuhn grid replaces the 12M-point Delaunay mesh"},
         "tets_per_s": round(N / value, 1),
+        "autotuned": tuned,
         "t_solve_ms_device": round(s0["t_solve_ms"], 3),
         "raw_flow": raw_flow,
         "cut_energy_fp64": energy,
@@ -369,6 +376,31 @@ def atomic_probe(torch, n=1 << 26, reps=3):
             out[f"{name}_{kind}_atomics_per_s"] = round(n / (ms * 1e-3), 1)
         assert int(dst[:16].min()) == 2 * reps + 2  # every address got every add
     return out
+
+
+def autotune_single(a, torch, W, N, nbr, res, term, res0, term0, part, stream, steps, want_flow):
+    """The project's autotuner on one graph: coordinate descent over the
+    schedule knobs (min of 2 solves per trial, every trial must return the
+    same flow and label count), then `steps` timed solves with the result."""
+    import wtv_autotune as AT
+    want = (int(want_flow), int(part.sum()))
+
+    def solve(p):
+        res.copy_(res0)
+        term.copy_(term0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = W.pr_solve(N, nbr.data_ptr(), res.data_ptr(), term.data_ptr(), part.data_ptr(), 1,
+                        W.params(global_relabel_period=a.period, **p), stream)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, (int(st.raw_flow), int(part.sum()))
+
+    t_tune = time.perf_counter()
+    best, _, trials = AT.tune(solve, reference=want, reps=2)
+    t_tune = time.perf_counter() - t_tune
+    ts = [solve(best)[0] for _ in range(max(steps, 3))]
+    return {"params": best, "s_per_solve": round(sum(ts) / len(ts), 5), "trials": len(trials),
+            "tune_seconds": round(t_tune, 2), "same_answer": True}
 
 
 def copy_ceiling(torch, nbytes=1 << 30, reps=3):
@@ -498,11 +530,13 @@ def delaunay_leg(a, torch, np, W, S, dev, stream):
     st = step().as_dict()
     cut, energy = cut_capacity(torch, nbr, res0, term0, part, g["cap_f32"], g["term_f32"])
     assert cut == flows[0], f"Delaunay leg: cut capacity {cut} != flow {flows[0]}"
+    tuned = None if a.no_tune else autotune_single(a, torch, W, N, nbr, res, term, res0, term0, part, stream,
+                                                   steps, flows[0])
     out = {"workload": f"Delaunay (scipy Qhull) of {a.delaunay_points} genus-3 points, 95% surface + 5% uniform, "
                        "Morton-ordered tets, soft seeds, lambda 10, 2% flips",
            "points": a.delaunay_points, "tets": N, "voxel_res": round(meta["res"], 2),
            "inside_frac": round(meta["inside_frac"], 4), "s_per_solve": round(t_s, 5),
-           "tets_per_s": round(N / t_s, 1), "raw_flow": flows[0], "cut_energy_fp64": energy,
+           "tets_per_s": round(N / t_s, 1), "autotuned": tuned, "raw_flow": flows[0], "cut_energy_fp64": energy,
            "certificate": "flow == integer cut capacity",
            "work": {k2: st[k2] for k2 in ("rounds", "sub_rounds", "global_relabels", "bfs_levels",
                                           "tail_sub_rounds")},
@@ -600,7 +634,7 @@ def batch_leg(a, torch, np, W, S, dev, stream):
             "graphs_per_s_batched": round(len(sizes) / t_b, 2),
             "speedup": round(t_loop / t_b, 2), "identical_to_looped": True,
             "autotuned": best, "autotune_trials": len(trials), "autotune_seconds": round(t_tune, 2),
-            "looped_params": "defaults (alpha 4, local_steps 8, tail_cap 2048, grid 1024)",
+            "looped_params": "defaults (alpha 2, local_steps 8, tail_cap 512, grid 1024, wg 256)",
             "batched_work": {"sub_rounds": st.sub_rounds, "global_relabels": st.global_relabels,
                              "bfs_levels": st.bfs_levels}}
 

@@ -96,12 +96,12 @@ typedef struct wtv_params {
     int32_t no_graph;              /* 1 = launch kernels one by one instead of
                                       replaying a captured macro round      */
     int32_t tail_cap;              /* active lists up to this size run in one
-                                      workgroup; 0 = default 2048, < 0 = off */
+                                      workgroup; 0 = default 512, < 0 = off */
     double relabel_alpha;          /* also relabel globally once the height
                                       increases since the last global relabel
                                       exceed alpha * (active vertices after it)
                                       + 1024 and the time since it has reached
-                                      its duration; 0 = default 4.0, < 0 = only
+                                      its duration; 0 = default 2.0, < 0 = only
                                       the reference period                    */
 } wtv_params;
 

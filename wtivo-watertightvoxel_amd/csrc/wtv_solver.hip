@@ -1237,7 +1237,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     if (period > 0) macro_period = std::max(1, (period + local_steps - 1) / local_steps);
     // Adaptive cadence: a global relabel also runs once the relabel work since
     // the last one exceeds alpha * (active vertices right after it) + 1024.
-    const double alpha = prm.relabel_alpha == 0.0 ? 4.0 : prm.relabel_alpha;
+    const double alpha = prm.relabel_alpha == 0.0 ? 2.0 : prm.relabel_alpha;
     uint64_t relabels_at_gr = stat_sum(*hctl, kStRelabels);
     uint64_t active_at_gr = active;
 
@@ -1301,7 +1301,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     };
 
     // Small active lists (<= tail_cap) go to the one-workgroup discharge kernel.
-    const uint32_t tail_cap = prm.tail_cap == 0 ? 2048u : (prm.tail_cap < 0 ? 0u : static_cast<uint32_t>(prm.tail_cap));
+    const uint32_t tail_cap = prm.tail_cap == 0 ? 512u : (prm.tail_cap < 0 ? 0u : static_cast<uint32_t>(prm.tail_cap));
     uint64_t relabels_now = relabels_at_gr;
     auto do_global_relabel = [&]() {
         global_relabel();

@@ -18,7 +18,7 @@ import json
 import os
 import time
 
-DEFAULT = {"relabel_alpha": 4.0, "local_steps": 8, "tail_cap": 2048, "grid_wgs": 1024, "wg_size": 256}
+DEFAULT = {"relabel_alpha": 2.0, "local_steps": 8, "tail_cap": 512, "grid_wgs": 1024, "wg_size": 256}
 SPACE = {
     "relabel_alpha": [2.0, 4.0, 8.0, 16.0, 32.0],
     "local_steps": [4, 8, 16, 32],
