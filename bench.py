@@ -60,7 +60,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20260001)
     ap.add_argument("--period", type=int, default=1024)
     ap.add_argument("--local-steps", type=int, default=8)
-    ap.add_argument("--alpha", type=float, default=0.0, help="adaptive global-relabel factor (0 = default)")
+    ap.add_argument("--alpha", type=float, default=0.0, help="adaptive global-relabel factor (0 = the library default, 2)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle check")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--batch-graphs", type=int, default=256, help="batched leg: number of graphs (0 = skip)")
@@ -286,7 +286,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"kuhn{k}_genus3_softseed_lambda10", "tets": N, "free_nodes": n,
                    "lambda_fill": 10.0, "h_cells": 1.0, "label_flips": 0.02, "seed": a.seed,
-                   "global_relabel_period": a.period, "local_steps": a.local_steps, "relabel_alpha": a.alpha,
+                   "global_relabel_period": a.period, "local_steps": a.local_steps,
+                   "relabel_alpha": a.alpha if a.alpha != 0.0 else 2.0, "tail_cap": 512, "grid_wgs": 1024,
+                   "wg_size": 256,
                    "stand_in": "Kuhn grid replaces the 12M-point Delaunay mesh"},
         "tets_per_s": round(N / value, 1),
         "autotuned": tuned,
