@@ -178,6 +178,27 @@ def test_workgroup_size_rejected(torch_cuda):
             gpu_solve(torch_cuda, nbr, res, term, wg_size=wg)
 
 
+@pytest.mark.parametrize("seed", range(3))
+def test_extreme_capacities(orc, torch_cuda, seed):
+    """Arc capacities up to 2^30 - 1 and terminals up to +-(2^31 - 1), the largest
+    the reference's build admits: 64-bit excess and flow sums, no overflow."""
+    nbr, res, term = S.random_graph(4000, 300 + seed, cap_max=(1 << 30) - 1, term_max=(1 << 31) - 1, p_term=0.9)
+    a = orc.bk_solve(nbr, res, term)
+    assert a.raw_flow > (1 << 36)  # far beyond 32-bit sums
+    assert_same(orc, nbr, res, term, gpu_solve(torch_cuda, nbr, res, term), a.raw_flow, a.partition)
+
+
+@pytest.mark.parametrize("t", [0, 7, -7])
+def test_single_vertex(orc, torch_cuda, t):
+    nbr = np.full((1, 4), -1, np.int32)
+    res = np.zeros((1, 4), np.uint32)
+    term = np.array([t], np.int32)
+    part, st, r_out, t_out = gpu_solve(torch_cuda, nbr, res, term)
+    a = orc.bk_solve(nbr, res, term)
+    assert st["raw_flow"] == a.raw_flow == 0
+    assert part.tolist() == a.partition.tolist()
+
+
 def test_deterministic_labels_and_flow(torch_cuda):
     _, _, g = _c0_soft(16, 5)
     p1, s1, r1, t1 = gpu_solve(torch_cuda, g.nbr, g.res, g.term)

@@ -311,3 +311,14 @@ def test_surface_is_closed_and_encloses_the_region(orc, seed):
     # vertices compacted in ascending original id
     used = np.unique(np.concatenate([m.tets[lab == 0].ravel()]))
     assert OV.shape[0] <= used.shape[0]
+
+
+def test_extreme_capacities_bk_equals_pr(orc):
+    """Arc capacities up to 2^30 - 1 and terminals up to +-(2^31 - 1): both CPU
+    solvers agree, and the flow passes the certificate (64-bit sums)."""
+    nbr, res, term = S.random_graph(3000, 311, cap_max=(1 << 30) - 1, term_max=(1 << 31) - 1, p_term=0.9)
+    b = orc.bk_solve(nbr, res, term)
+    p = orc.pr_solve(nbr, res, term)
+    assert b.raw_flow == p.raw_flow > (1 << 36)  # far beyond 32-bit sums
+    assert np.array_equal(b.partition, p.partition)
+    assert orc.certificate(nbr, res, term, p.res, p.term, p.partition, p.raw_flow) == 0
