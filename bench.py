@@ -73,6 +73,8 @@ def main():
                     help="skip the autotuned re-runs of the single-graph legs")
     ap.add_argument("--no-cut-leg", action="store_true",
                     help="skip the graph_cut_fast leg (reference capacity model, full drop-in call)")
+    ap.add_argument("--no-topology-leg", action="store_true",
+                    help="skip the topology leg (host upload and device neighbour construction at --k)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy three times (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     ap.add_argument("--atomic-probe", action="store_true",
@@ -257,6 +259,10 @@ def main():
     if not a.no_cut_leg:
         cutleg = cut_leg(a, torch, np, W, S, dev, stream)
 
+    topology = None
+    if not a.no_topology_leg:
+        topology = topology_leg(a, torch, np, W, S, dev, stream)
+
     delaunay = None
     if a.delaunay_points > 0:
         delaunay = delaunay_leg(a, torch, np, W, S, dev, stream)
@@ -308,6 +314,7 @@ def main():
         "cpu_baseline": cpu,
         "atomics": atomics,
         "graph_cut_fast": cutleg,
+        "topology": topology,
         "delaunay": delaunay,
         "batched": batched,
         "serving": serving,
@@ -499,6 +506,99 @@ def cut_leg(a, torch, np, W, S, dev, stream):
     log(f"[bench] graph_cut_fast: {N} tets, {F} free, wall {min(walls) * 1e3:.1f} ms "
         f"(solve {res.solve_seconds * 1e3:.1f} ms, build {res.build_upload_seconds * 1e3:.1f} ms)")
     return out
+
+
+def topology_leg(a, torch, np, W, S, dev, stream):
+    """The topology on its way in, at the Kuhn k^3 size of the headline mesh:
+    * wtv_topology_upload of the tets as the Fortran-ordered int32 matrix an
+      Eigen export hands over, against the reference's loop of synchronous
+      262,144-row copies (restated in the oracle) and a pinned 1 GiB copy
+      (the PCIe ceiling); C-ordered input timed too;
+    * wtv_tet_neighbors on the device tets, checked equal to the closed-form
+      Kuhn neighbours; the oracle's face sort on a 64^3 Kuhn mesh (1 thread)
+      is the CPU baseline."""
+    import ctypes as C
+    k = a.k
+    t = time.time()
+    T, Nb = S.kuhn_topology_t(k, dev)
+    N = int(T.shape[0])
+    nv = (k + 1) ** 3
+    Th = T.cpu().numpy()
+    Tf = np.asfortranarray(Th)
+    gen_s = time.time() - t
+    L = W.lib()
+    nbytes = N * 16
+
+    # ---- device neighbours
+    out = torch.empty_like(T)
+
+    def nbr_call():
+        W.check(L.wtv_tet_neighbors(T.data_ptr(), N, nv, out.data_ptr(), stream))
+
+    nbr_call()
+    walls = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        nbr_call()
+        walls.append(time.perf_counter() - t0)
+    assert torch.equal(out, Nb), "topology leg: device neighbours differ from the closed form"
+    del out
+
+    # ---- upload
+    dst = torch.empty_like(T)
+
+    def up(arr):
+        rs, cs = (s2 // 4 for s2 in arr.strides)
+        t0 = time.perf_counter()
+        W.check(L.wtv_topology_upload(arr.ctypes.data, N, rs, cs, dst.data_ptr(), 0, stream))
+        return time.perf_counter() - t0
+
+    up(Tf)
+    up_f = min(up(Tf) for _ in range(3))
+    assert torch.equal(dst, T), "topology leg: uploaded tets differ (Fortran order)"
+    up_c = min(up(Th) for _ in range(3))
+    assert torch.equal(dst, T), "topology leg: uploaded tets differ (C order)"
+    del dst
+    pin = torch.empty(1 << 28, dtype=torch.int32, pin_memory=True)
+    gdst = torch.empty(1 << 28, dtype=torch.int32, device=dev)
+    gdst.copy_(pin, non_blocking=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        gdst.copy_(pin, non_blocking=True)
+    torch.cuda.synchronize()
+    pcie = 3 * (1 << 30) / (time.perf_counter() - t0) / 1e9
+    del pin, gdst
+    out_d = {"workload": f"Kuhn {k}^3 topology: {N} tets, {nv} vertices", "tets": N,
+             "neighbors_s": round(min(walls), 4), "neighbors_tets_per_s": round(N / min(walls), 1),
+             "neighbors_check": "equal to the closed-form Kuhn neighbours",
+             "upload_fortran_s": round(up_f, 4), "upload_fortran_GBs": round(nbytes / up_f / 1e9, 2),
+             "upload_c_s": round(up_c, 4), "upload_c_GBs": round(nbytes / up_c / 1e9, 2),
+             "pinned_copy_ceiling_GBs": round(pcie, 2), "upload_bytes": nbytes, "gen_seconds": round(gen_s, 1)}
+    if not a.no_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        ref = oracle.topology_to_cuda_reference(Tf, torch)
+        ref_s = time.perf_counter() - t0
+        assert torch.equal(ref, T)
+        del ref
+        out_d["reference_upload_s"] = round(ref_s, 4)
+        out_d["reference_upload_GBs"] = round(nbytes / ref_s / 1e9, 2)
+        out_d["reference_upload"] = ("topology_to_cuda_chunked's loop restated (wtivo.py:284-309): "
+                                     "262,144-row np.ascontiguousarray slices, synchronous copy_ each")
+        kc = 64
+        tc, nc = S.kuhn_topology(kc)
+        t0 = time.perf_counter()
+        got = oracle.tet_neighbors(tc, (kc + 1) ** 3)
+        cpu_s = time.perf_counter() - t0
+        assert np.array_equal(got, nc)
+        out_d["cpu_neighbors"] = {"value": round(tc.shape[0] / cpu_s, 1), "unit": "tets/s", "cores": 1,
+                                  "kind": "port", "sample": f"oracle face sort (numpy) on Kuhn {kc}^3 = "
+                                                            f"{tc.shape[0]} tets, equal to the closed form"}
+    log(f"[bench] topology: neighbours {min(walls) * 1e3:.1f} ms, upload F {up_f * 1e3:.1f} ms "
+        f"({nbytes / up_f / 1e9:.1f} GB/s), C {up_c * 1e3:.1f} ms, reference loop "
+        f"{out_d.get('reference_upload_s', float('nan')) * 1e3:.1f} ms, pinned ceiling {pcie:.1f} GB/s")
+    return out_d
 
 
 def delaunay_leg(a, torch, np, W, S, dev, stream):

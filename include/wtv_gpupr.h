@@ -20,6 +20,9 @@
  *                          (native/gpupr/wtivo_gpupr_bindings.cpp:400-577,597-599)
  *   wtv_server_*        <- no reference counterpart: a serving loop around
  *                          the same solver (continuous batching, HBM admission)
+ *   wtv_tet_neighbors   <- the neighbour export of tetrahedralize_neighbors
+ *                          (native/core/wtivo_core.cpp:136-154)
+ *   wtv_topology_upload <- topology_to_cuda_chunked (wtivo.py:284-309)
  */
 #ifndef WTV_GPUPR_H
 #define WTV_GPUPR_H
@@ -266,6 +269,27 @@ int wtv_surface_extract(const uint8_t* labels, const double* V, size_t nv,
                         const int32_t* d_tets, const int32_t* d_nbr, size_t N,
                         wtv_surface* out, void* stream);
 void wtv_surface_free(wtv_surface* s);
+
+/* 4-slot neighbours of a tet mesh, built on the device: d_nbr[i*4+fi] is the
+ * tet across face {012,013,023,123}[fi] of tet i (the face opposite local
+ * vertex 3-fi, CGAL's neighbor(3-fi) in the reference export) and -1 on the
+ * hull.  d_tets int32[N*4] holds vertex ids in [0, nv); d_nbr int32[N*4] is
+ * written.  WTV_EINVAL for an out-of-range or repeated vertex id in a tet and
+ * for a face held by more than two tets; WTV_ENOMEM when the face table
+ * (4 B per entry, a power of two >= 6N entries; 8 B for N >= 2^30) does not
+ * fit.  Synchronous on `stream`. */
+int wtv_tet_neighbors(const int32_t* d_tets, size_t N, size_t nv, int32_t* d_nbr, void* stream);
+
+/* Upload of an [rows, 4] int32 host matrix into a row-major device array
+ * d_out (int32[rows*4]).  Element strides: (4, 1) for C order, (1, rows) for
+ * the Fortran order an Eigen MatrixXi exports, anything else is gathered row
+ * by row.  Worker threads (threads <= 0: the host's core count, at most 16)
+ * pack 262,144-row chunks into pinned buffers kept for the process lifetime
+ * (2 x 4 MiB per worker) and copy them on their own streams.  Waits for the
+ * work queued on `stream` before writing d_out and returns when every row has
+ * arrived. */
+int wtv_topology_upload(const int32_t* host, size_t rows, int64_t row_stride, int64_t col_stride,
+                        int32_t* d_out, int threads, void* stream);
 
 /* Stats of the last wtv_pr_solve / wtv_graph_cut_fast on this thread. */
 void wtv_last_stats(wtv_stats* out);

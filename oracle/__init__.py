@@ -6,6 +6,10 @@ module.  The product path (wtivo_gpupr / libwtv_gpupr.so) never calls it.
 
 Each wrapper names the reference code its C function restates; see the header
 of wtv_oracle.c for the full list and for what pins each function.
+tet_neighbors is numpy only (a sort of the face triples); it is pinned by the
+closed-form Kuhn neighbours, by Qhull's own neighbours of Delaunay meshes, by
+equivariance under tet and vertex permutations, and by a brute-force search
+on tiny meshes (tests/test_oracle.py).
 """
 from __future__ import annotations
 
@@ -253,3 +257,56 @@ def surface(labels, V, tets, nbrs):
     if rc != OK:
         raise OracleError(rc, "surface")
     return OV, OF
+
+
+_FACE = np.array([[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]])
+
+
+def tet_neighbors(tets, nv=None):
+    """4-slot neighbours by face matching: the export of tetrahedralize_neighbors
+    (native/core/wtivo_core.cpp:136-154). Slot fi is the tet across face
+    {012,013,023,123}[fi], which is CGAL's neighbor(3 - fi); a hull face gets -1.
+
+    Sorts the 4N sorted vertex triples and pairs equal neighbours in that order.
+    Raises OracleError(EINVAL) for a vertex id outside [0, nv), a vertex repeated
+    within a tet, or a face held by more than two tets."""
+    tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+    N = tets.shape[0]
+    nbr = np.full((N, 4), -1, np.int32)
+    if N == 0:
+        return nbr
+    t64 = tets.astype(np.int64)
+    if nv is not None and ((t64 < 0) | (t64 >= nv)).any():
+        raise OracleError(EINVAL, "tet_neighbors: vertex id out of range")
+    st = np.sort(t64, axis=1)
+    if (st[:, 1:] == st[:, :-1]).any():
+        raise OracleError(EINVAL, "tet_neighbors: repeated vertex")
+    F = np.sort(t64[:, _FACE], axis=2).reshape(-1, 3)   # row 4i+fi = face fi of tet i
+    order = np.lexsort((F[:, 2], F[:, 1], F[:, 0]))
+    Fs = F[order]
+    eq = (Fs[1:] == Fs[:-1]).all(axis=1)
+    if (eq[1:] & eq[:-1]).any():
+        raise OracleError(EINVAL, "tet_neighbors: face held by more than two tets")
+    k = np.nonzero(eq)[0]
+    p, q = order[k], order[k + 1]
+    flat = nbr.reshape(-1)
+    flat[p] = (q // 4).astype(np.int32)
+    flat[q] = (p // 4).astype(np.int32)
+    return nbr
+
+
+def topology_to_cuda_reference(host_array, torch, chunk_rows: int = 262144):
+    """Restatement of the reference's upload loop, topology_to_cuda_chunked
+    (wtivo.py:284-309): repack 262,144-row slices with np.ascontiguousarray and
+    copy each one synchronously into a torch CUDA tensor.  Timed beside
+    wtv_topology_upload as the reference's way of doing the same copy."""
+    a = np.asarray(host_array)
+    n = int(a.shape[0])
+    out = torch.empty((n, 4), dtype=torch.int32, device="cuda")
+    for i in range(0, n, chunk_rows):
+        j = min(n, i + chunk_rows)
+        h = np.ascontiguousarray(a[i:j], dtype=np.int32)
+        out[i:j].copy_(torch.from_numpy(h), non_blocking=False)
+        del h
+    torch.cuda.synchronize()
+    return out

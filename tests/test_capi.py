@@ -83,3 +83,23 @@ def test_pybind_rejects_non_cuda_like_reference():
 def test_request_footprint_is_linear():
     f = W.lib().wtv_request_footprint
     assert f(0) > 0 and f(2000) - f(1000) == f(3000) - f(2000)
+
+
+def test_topology_entry_points_reject_bad_arguments_without_device():
+    L = W.lib()
+    assert L.wtv_tet_neighbors(None, 0, 1, None, None) == W.WTV_OK           # N == 0: nothing to do
+    assert L.wtv_tet_neighbors(None, 10, 5, None, None) == W.WTV_EINVAL
+    buf = (C.c_int32 * 16)()
+    assert L.wtv_tet_neighbors(buf, 1 << 31, 5, buf, None) == W.WTV_EINVAL   # ids must fit int32
+    assert L.wtv_tet_neighbors(buf, 4, 0, buf, None) == W.WTV_EINVAL         # no vertices
+    assert L.wtv_topology_upload(None, 0, 4, 1, None, 0, None) == W.WTV_OK
+    assert L.wtv_topology_upload(None, 5, 4, 1, None, 0, None) == W.WTV_EINVAL
+    assert L.wtv_topology_upload(buf, 4, 0, 1, buf, 0, None) == W.WTV_EINVAL  # every row aliased
+    assert "row stride" in W.lib().wtv_last_error().decode()
+
+
+def test_topology_to_cuda_chunked_shape_check_like_reference():
+    import numpy as np
+    import wtv_topology as T
+    with pytest.raises(RuntimeError, match=r"tets: expected Nx4 topology array, got \(3, 3\)"):
+        T.topology_to_cuda_chunked(np.zeros((3, 3), np.int32), "tets")

@@ -322,3 +322,73 @@ def test_extreme_capacities_bk_equals_pr(orc):
     assert b.raw_flow == p.raw_flow > (1 << 36)  # far beyond 32-bit sums
     assert np.array_equal(b.partition, p.partition)
     assert orc.certificate(nbr, res, term, p.res, p.term, p.partition, p.raw_flow) == 0
+
+
+# ------------------------------------------------------------ tet neighbours
+def _brute_neighbors(tets):
+    """Plain-Python search: the other tet holding all three vertices of each face."""
+    faces = [(0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)]
+    sets = [set(int(x) for x in t) for t in tets]
+    out = np.full((len(tets), 4), -1, np.int32)
+    for i, t in enumerate(tets):
+        for fi, f in enumerate(faces):
+            key = {int(t[k]) for k in f}
+            hits = [j for j, s in enumerate(sets) if j != i and key <= s]
+            assert len(hits) <= 1
+            if hits:
+                out[i, fi] = hits[0]
+    return out
+
+
+@pytest.mark.parametrize("k", [1, 2, 5, 9])
+def test_tet_neighbors_closed_form_kuhn(orc, k):
+    tets, nbrs = S.kuhn_topology(k)
+    assert np.array_equal(orc.tet_neighbors(tets, (k + 1) ** 3), nbrs)
+
+
+def test_tet_neighbors_brute_force_tiny(orc):
+    rng = np.random.default_rng(7)
+    tets, _ = S.kuhn_topology(2)
+    assert np.array_equal(orc.tet_neighbors(tets), _brute_neighbors(tets))
+    m = S.delaunay_mesh(rng.random((40, 3)), morton=False)
+    assert np.array_equal(orc.tet_neighbors(m.tets), _brute_neighbors(m.tets))
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_tet_neighbors_match_qhull(orc, seed):
+    # Qhull computes neighbours itself; delaunay_mesh maps them to the 4-slot layout
+    rng = np.random.default_rng(seed)
+    for pts in (rng.random((3000, 3)), S.genus3_point_cloud(20000, seed)[0]):
+        m = S.delaunay_mesh(pts, morton=bool(seed))
+        assert np.array_equal(orc.tet_neighbors(m.tets, pts.shape[0]), m.nbrs)
+
+
+def test_tet_neighbors_equivariant_under_permutations(orc):
+    tets, nbrs = S.kuhn_topology(6)
+    for seed in range(3):
+        t2, n2 = S.permute_topology(tets, nbrs, seed)
+        assert np.array_equal(orc.tet_neighbors(t2), n2)
+
+
+def test_tet_neighbors_hull_and_errors(orc):
+    one = np.array([[0, 1, 2, 3]], np.int32)
+    assert (orc.tet_neighbors(one) == -1).all()
+    two = np.array([[0, 1, 2, 3], [4, 2, 1, 0]], np.int32)   # share face {0,1,2}
+    n = orc.tet_neighbors(two)
+    assert n.tolist() == [[1, -1, -1, -1], [-1, -1, -1, 0]]
+    three = np.array([[0, 1, 2, 3], [0, 1, 2, 4], [0, 1, 2, 5]], np.int32)
+    with pytest.raises(orc.OracleError):
+        orc.tet_neighbors(three)
+    with pytest.raises(orc.OracleError):
+        orc.tet_neighbors(np.array([[0, 1, 1, 3]], np.int32))
+    with pytest.raises(orc.OracleError):
+        orc.tet_neighbors(np.array([[0, 1, 2, 9]], np.int32), nv=5)
+    assert orc.tet_neighbors(np.zeros((0, 4), np.int32)).shape == (0, 4)
+
+
+def test_tet_neighbors_planted_slot_mistake_is_caught(orc):
+    # a restatement that used the face list {012,013,023,123} reversed (slot fi
+    # = CGAL's neighbor(fi)) would disagree with the closed form on Kuhn meshes
+    tets, nbrs = S.kuhn_topology(3)
+    wrong = orc.tet_neighbors(tets)[:, ::-1]
+    assert not np.array_equal(wrong, nbrs)

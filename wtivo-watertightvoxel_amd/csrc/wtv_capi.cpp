@@ -154,6 +154,15 @@ void wtv_surface_free(wtv_surface* s) {
     std::memset(s, 0, sizeof(*s));
 }
 
+int wtv_tet_neighbors(const int32_t* d_tets, size_t N, size_t nv, int32_t* d_nbr, void* stream) {
+    return guarded([&]() { wtv::tet_neighbors(d_tets, N, nv, d_nbr, stream); });
+}
+
+int wtv_topology_upload(const int32_t* host, size_t rows, int64_t row_stride, int64_t col_stride,
+                        int32_t* d_out, int threads, void* stream) {
+    return guarded([&]() { wtv::topology_upload(host, rows, row_stride, col_stride, d_out, threads, stream); });
+}
+
 uint64_t wtv_request_footprint(uint32_t n) { return wtv::request_footprint(n); }
 
 void* wtv_server_create(const wtv_server_config* cfg) {

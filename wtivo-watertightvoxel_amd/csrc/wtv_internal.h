@@ -1,6 +1,7 @@
 // wtv_internal.h -- shared declarations of libwtv_gpupr (not part of the C-ABI).
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -52,6 +53,13 @@ void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t
 // surface_extraction_topology_cuda on the device (wtv_graph_build.hip).
 void surface_extract(const uint8_t* labels, const double* V, size_t nv, const int32_t* dT, const int32_t* dN,
                      size_t N, std::vector<double>& OV, std::vector<int32_t>& OF, void* stream);
+
+// 4-slot tet neighbours by face matching (wtv_topology.hip).
+void tet_neighbors(const int32_t* d_tets, size_t N, size_t nv, int32_t* d_nbr, void* stream);
+
+// Pinned, multi-stream upload of an [rows, 4] int32 matrix (wtv_upload.cpp).
+void topology_upload(const int32_t* host, size_t rows, int64_t row_stride, int64_t col_stride, int32_t* d_out,
+                     int threads, void* stream);
 
 // Thread-local stats of the last solve (wtv_capi.cpp).
 void set_last_stats(const wtv_stats& st);

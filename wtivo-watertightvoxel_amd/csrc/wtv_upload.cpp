@@ -1,0 +1,151 @@
+// wtv_upload.cpp -- host-to-HBM upload of an [rows, 4] int32 topology matrix.
+//
+// Replaces topology_to_cuda_chunked (wtivo.py:284-309), which repacks
+// 262,144-row slices of a possibly Fortran-ordered array (Eigen's MatrixXi is
+// column-major) into a pageable buffer and copies each slice synchronously.
+// Here W worker threads each own a HIP stream and two pinned staging buffers
+// of kChunkRows rows: a worker packs chunk c (a memcpy for a C-ordered source,
+// a 4-stream interleave for a column-contiguous one, a strided gather
+// otherwise) into one buffer while the DMA of its previous chunk runs from the
+// other.  Chunks are dealt round-robin, so W copies are in flight at once.
+//
+// The pinned buffers are allocated with hipHostMalloc on first use and kept
+// for the life of the process (at most kMaxWorkers * 2 * 4 MiB); uploads on
+// one process are serialised by a mutex.  The call is synchronous: it returns
+// once d_out holds every row, after the work already queued on the caller's
+// stream (which may still be using d_out's memory) has finished.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "wtv_internal.h"
+
+namespace wtv {
+namespace {
+
+constexpr size_t kChunkRows = 262144;  // 4 MiB of int32x4 rows per staging buffer
+constexpr int kMaxWorkers = 16;
+
+inline void cku(hipError_t e, const char* where) {
+    if (e == hipSuccess) return;
+    fail(WTV_EDEVICE, std::string(kPrefix) + "HIP at " + where + ": " + hipGetErrorString(e));
+}
+
+struct Worker {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int32_t* pinned[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+};
+
+std::mutex g_mu;
+Worker g_workers[kMaxWorkers];
+
+// Creates the worker's stream, events and pinned buffers on first use (and
+// its stream and events again if the current device changed).
+void ready(Worker& w, int device) {
+    if (w.device != device) {
+        if (w.stream) (void)hipStreamDestroy(w.stream);
+        for (auto& e : w.done)
+            if (e) (void)hipEventDestroy(e);
+        w.stream = nullptr;
+        w.done[0] = w.done[1] = nullptr;
+        cku(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking), "upload stream");
+        for (auto& e : w.done) cku(hipEventCreateWithFlags(&e, hipEventDisableTiming), "upload event");
+        w.device = device;
+    }
+    for (auto& p : w.pinned)
+        if (!p) cku(hipHostMalloc(reinterpret_cast<void**>(&p), kChunkRows * 16u, hipHostMallocPortable),
+                    "upload pinned buffer");
+}
+
+// Packs rows [r0, r1) of the strided source into dst as row-major int32x4.
+void pack(const int32_t* src, int64_t rs, int64_t cs, size_t r0, size_t r1, int32_t* dst) {
+    const size_t n = r1 - r0;
+    if (rs == 4 && cs == 1) {
+        std::memcpy(dst, src + r0 * 4, n * 16u);
+        return;
+    }
+    if (rs == 1) {
+        const int32_t* c0 = src + r0;
+        const int32_t* c1 = c0 + cs;
+        const int32_t* c2 = c1 + cs;
+        const int32_t* c3 = c2 + cs;
+        for (size_t r = 0; r < n; ++r) {
+            dst[4 * r + 0] = c0[r];
+            dst[4 * r + 1] = c1[r];
+            dst[4 * r + 2] = c2[r];
+            dst[4 * r + 3] = c3[r];
+        }
+        return;
+    }
+    for (size_t r = 0; r < n; ++r) {
+        const int32_t* row = src + static_cast<int64_t>(r0 + r) * rs;
+        for (int k = 0; k < 4; ++k) dst[4 * r + k] = row[k * cs];
+    }
+}
+
+}  // namespace
+
+void topology_upload(const int32_t* host, size_t rows, int64_t row_stride, int64_t col_stride, int32_t* d_out,
+                     int threads, void* stream_v) {
+    if (rows == 0) return;
+    if (!host || !d_out) fail(WTV_EINVAL, std::string(kPrefix) + "null topology pointer");
+    if (row_stride == 0 && rows > 1) fail(WTV_EINVAL, std::string(kPrefix) + "row stride 0 with more than one row");
+    std::lock_guard<std::mutex> lk(g_mu);
+    int device = 0;
+    cku(hipGetDevice(&device), "hipGetDevice");
+    const size_t chunks = (rows + kChunkRows - 1) / kChunkRows;
+    int W = threads > 0 ? threads : static_cast<int>(std::thread::hardware_concurrency());
+    W = static_cast<int>(std::min<size_t>(std::max(1, std::min(W, kMaxWorkers)), chunks));
+    for (int w = 0; w < W; ++w) ready(g_workers[w], device);
+
+    // the caller's queued work first: d_out may be memory it just released
+    hipEvent_t start = nullptr;
+    cku(hipEventCreateWithFlags(&start, hipEventDisableTiming), "upload start event");
+    struct EvGuard {
+        hipEvent_t e;
+        ~EvGuard() { (void)hipEventDestroy(e); }
+    } guard{start};
+    cku(hipEventRecord(start, static_cast<hipStream_t>(stream_v)), "upload start record");
+    for (int w = 0; w < W; ++w) cku(hipStreamWaitEvent(g_workers[w].stream, start, 0), "upload wait");
+
+    std::vector<int> status(W, WTV_OK);
+    std::vector<std::string> msg(W);
+    auto run = [&](int w) {
+        Worker& wk = g_workers[w];
+        try {
+            cku(hipSetDevice(device), "hipSetDevice");
+            bool used[2] = {false, false};
+            int b = 0;
+            for (size_t c = static_cast<size_t>(w); c < chunks; c += static_cast<size_t>(W), b ^= 1) {
+                const size_t r0 = c * kChunkRows, r1 = std::min(rows, r0 + kChunkRows);
+                if (used[b]) cku(hipEventSynchronize(wk.done[b]), "upload buffer wait");
+                pack(host, row_stride, col_stride, r0, r1, wk.pinned[b]);
+                cku(hipMemcpyAsync(d_out + r0 * 4, wk.pinned[b], (r1 - r0) * 16u, hipMemcpyHostToDevice, wk.stream),
+                    "upload copy");
+                cku(hipEventRecord(wk.done[b], wk.stream), "upload record");
+                used[b] = true;
+            }
+            cku(hipStreamSynchronize(wk.stream), "upload sync");
+        } catch (const Error& e) {
+            status[w] = e.code;
+            msg[w] = e.what();
+            (void)hipStreamSynchronize(wk.stream);
+        }
+    };
+    std::vector<std::thread> pool;
+    pool.reserve(W - 1);
+    for (int w = 1; w < W; ++w) pool.emplace_back(run, w);
+    run(0);
+    for (auto& t : pool) t.join();
+    for (int w = 0; w < W; ++w)
+        if (status[w] != WTV_OK) fail(status[w], msg[w]);
+}
+
+}  // namespace wtv

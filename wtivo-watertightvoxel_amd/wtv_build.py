@@ -27,8 +27,8 @@ ARCH = "gfx950"
 LIB = os.path.join(HERE, "libwtv_gpupr.so")
 EXT = os.path.join(HERE, "wtivo_gpupr" + sysconfig.get_config_var("EXT_SUFFIX"))
 
-HIP_SOURCES = ["wtv_solver.hip", "wtv_graph_build.hip"]
-HOST_SOURCES = ["wtv_server.cpp", "wtv_capi.cpp"]
+HIP_SOURCES = ["wtv_solver.hip", "wtv_graph_build.hip", "wtv_topology.hip"]
+HOST_SOURCES = ["wtv_server.cpp", "wtv_upload.cpp", "wtv_capi.cpp"]
 
 # -packed-fp32-ops: no v_pk_*_f32 generation (the kernels are integer work;
 # this keeps hipcc from forming packed fp32 ops anywhere in the library).

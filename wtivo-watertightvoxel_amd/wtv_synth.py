@@ -143,6 +143,23 @@ def kuhn_topology_t(k: int, device=None, chunk_cells: int = 1 << 22):
     return tets, nbrs
 
 
+def permute_topology(tets: np.ndarray, nbrs: np.ndarray, seed: int):
+    """The same mesh with its tets renumbered and each tet's vertices reordered,
+    with the neighbour table that follows: new slot f of new tet pi[i] leaves
+    out new local vertex 3-f, which is old local vertex sig[i, 3-f], i.e. old
+    slot 3 - sig[i, 3-f].  Returns (tets, nbrs) of the permuted mesh."""
+    rng = np.random.default_rng(seed)
+    N = tets.shape[0]
+    pi = rng.permutation(N)                       # old id -> new id
+    sig = np.argsort(rng.random((N, 4)), axis=1)  # new local k <- old local sig[i, k]
+    new_t = np.empty_like(tets)
+    new_t[pi] = np.take_along_axis(tets, sig, axis=1)
+    old_n = np.take_along_axis(nbrs, 3 - sig[:, ::-1], axis=1)
+    new_n = np.empty_like(nbrs)
+    new_n[pi] = np.where(old_n >= 0, pi[np.maximum(old_n, 0)], -1)
+    return new_t, new_n
+
+
 def reverse_slots(nbrs: np.ndarray) -> np.ndarray:
     """rs[i,k] = slot of i in nbrs[nbrs[i,k]] (or -1 for hull / missing)."""
     N = nbrs.shape[0]
