@@ -276,7 +276,7 @@ void wtv_surface_free(wtv_surface* s);
  * hull.  d_tets int32[N*4] holds vertex ids in [0, nv); d_nbr int32[N*4] is
  * written.  WTV_EINVAL for an out-of-range or repeated vertex id in a tet and
  * for a face held by more than two tets; WTV_ENOMEM when the face table
- * (4 B per entry, a power of two >= 6N entries; 8 B for N >= 2^30) does not
+ * (4 B per entry, a power of two >= 6N entries; 8 B for N >= 2^29) does not
  * fit.  Synchronous on `stream`. */
 int wtv_tet_neighbors(const int32_t* d_tets, size_t N, size_t nv, int32_t* d_nbr, void* stream);
 

@@ -47,6 +47,18 @@ def test_neighbors_kuhn_6m_tets_closed_form(torch_cuda):
     assert torch_cuda.equal(out, nbrs)
 
 
+def test_neighbors_kuhn_vertex_ids_shuffled(torch_cuda):
+    """Random vertex numbering takes the hashed-start path; the result is the same."""
+    torch = torch_cuda
+    k = 60
+    tets, nbrs = S.kuhn_topology_t(k, "cuda")
+    g = torch.Generator(device="cpu")
+    g.manual_seed(5)
+    perm = torch.randperm((k + 1) ** 3, generator=g).to(torch.int32).cuda()
+    out = T.tet_neighbors_cuda(perm[tets.long()].contiguous(), (k + 1) ** 3)
+    assert torch.equal(out, nbrs)
+
+
 @pytest.mark.parametrize("morton", [False, True])
 def test_neighbors_delaunay_match_qhull(orc, torch_cuda, morton):
     pts = S.genus3_point_cloud(200_000, 11)[0]
