@@ -75,6 +75,8 @@ def main():
                     help="skip the graph_cut_fast leg (reference capacity model, full drop-in call)")
     ap.add_argument("--no-topology-leg", action="store_true",
                     help="skip the topology leg (host upload and device neighbour construction at --k)")
+    ap.add_argument("--no-udf-leg", action="store_true",
+                    help="skip the sparse-grid tet label leg (SparseUDF.sample_tet_labels at --k)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy three times (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     ap.add_argument("--atomic-probe", action="store_true",
@@ -263,6 +265,10 @@ def main():
     if not a.no_topology_leg:
         topology = topology_leg(a, torch, np, W, S, dev, stream)
 
+    udf = None
+    if not a.no_udf_leg:
+        udf = udf_leg(a, torch, np, W, S, dev, stream)
+
     delaunay = None
     if a.delaunay_points > 0:
         delaunay = delaunay_leg(a, torch, np, W, S, dev, stream)
@@ -315,6 +321,7 @@ def main():
         "atomics": atomics,
         "graph_cut_fast": cutleg,
         "topology": topology,
+        "udf_labels": udf,
         "delaunay": delaunay,
         "batched": batched,
         "serving": serving,
@@ -599,6 +606,65 @@ def topology_leg(a, torch, np, W, S, dev, stream):
         f"({nbytes / up_f / 1e9:.1f} GB/s), C {up_c * 1e3:.1f} ms, reference loop "
         f"{out_d.get('reference_upload_s', float('nan')) * 1e3:.1f} ms, pinned ceiling {pcie:.1f} GB/s")
     return out_d
+
+
+def udf_leg(a, torch, np, W, S, dev, stream):
+    """SparseUDF.sample_tet_labels on the device at the Kuhn k^3 size: an
+    unsigned distance grid of the genus-3 surface (voxel 1/1024, 6-voxel band,
+    8^3 leaves) sampled at every tet centroid, threshold one Kuhn cell.  Timed
+    per call with host vertices and host labels (the reference's interface)
+    and with labels left on the device.  The oracle's restatement labels the
+    first 2M tets on one core (CPU baseline) and must agree bit for bit."""
+    import ctypes as C
+    k = a.k
+    t = time.time()
+    V, T, _ = S.kuhn_mesh_t(k, seed=a.seed, device=dev)
+    Vh = np.ascontiguousarray(V.cpu().numpy())
+    del V
+    N, nv = int(T.shape[0]), int(Vh.shape[0])
+    vs, band = 1.0 / 1024, 6.0
+    o, vals, bg = S.udf_leaves(S.genus3(a.seed), vs, band, device=dev)
+    gen_s = time.time() - t
+    L = W.lib()
+    h = C.c_void_p(0)
+    W.check(L.wtv_udf_create(o.ctypes.data, vals.ctypes.data, o.shape[0], float(bg), vs, C.byref(h), stream))
+    bmin, bmax = np.zeros(3), np.ones(3)
+    thr = 1.0 / k
+    lab = np.empty(N, np.uint8)
+    lab_d = torch.empty(N, dtype=torch.uint8, device=dev)
+    try:
+        def call(dst, on_dev):
+            t0 = time.perf_counter()
+            W.check(L.wtv_udf_sample_tet_labels(h, Vh.ctypes.data, nv, T.data_ptr(), N, bmin.ctypes.data,
+                                                bmax.ctypes.data, 0.0, thr, dst, on_dev, None, stream))
+            return time.perf_counter() - t0
+
+        call(lab.ctypes.data, 0)
+        host_s = min(call(lab.ctypes.data, 0) for _ in range(3))
+        dev_s = min(call(lab_d.data_ptr(), 1) for _ in range(3))
+        assert np.array_equal(lab_d.cpu().numpy(), lab), "udf leg: device and host label paths differ"
+    finally:
+        W.check(L.wtv_udf_destroy(h))
+    out = {"workload": f"SparseUDF.sample_tet_labels on Kuhn {k}^3 ({N} tets): genus-3 unsigned distance grid, "
+                       f"voxel 1/1024, band {band:g} voxels, {o.shape[0]} leaves; threshold 1/{k}",
+           "tets": N, "leaves": int(o.shape[0]), "grid_bytes": int(o.shape[0]) * 2048,
+           "call_host_labels_s": round(host_s, 4), "call_device_labels_s": round(dev_s, 4),
+           "tets_per_s": round(N / dev_s, 1), "label0_fraction": round(float((lab == 0).mean()), 5),
+           "gen_seconds": round(gen_s, 1)}
+    if not a.no_cpu:
+        import oracle
+        m = min(N, 2_000_000)
+        Th = T[:m].cpu().numpy()
+        t0 = time.perf_counter()
+        want, _ = oracle.sample_labels(o, vals, bg, vs, Vh, Th, bmin, bmax, 0.0, thr)
+        cpu_s = time.perf_counter() - t0
+        assert np.array_equal(want, lab[:m]), "udf leg: labels differ from the oracle's"
+        out["cpu_baseline"] = {"value": round(m / cpu_s, 1), "unit": "tets/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle restatement (C, binary-searched leaves) on the first {m} tets; "
+                                         "labels equal to the device's"}
+    log(f"[bench] udf labels: {N} tets, {o.shape[0]} leaves, call {host_s * 1e3:.1f} ms (host labels), "
+        f"{dev_s * 1e3:.1f} ms (device labels)")
+    return out
 
 
 def delaunay_leg(a, torch, np, W, S, dev, stream):

@@ -23,6 +23,8 @@
  *   wtv_tet_neighbors   <- the neighbour export of tetrahedralize_neighbors
  *                          (native/core/wtivo_core.cpp:136-154)
  *   wtv_topology_upload <- topology_to_cuda_chunked (wtivo.py:284-309)
+ *   wtv_udf_*           <- SparseUDF's grid and SparseUDF.sample_tet_labels
+ *                          (native/vdb/wtivo_vdb.cpp:1006-1075,1519-1584)
  */
 #ifndef WTV_GPUPR_H
 #define WTV_GPUPR_H
@@ -290,6 +292,33 @@ int wtv_tet_neighbors(const int32_t* d_tets, size_t N, size_t nv, int32_t* d_nbr
  * arrived. */
 int wtv_topology_upload(const int32_t* host, size_t rows, int64_t row_stride, int64_t col_stride,
                         int32_t* d_out, int threads, void* stream);
+
+/* Sparse distance grid in HBM, the grid SparseUDF samples: a float grid with
+ * a uniform linear transform (index = world / voxel_size) stored as 8^3
+ * leaves.  leaf_origins int32[n_leaves*3] are index-space leaf origins
+ * (multiples of 8, distinct); leaf_values float[n_leaves*512] hold each
+ * leaf's voxels in OpenVDB order, offset (x&7)<<6 | (y&7)<<3 | (z&7); every
+ * voxel outside the leaves reads `background`.  Both arrays are host memory,
+ * copied to the device.  The device keeps the values plus a dense int32 index
+ * over the leaf cells' bounding box (WTV_ENOMEM beyond 2^31 cells).  On
+ * success *udf is a handle for wtv_udf_sample_tet_labels; release it with
+ * wtv_udf_destroy. */
+int wtv_udf_create(const int32_t* leaf_origins, const float* leaf_values, size_t n_leaves,
+                   float background, double voxel_size, void** udf, void* stream);
+int wtv_udf_destroy(void* udf);
+
+/* SparseUDF.sample_tet_labels on the device: label = 0 when the trilinear
+ * (OpenVDB BoxSampler) sample of the grid at the tet's centroid, mapped into
+ * the padded unit box ((c - bbox_min) / (bbox_max - bbox_min) * (1 - 2 *
+ * padding) + padding), is <= threshold, else 1.  vertices float64[nv*3] on
+ * the host; d_tets int32[N*4] on the device; labels uint8[N] on the host
+ * (labels_on_device = 0) or the device (1).  *seconds (may be NULL) receives
+ * the call's wall time, the reference's second result.  WTV_EINVAL for a
+ * degenerate bbox or a vertex id outside [0, nv). */
+int wtv_udf_sample_tet_labels(const void* udf, const double* vertices, size_t nv, const int32_t* d_tets,
+                              size_t N, const double* bbox_min, const double* bbox_max, double padding,
+                              double threshold, uint8_t* labels, int labels_on_device, double* seconds,
+                              void* stream);
 
 /* Stats of the last wtv_pr_solve / wtv_graph_cut_fast on this thread. */
 void wtv_last_stats(wtv_stats* out);

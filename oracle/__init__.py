@@ -310,3 +310,32 @@ def topology_to_cuda_reference(host_array, torch, chunk_rows: int = 262144):
         del h
     torch.cuda.synchronize()
     return out
+
+
+def sample_labels(origins, values, background, voxel_size, V, tets, bmin, bmax, padding, threshold):
+    """SparseUDF.sample_tet_labels (native/vdb/wtivo_vdb.cpp:1519-1584) with
+    OpenVDB's BoxSampler, restated in wtv_oracle.c.  Returns (labels uint8[N],
+    sampled values float32[N])."""
+    L = lib()
+    if not hasattr(L, "_udf_ready"):
+        L.wtvo_sample_labels.restype = C.c_int
+        L.wtvo_sample_labels.argtypes = [C.c_size_t, C.c_void_p, C.c_void_p, C.c_float, C.c_double, C.c_size_t,
+                                         C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                         C.c_double, C.c_void_p, C.c_void_p]
+        L._udf_ready = True
+    origins = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1, 3)
+    values = np.ascontiguousarray(values, dtype=np.float32).reshape(-1, 512)
+    V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1, 3)
+    tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+    bmin = np.ascontiguousarray(bmin, dtype=np.float64).reshape(3)
+    bmax = np.ascontiguousarray(bmax, dtype=np.float64).reshape(3)
+    N = tets.shape[0]
+    labels = np.empty(max(N, 1), np.uint8)
+    vals = np.empty(max(N, 1), np.float32)
+    rc = L.wtvo_sample_labels(origins.shape[0], origins.ctypes.data, values.ctypes.data, float(background),
+                              float(voxel_size), V.shape[0], V.ctypes.data, N, tets.ctypes.data, bmin.ctypes.data,
+                              bmax.ctypes.data, float(padding), float(threshold), labels.ctypes.data,
+                              vals.ctypes.data)
+    if rc != OK:
+        raise OracleError(rc, "sample_labels")
+    return labels[:N], vals[:N]

@@ -20,6 +20,13 @@
  *                      graph-cut solver, used as the CPU baseline.
  *   wtvo_certificate   max-flow/min-cut certificate (no reference counterpart;
  *                      the reference ships no checker).
+ *   wtvo_sample_labels native/vdb/wtivo_vdb.cpp:1519-1584 with OpenVDB's
+ *                      BoxSampler (trilinear, openvdb/tools/Interpolation.h)
+ *                      restated from its published source; OpenVDB is not in
+ *                      this image, so parity with OpenVDB itself is pinned
+ *                      only through the closed forms in tests/test_oracle.py
+ *                      (constant and linear fields, voxel-centre samples,
+ *                      background outside leaves).
  *
  * Pinning (what makes these functions trustworthy, see tests/test_oracle.py):
  *   capacity  - hand-computed known answers (right-triangle faces), the
@@ -896,4 +903,124 @@ int wtvo_surface(size_t N, const uint8_t* labels, const double* V, size_t nv,
     free(raw);
     free(remap);
     return WTVO_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Tet labels from a sparse grid: SparseUDF.sample_tet_labels               */
+/* (native/vdb/wtivo_vdb.cpp:1519-1584) with OpenVDB's BoxSampler as        */
+/* published (openvdb/tools/Interpolation.h: floorVec3, uvw = p - floor(p), */
+/* trilinearInterpolation's lerp a + ValueT((b - a) * w), z then y then x)  */
+/* and the linear transform of Transform::createLinearTransform(voxel)      */
+/* (index = world * (1 / voxel)).  Leaves are found by binary search over   */
+/* their sorted cell keys.                                                   */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int64_t key;
+    int64_t leaf;
+} wtvo_leaf_key;
+
+static int64_t wtvo_cell_key(int64_t cx, int64_t cy, int64_t cz)
+{
+    /* cells lie within +-2^20 on each axis for any grid this oracle is given */
+    return ((cx + (1 << 20)) << 42) | ((cy + (1 << 20)) << 21) | (cz + (1 << 20));
+}
+
+static int wtvo_key_cmp(const void* a, const void* b)
+{
+    const int64_t x = ((const wtvo_leaf_key*)a)->key, y = ((const wtvo_leaf_key*)b)->key;
+    return (x > y) - (x < y);
+}
+
+static float wtvo_grid_value(const wtvo_leaf_key* keys, size_t L, const float* values, float background,
+                             int64_t x, int64_t y, int64_t z)
+{
+    const int64_t cx = x >> 3, cy = y >> 3, cz = z >> 3;
+    const int64_t lim = (int64_t)1 << 20;
+    if (cx < -lim || cx >= lim || cy < -lim || cy >= lim || cz < -lim || cz >= lim) return background;
+    wtvo_leaf_key probe = {wtvo_cell_key(cx, cy, cz), 0};
+    const wtvo_leaf_key* hit = (const wtvo_leaf_key*)bsearch(&probe, keys, L, sizeof(wtvo_leaf_key), wtvo_key_cmp);
+    if (!hit) return background;
+    const size_t off = ((size_t)(x & 7) << 6) | ((size_t)(y & 7) << 3) | (size_t)(z & 7);
+    return values[(size_t)hit->leaf * 512u + off];
+}
+
+static float wtvo_lerp(float a, float b, double w)
+{
+    const double t = (double)(b - a) * w;
+    return a + (float)t;
+}
+
+/* labels[i] (and, if values_out, the sampled value) for every tet.  Returns
+ * WTVO_EINVAL for a degenerate bbox, a vertex id out of range, a leaf origin
+ * that is not a multiple of 8 or outside +-2^23, or two leaves on one cell. */
+int wtvo_sample_labels(size_t L, const int32_t* origins, const float* values, float background,
+                       double voxel_size, size_t nv, const double* V, size_t N, const int32_t* tets,
+                       const double* bmin, const double* bmax, double padding, double threshold,
+                       uint8_t* labels, float* values_out)
+{
+    const double sx = bmax[0] - bmin[0], sy = bmax[1] - bmin[1], sz = bmax[2] - bmin[2];
+    if (!(sx > 0 && sy > 0 && sz > 0)) return WTVO_EINVAL;
+    wtvo_leaf_key* keys = (wtvo_leaf_key*)malloc((L ? L : 1) * sizeof(wtvo_leaf_key));
+    if (!keys) return WTVO_ENOMEM;
+    for (size_t l = 0; l < L; ++l) {
+        const int32_t* o = origins + 3u * l;
+        for (int a = 0; a < 3; ++a)
+            if ((o[a] & 7) || o[a] < -(1 << 23) || o[a] >= (1 << 23)) {
+                free(keys);
+                return WTVO_EINVAL;
+            }
+        keys[l].key = wtvo_cell_key(o[0] >> 3, o[1] >> 3, o[2] >> 3);
+        keys[l].leaf = (int64_t)l;
+    }
+    qsort(keys, L, sizeof(wtvo_leaf_key), wtvo_key_cmp);
+    for (size_t l = 1; l < L; ++l)
+        if (keys[l].key == keys[l - 1].key) {
+            free(keys);
+            return WTVO_EINVAL;
+        }
+    const double scale = 1.0 - 2.0 * padding;
+    const double inv = 1.0 / voxel_size;
+    int rc = WTVO_OK;
+    for (size_t i = 0; i < N; ++i) {
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        for (int q = 0; q < 4; ++q) {
+            const int32_t vi = tets[4u * i + (size_t)q];
+            if (vi < 0 || (size_t)vi >= nv) {
+                free(keys);
+                return WTVO_EINVAL;
+            }
+            cx += V[3u * (size_t)vi + 0];
+            cy += V[3u * (size_t)vi + 1];
+            cz += V[3u * (size_t)vi + 2];
+        }
+        cx *= 0.25;
+        cy *= 0.25;
+        cz *= 0.25;
+        const double p[3] = {(((cx - bmin[0]) / sx) * scale + padding) * inv,
+                             (((cy - bmin[1]) / sy) * scale + padding) * inv,
+                             (((cz - bmin[2]) / sz) * scale + padding) * inv};
+        int64_t ix[3];
+        double uvw[3];
+        for (int a = 0; a < 3; ++a) {
+            double f = floor(p[a]);
+            uvw[a] = p[a] - f;
+            if (f < -1099511627776.0) f = -1099511627776.0;
+            if (f > 1099511627776.0) f = 1099511627776.0;
+            ix[a] = (int64_t)f;
+        }
+        float v[2][2][2];
+        for (int dx = 0; dx < 2; ++dx)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dz = 0; dz < 2; ++dz)
+                    v[dx][dy][dz] = wtvo_grid_value(keys, L, values, background, ix[0] + dx, ix[1] + dy, ix[2] + dz);
+        const float d = wtvo_lerp(wtvo_lerp(wtvo_lerp(v[0][0][0], v[0][0][1], uvw[2]),
+                                            wtvo_lerp(v[0][1][0], v[0][1][1], uvw[2]), uvw[1]),
+                                  wtvo_lerp(wtvo_lerp(v[1][0][0], v[1][0][1], uvw[2]),
+                                            wtvo_lerp(v[1][1][0], v[1][1][1], uvw[2]), uvw[1]),
+                                  uvw[0]);
+        labels[i] = ((double)d <= threshold) ? 0u : 1u;
+        if (values_out) values_out[i] = d;
+    }
+    free(keys);
+    return rc;
 }

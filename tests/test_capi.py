@@ -103,3 +103,18 @@ def test_topology_to_cuda_chunked_shape_check_like_reference():
     import wtv_topology as T
     with pytest.raises(RuntimeError, match=r"tets: expected Nx4 topology array, got \(3, 3\)"):
         T.topology_to_cuda_chunked(np.zeros((3, 3), np.int32), "tets")
+
+
+def test_udf_entry_points_reject_bad_arguments_without_device():
+    L = W.lib()
+    h = C.c_void_p(0)
+    assert L.wtv_udf_create(None, None, 0, 1.0, 0.1, C.byref(h), None) == W.WTV_EINVAL
+    assert "grid is empty" in L.wtv_last_error().decode()
+    o = (C.c_int32 * 3)(0, 0, 4)
+    v = (C.c_float * 512)()
+    assert L.wtv_udf_create(o, v, 1, 1.0, 0.1, C.byref(h), None) == W.WTV_EINVAL    # origin not a multiple of 8
+    assert L.wtv_udf_create(o, v, 1, 1.0, 0.1, None, None) == W.WTV_EINVAL          # no handle pointer
+    b0 = (C.c_double * 3)(0, 0, 0)
+    b1 = (C.c_double * 3)(1, 1, 1)
+    assert L.wtv_udf_sample_tet_labels(None, None, 0, None, 0, b0, b1, 0.0, 0.5, None, 0, None, None) == W.WTV_EINVAL
+    assert "grid is empty" in L.wtv_last_error().decode()

@@ -392,3 +392,117 @@ def test_tet_neighbors_planted_slot_mistake_is_caught(orc):
     tets, nbrs = S.kuhn_topology(3)
     wrong = orc.tet_neighbors(tets)[:, ::-1]
     assert not np.array_equal(wrong, nbrs)
+
+
+# ------------------------------------------------------------ tet labels from a sparse grid
+def _tets_centred_at(pts, h=2.0 ** -12):
+    """One tet per point whose FP64 centroid is exactly the (dyadic) point."""
+    off = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0]], np.float64) * h
+    V = (pts[:, None, :] + off[None]).reshape(-1, 3)
+    return V, np.arange(4 * len(pts), dtype=np.int32).reshape(-1, 4)
+
+
+def _block_leaves(n_leaf_cells, fn, lo_cell=0):
+    """Leaves covering cells [lo_cell, lo_cell + n)^3 with voxel values fn(x, y, z) (index coords)."""
+    c = np.arange(lo_cell, lo_cell + n_leaf_cells)
+    cells = np.stack(np.meshgrid(c, c, c, indexing="ij"), -1).reshape(-1, 3)
+    o = np.arange(8)
+    off = np.stack(np.meshgrid(o, o, o, indexing="ij"), -1).reshape(512, 3)   # (x<<6)|(y<<3)|z
+    idx = cells[:, None, :] * 8 + off[None]
+    vals = fn(idx[..., 0], idx[..., 1], idx[..., 2]).astype(np.float32)
+    return (cells * 8).astype(np.int32), vals
+
+
+def _dense_trilinear(origins, values, bg, p):
+    """Independent float64 trilinear over a dense copy of the grid (background outside leaves)."""
+    lo = origins.min(0) - 8
+    hi = origins.max(0) + 16
+    D = np.full(tuple(hi - lo), bg, np.float64)
+    o = np.arange(8)
+    off = np.stack(np.meshgrid(o, o, o, indexing="ij"), -1).reshape(512, 3)
+    for org, val in zip(origins, values):
+        q = org - lo + off
+        D[q[:, 0], q[:, 1], q[:, 2]] = val
+    f = np.floor(p)
+    u = p - f
+    i = f.astype(np.int64) - lo
+    out = np.zeros(len(p))
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                wgt = (u[:, 0] if dx else 1 - u[:, 0]) * (u[:, 1] if dy else 1 - u[:, 1]) * \
+                      (u[:, 2] if dz else 1 - u[:, 2])
+                out += wgt * D[i[:, 0] + dx, i[:, 1] + dy, i[:, 2] + dz]
+    return out
+
+
+UNIT = (np.zeros(3), np.ones(3))
+
+
+def test_sample_labels_constant_field(orc):
+    o, v = _block_leaves(3, lambda x, y, z: np.full(x.shape, 0.5))
+    rng = np.random.default_rng(1)
+    V, t = _tets_centred_at(rng.integers(-64, 400, (500, 3)) / 256.0)
+    for thr, want in ((0.5, 0), (0.4999, 1)):
+        lab, val = orc.sample_labels(o, v, 0.5, 1 / 64, V, t, *UNIT, 0.0, thr)
+        assert (val == np.float32(0.5)).all() and (lab == want).all()
+
+
+def test_sample_labels_linear_field_closed_form(orc):
+    # trilinear interpolation reproduces x + 2y + 3z; at dyadic points every
+    # float step is exact, so a swapped axis or leaf offset shows at once
+    o, v = _block_leaves(4, lambda x, y, z: x + 2.0 * y + 3.0 * z)
+    rng = np.random.default_rng(2)
+    idx = rng.integers(0, 30 * 16, (2000, 3)) / 16.0          # index coords in [0, 30)
+    V, t = _tets_centred_at(idx / 64.0)                       # voxel 1/64: world = index / 64
+    lab, val = orc.sample_labels(o, v, 0.0, 1 / 64, V, t, *UNIT, 0.0, 40.0)
+    want = idx[:, 0] + 2 * idx[:, 1] + 3 * idx[:, 2]
+    assert np.array_equal(val.astype(np.float64), want)
+    assert np.array_equal(lab, (want > 40.0).astype(np.uint8))
+
+
+def test_sample_labels_voxel_centres_return_stored_values(orc):
+    rng = np.random.default_rng(3)
+    o, v = _block_leaves(2, lambda x, y, z: rng.random(x.shape))
+    idx = rng.integers(0, 15, (1000, 3)).astype(np.float64)
+    V, t = _tets_centred_at(idx / 64.0)
+    lab, val = orc.sample_labels(o, v, 9.0, 1 / 64, V, t, *UNIT, 0.0, 0.5)
+    flat = {tuple(oo): vv for oo, vv in zip(o, v)}
+    want = np.array([flat[tuple((q // 8 * 8).astype(int))][int(q[0] % 8) * 64 + int(q[1] % 8) * 8 + int(q[2] % 8)]
+                     for q in idx], np.float32)
+    assert np.array_equal(val, want)
+    assert np.array_equal(lab, (want > 0.5).astype(np.uint8))
+    # threshold is inclusive: a sample equal to it is label 0
+    lab2, _ = orc.sample_labels(o, v, 9.0, 1 / 64, V[:4], t[:1], *UNIT, 0.0, float(want[0]))
+    assert lab2[0] == 0
+
+
+def test_sample_labels_sparse_grid_matches_dense_float64(orc):
+    rng = np.random.default_rng(4)
+    o, v = _block_leaves(4, lambda x, y, z: rng.random(x.shape) * 3)
+    keep = rng.random(len(o)) < 0.6                           # holes -> background stencils
+    o, v = o[keep], v[keep]
+    bmin, bmax, pad = np.array([-1.0, -2.0, 0.5]), np.array([3.0, 2.0, 2.5]), 0.05
+    pts = bmin + rng.random((3000, 3)) * (bmax - bmin)
+    V, t = _tets_centred_at(pts, h=1e-7)
+    vs = 1 / 40.0
+    lab, val = orc.sample_labels(o, v, 2.0, vs, V, t, bmin, bmax, pad, 1.0)
+    c = V.reshape(-1, 4, 3).sum(1) * 0.25
+    p = (((c - bmin) / (bmax - bmin)) * (1 - 2 * pad) + pad) * (1 / vs)
+    ref = _dense_trilinear(o, v, 2.0, p)
+    assert np.allclose(val, ref, rtol=0, atol=2e-6)
+    far = np.abs(ref - 1.0) > 1e-5
+    assert np.array_equal(lab[far], (ref[far] > 1.0).astype(np.uint8))
+
+
+def test_sample_labels_errors(orc):
+    o, v = _block_leaves(1, lambda x, y, z: np.zeros(x.shape))
+    V, t = _tets_centred_at(np.array([[0.05, 0.05, 0.05]]))
+    with pytest.raises(orc.OracleError):
+        orc.sample_labels(o, v, 1.0, 1 / 64, V, t, np.zeros(3), np.array([1.0, 0.0, 1.0]), 0.0, 0.5)
+    with pytest.raises(orc.OracleError):
+        orc.sample_labels(o, v, 1.0, 1 / 64, V, t + 10, *UNIT, 0.0, 0.5)
+    with pytest.raises(orc.OracleError):
+        orc.sample_labels(np.concatenate([o, o]), np.concatenate([v, v]), 1.0, 1 / 64, V, t, *UNIT, 0.0, 0.5)
+    with pytest.raises(orc.OracleError):
+        orc.sample_labels(o + 3, v, 1.0, 1 / 64, V, t, *UNIT, 0.0, 0.5)

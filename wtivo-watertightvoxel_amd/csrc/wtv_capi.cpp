@@ -163,6 +163,29 @@ int wtv_topology_upload(const int32_t* host, size_t rows, int64_t row_stride, in
     return guarded([&]() { wtv::topology_upload(host, rows, row_stride, col_stride, d_out, threads, stream); });
 }
 
+int wtv_udf_create(const int32_t* leaf_origins, const float* leaf_values, size_t n_leaves, float background,
+                   double voxel_size, void** udf, void* stream) {
+    return guarded([&]() {
+        if (!udf) throw wtv::Error(WTV_EINVAL, "null udf handle pointer");
+        *udf = nullptr;
+        *udf = wtv::udf_create(leaf_origins, leaf_values, n_leaves, background, voxel_size, stream);
+    });
+}
+
+int wtv_udf_destroy(void* udf) {
+    return guarded([&]() { wtv::udf_destroy(udf); });
+}
+
+int wtv_udf_sample_tet_labels(const void* udf, const double* vertices, size_t nv, const int32_t* d_tets, size_t N,
+                              const double* bbox_min, const double* bbox_max, double padding, double threshold,
+                              uint8_t* labels, int labels_on_device, double* seconds, void* stream) {
+    return guarded([&]() {
+        const double t = wtv::udf_sample_tet_labels(udf, vertices, nv, d_tets, N, bbox_min, bbox_max, padding,
+                                                    threshold, labels, labels_on_device != 0, stream);
+        if (seconds) *seconds = t;
+    });
+}
+
 uint64_t wtv_request_footprint(uint32_t n) { return wtv::request_footprint(n); }
 
 void* wtv_server_create(const wtv_server_config* cfg) {

@@ -61,6 +61,15 @@ void tet_neighbors(const int32_t* d_tets, size_t N, size_t nv, int32_t* d_nbr, v
 void topology_upload(const int32_t* host, size_t rows, int64_t row_stride, int64_t col_stride, int32_t* d_out,
                      int threads, void* stream);
 
+// Sparse distance grid on the device and tet labels from it (wtv_udf.hip).
+void* udf_create(const int32_t* origin, const float* values, size_t L, float background, double voxel_size,
+                 void* stream);
+void udf_destroy(void* udf);
+uint64_t udf_bytes(const void* udf);
+double udf_sample_tet_labels(const void* udf, const double* V, size_t nv, const int32_t* d_tets, size_t N,
+                             const double* bmin, const double* bmax, double padding, double threshold,
+                             uint8_t* labels, bool labels_on_device, void* stream);
+
 // Thread-local stats of the last solve (wtv_capi.cpp).
 void set_last_stats(const wtv_stats& st);
 

@@ -17,7 +17,8 @@ WTV_OK, WTV_EINVAL, WTV_EOVERFLOW, WTV_EINVARIANT, WTV_ENOMEM, WTV_ELIMIT, WTV_E
 EXPORTS = ["wtv_params_default", "wtv_pr_solve", "wtv_batch_solve", "wtv_graph_cut_fast", "wtv_last_stats",
            "wtv_last_error", "wtv_version", "wtv_build_reduced_graph", "wtv_surface_extract",
            "wtv_surface_free", "wtv_request_footprint", "wtv_server_create", "wtv_server_submit",
-           "wtv_server_wait", "wtv_server_destroy", "wtv_tet_neighbors", "wtv_topology_upload"]
+           "wtv_server_wait", "wtv_server_destroy", "wtv_tet_neighbors", "wtv_topology_upload",
+           "wtv_udf_create", "wtv_udf_destroy", "wtv_udf_sample_tet_labels"]
 
 
 class WtvStats(C.Structure):
@@ -133,6 +134,15 @@ def lib():
         L.wtv_topology_upload.restype = C.c_int
         L.wtv_topology_upload.argtypes = [C.c_void_p, C.c_size_t, C.c_int64, C.c_int64, C.c_void_p, C.c_int,
                                           C.c_void_p]
+        L.wtv_udf_create.restype = C.c_int
+        L.wtv_udf_create.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_float, C.c_double,
+                                     C.POINTER(C.c_void_p), C.c_void_p]
+        L.wtv_udf_destroy.restype = C.c_int
+        L.wtv_udf_destroy.argtypes = [C.c_void_p]
+        L.wtv_udf_sample_tet_labels.restype = C.c_int
+        L.wtv_udf_sample_tet_labels.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
+                                                C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p,
+                                                C.c_int, C.POINTER(C.c_double), C.c_void_p]
         L.wtv_last_stats.restype = None
         L.wtv_last_stats.argtypes = [C.POINTER(WtvStats)]
         L.wtv_last_error.restype = C.c_char_p

@@ -691,3 +691,43 @@ def delaunay_genus3_graph(npts: int, seed: int, lam: float = 10.0, flip: float =
     meta = {"points": npts, "tets": mesh.N, "res": res, "h": 1.0 / res, "lam": lam, "flip": flip,
             "inside_frac": float((lab == 0).float().mean())}
     return g, mesh, meta
+
+
+def udf_leaves(shape: Shape, voxel_size: float, band_voxels: float, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
+               device=None, chunk: int = 1 << 21):
+    """Sparse unsigned distance grid of `shape`'s surface in the layout the
+    device sampler takes: 8^3 leaves whose index-space cells come within the
+    band of the surface, voxel (i,j,k) at world (i,j,k) * voxel_size (OpenVDB's
+    linear transform), value min(|sdf|, background) with background =
+    band_voxels * voxel_size, leaf order offset (x&7)<<6 | (y&7)<<3 | (z&7).
+    Returns (origins int32 [L,3], values float32 [L,512], background)."""
+    dev = _dev(device)
+    bg = float(band_voxels * voxel_size)
+    i0 = [int(np.floor(l / voxel_size)) - 1 for l in lo]
+    i1 = [int(np.ceil(h / voxel_size)) + 1 for h in hi]
+    c0 = [v >> 3 for v in i0]
+    c1 = [v >> 3 for v in i1]
+    ext = [b - a + 1 for a, b in zip(c0, c1)]
+    ncell = ext[0] * ext[1] * ext[2]
+    reach = bg + 8 * voxel_size * 0.8661   # band + half the leaf diagonal
+    keep = []
+    for s0 in range(0, ncell, chunk):
+        c = torch.arange(s0, min(ncell, s0 + chunk), dtype=torch.int64, device=dev)
+        cx = c % ext[0] + c0[0]
+        cy = (c // ext[0]) % ext[1] + c0[1]
+        cz = c // (ext[0] * ext[1]) + c0[2]
+        ctr = torch.stack([cx, cy, cz], 1).to(torch.float64) * 8.0 + 3.5
+        d = shape.sdf(ctr * voxel_size).abs()
+        keep.append(torch.stack([cx, cy, cz], 1)[d <= reach])
+    cells = torch.cat(keep) if keep else torch.zeros((0, 3), dtype=torch.int64, device=dev)
+    o = torch.arange(8, device=dev)
+    off = torch.stack(torch.meshgrid(o, o, o, indexing="ij"), -1).reshape(512, 3)   # x-major: (x<<6)|(y<<3)|z
+    vals = []
+    per = max(1, chunk // 512)
+    for s0 in range(0, cells.shape[0], per):
+        cc = cells[s0:s0 + per]
+        idx = (cc[:, None, :] * 8 + off[None, :, :]).to(torch.float64)
+        d = shape.sdf(idx.reshape(-1, 3) * voxel_size).abs().clamp(max=bg)
+        vals.append(d.to(torch.float32).reshape(-1, 512))
+    values = torch.cat(vals) if vals else torch.zeros((0, 512), dtype=torch.float32, device=dev)
+    return ((cells * 8).to(torch.int32).cpu().numpy(), values.cpu().numpy(), np.float32(bg))
