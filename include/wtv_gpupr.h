@@ -311,14 +311,15 @@ int wtv_udf_destroy(void* udf);
  * (OpenVDB BoxSampler) sample of the grid at the tet's centroid, mapped into
  * the padded unit box ((c - bbox_min) / (bbox_max - bbox_min) * (1 - 2 *
  * padding) + padding), is <= threshold, else 1.  vertices float64[nv*3] on
- * the host; d_tets int32[N*4] on the device; labels uint8[N] on the host
+ * the host (vertices_on_device = 0, as the reference takes them) or already
+ * in HBM (1); d_tets int32[N*4] on the device; labels uint8[N] on the host
  * (labels_on_device = 0) or the device (1).  *seconds (may be NULL) receives
  * the call's wall time, the reference's second result.  WTV_EINVAL for a
  * degenerate bbox or a vertex id outside [0, nv). */
-int wtv_udf_sample_tet_labels(const void* udf, const double* vertices, size_t nv, const int32_t* d_tets,
-                              size_t N, const double* bbox_min, const double* bbox_max, double padding,
-                              double threshold, uint8_t* labels, int labels_on_device, double* seconds,
-                              void* stream);
+int wtv_udf_sample_tet_labels(const void* udf, const double* vertices, size_t nv, int vertices_on_device,
+                              const int32_t* d_tets, size_t N, const double* bbox_min, const double* bbox_max,
+                              double padding, double threshold, uint8_t* labels, int labels_on_device,
+                              double* seconds, void* stream);
 
 /* Stats of the last wtv_pr_solve / wtv_graph_cut_fast on this thread. */
 void wtv_last_stats(wtv_stats* out);

@@ -116,5 +116,5 @@ def test_udf_entry_points_reject_bad_arguments_without_device():
     assert L.wtv_udf_create(o, v, 1, 1.0, 0.1, None, None) == W.WTV_EINVAL          # no handle pointer
     b0 = (C.c_double * 3)(0, 0, 0)
     b1 = (C.c_double * 3)(1, 1, 1)
-    assert L.wtv_udf_sample_tet_labels(None, None, 0, None, 0, b0, b1, 0.0, 0.5, None, 0, None, None) == W.WTV_EINVAL
+    assert L.wtv_udf_sample_tet_labels(None, None, 0, 0, None, 0, b0, b1, 0.0, 0.5, None, 0, None, None) == W.WTV_EINVAL
     assert "grid is empty" in L.wtv_last_error().decode()

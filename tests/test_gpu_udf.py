@@ -86,7 +86,8 @@ def test_labels_on_device_and_graph_cut(orc, torch_cuda):
     W.check(L.wtv_udf_create(oo.ctypes.data, vv.ctypes.data, oo.shape[0], float(bg), 1 / 128, C.byref(h), None))
     try:
         bmin, bmax = np.zeros(3), np.ones(3)
-        W.check(L.wtv_udf_sample_tet_labels(h, np.ascontiguousarray(m.V).ctypes.data, m.V.shape[0], dt.data_ptr(),
+        dV = torch.from_numpy(np.ascontiguousarray(m.V)).cuda()     # vertices resident in HBM
+        W.check(L.wtv_udf_sample_tet_labels(h, dV.data_ptr(), m.V.shape[0], 1, dt.data_ptr(),
                                             m.tets.shape[0], bmin.ctypes.data, bmax.ctypes.data, 0.0, thr,
                                             lab_d.data_ptr(), 1, None, torch.cuda.current_stream().cuda_stream))
     finally:

@@ -176,12 +176,14 @@ int wtv_udf_destroy(void* udf) {
     return guarded([&]() { wtv::udf_destroy(udf); });
 }
 
-int wtv_udf_sample_tet_labels(const void* udf, const double* vertices, size_t nv, const int32_t* d_tets, size_t N,
-                              const double* bbox_min, const double* bbox_max, double padding, double threshold,
-                              uint8_t* labels, int labels_on_device, double* seconds, void* stream) {
+int wtv_udf_sample_tet_labels(const void* udf, const double* vertices, size_t nv, int vertices_on_device,
+                              const int32_t* d_tets, size_t N, const double* bbox_min, const double* bbox_max,
+                              double padding, double threshold, uint8_t* labels, int labels_on_device,
+                              double* seconds, void* stream) {
     return guarded([&]() {
-        const double t = wtv::udf_sample_tet_labels(udf, vertices, nv, d_tets, N, bbox_min, bbox_max, padding,
-                                                    threshold, labels, labels_on_device != 0, stream);
+        const double t = wtv::udf_sample_tet_labels(udf, vertices, nv, vertices_on_device != 0, d_tets, N, bbox_min,
+                                                    bbox_max, padding, threshold, labels, labels_on_device != 0,
+                                                    stream);
         if (seconds) *seconds = t;
     });
 }

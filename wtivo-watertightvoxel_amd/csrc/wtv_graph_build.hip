@@ -546,7 +546,7 @@ bool build_reduced(const double* V, size_t nv, const int32_t* dT, const int32_t*
     if (h.free_n > static_cast<uint64_t>(std::numeric_limits<int32_t>::max()))
         fail(WTV_EINVAL, ">INT32_MAX reduced nodes unsupported");
     ckg(hipMalloc(&dV.p, std::max<size_t>(nv, 1) * 24u), "malloc vertices");
-    if (nv) ckg(hipMemcpyAsync(dV.p, V, nv * 24u, hipMemcpyHostToDevice, s), "vertices H2D");
+    pinned_to_device(V, nv * 24u, dV.p, 0, s);
     k_tile_scan<<<1, kB, 0, s>>>(tiles, dTiles.as<uint32_t>());
     k_assign<<<tiles, kB, 0, s>>>(n32, dCls.as<int32_t>(), dTiles.as<uint32_t>(), free_old);
     const int grid = static_cast<int>(std::min<uint64_t>((N + kB - 1) / kB, 8192));
@@ -586,7 +586,7 @@ void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t
     hipStream_t s = static_cast<hipStream_t>(stream_v);
     Dev dLab;
     ckg(hipMalloc(&dLab.p, N), "malloc labels");
-    ckg(hipMemcpyAsync(dLab.p, lp, N, hipMemcpyHostToDevice, s), "labels H2D");
+    pinned_to_device(lp, N, dLab.p, 0, s);
 
     // VRAM precheck with the reference's estimate (:200-212) plus this
     // solver's workspace and the build's own buffers, for the worst case n = N
@@ -632,7 +632,7 @@ void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t
     k_scatter<<<static_cast<int>(std::min<uint64_t>((free_N + kB - 1) / kB, 8192)), kB, 0, s>>>(
         static_cast<uint32_t>(free_N), dFreeOld.as<uint32_t>(), dPart.as<uint8_t>(), dOut.as<uint8_t>());
     ckg(hipGetLastError(), "scatter");
-    ckg(hipMemcpyAsync(out_labels, dOut.p, N, hipMemcpyDeviceToHost, s), "labels D2H");
+    pinned_to_host(dOut.p, N, out_labels, 0, s);
     ckg(hipStreamSynchronize(s), "labels sync");
 
     R.adjusted_flow = st.raw_flow + bi.base_flow;
@@ -660,7 +660,7 @@ void build_reduced_graph(const double* V, size_t nv, const int32_t* dT, const in
     hipStream_t s = static_cast<hipStream_t>(stream_v);
     Dev dLab;
     ckg(hipMalloc(&dLab.p, N), "malloc labels");
-    ckg(hipMemcpyAsync(dLab.p, labels, N, hipMemcpyHostToDevice, s), "labels H2D");
+    pinned_to_device(labels, N, dLab.p, 0, s);
     build_reduced(V, nv, dT, dN, N, dLab.as<uint8_t>(), fill_T, gN, gR, gT, free_old, info, s);
     ckg(hipStreamSynchronize(s), "build");
 }
@@ -682,7 +682,7 @@ void surface_extract(const uint8_t* lp, const double* V, size_t nv, const int32_
     ckg(hipMalloc(&dLab.p, N), "malloc labels");
     ckg(hipMalloc(&dTiles.p, static_cast<size_t>(tiles) * 4u), "malloc tiles");
     ckg(hipMalloc(&dCtl.p, sizeof(BuildCtl)), "malloc ctl");
-    ckg(hipMemcpyAsync(dLab.p, lp, N, hipMemcpyHostToDevice, s), "labels H2D");
+    pinned_to_device(lp, N, dLab.p, 0, s);
     ckg(hipMemsetAsync(dCtl.p, 0, sizeof(BuildCtl), s), "ctl clear");
     BuildCtl* ctl = dCtl.as<BuildCtl>();
     BuildSums h;
@@ -700,7 +700,7 @@ void surface_extract(const uint8_t* lp, const double* V, size_t nv, const int32_
     ckg(hipMalloc(&dUsed.p, std::max<size_t>(nv, 1)), "malloc used");
     ckg(hipMalloc(&dRemap.p, std::max<size_t>(nv, 1) * 4u), "malloc remap");
     ckg(hipMalloc(&dVt.p, static_cast<size_t>(vtiles) * 4u), "malloc vertex tiles");
-    if (nv) ckg(hipMemcpyAsync(dV.p, V, nv * 24u, hipMemcpyHostToDevice, s), "vertices H2D");
+    pinned_to_device(V, nv * 24u, dV.p, 0, s);
     ckg(hipMemsetAsync(dUsed.p, 0, std::max<size_t>(nv, 1), s), "used clear");
     k_tile_scan<<<1, kB, 0, s>>>(tiles, dTiles.as<uint32_t>());
     k_surf_emit<<<tiles, kB, 0, s>>>(n32, tets4, nbr4, dLab.as<uint8_t>(), dV.as<double>(), nv, dTiles.as<uint32_t>(),

@@ -66,9 +66,14 @@ void* udf_create(const int32_t* origin, const float* values, size_t L, float bac
                  void* stream);
 void udf_destroy(void* udf);
 uint64_t udf_bytes(const void* udf);
-double udf_sample_tet_labels(const void* udf, const double* V, size_t nv, const int32_t* d_tets, size_t N,
-                             const double* bmin, const double* bmax, double padding, double threshold,
-                             uint8_t* labels, bool labels_on_device, void* stream);
+double udf_sample_tet_labels(const void* udf, const double* V, size_t nv, bool vertices_on_device,
+                             const int32_t* d_tets, size_t N, const double* bmin, const double* bmax, double padding,
+                             double threshold, uint8_t* labels, bool labels_on_device, void* stream);
+
+// Pinned, multi-stream copies of a byte range (wtv_upload.cpp); synchronous,
+// ordered after the work queued on `stream`.
+void pinned_to_device(const void* host, size_t bytes, void* d_out, int threads, void* stream);
+void pinned_to_host(const void* d_src, size_t bytes, void* host, int threads, void* stream);
 
 // Thread-local stats of the last solve (wtv_capi.cpp).
 void set_last_stats(const wtv_stats& st);

@@ -202,7 +202,7 @@ void* udf_create(const int32_t* origin, const float* values, size_t L, float bac
     DevBuf d_origin, d_bad;
     cku(hipMalloc(&d_origin.p, L * 3u * sizeof(int32_t)), "origin alloc");
     cku(hipMalloc(&d_bad.p, sizeof(int32_t)), "flag alloc");
-    cku(hipMemcpyAsync(u->values, values, L * 512u * sizeof(float), hipMemcpyHostToDevice, s), "values upload");
+    pinned_to_device(values, L * 512u * sizeof(float), u->values, 0, s);
     cku(hipMemcpyAsync(d_origin.p, origin, L * 3u * sizeof(int32_t), hipMemcpyHostToDevice, s), "origin upload");
     auto* idx = static_cast<int32_t*>(u->index);
     auto* bad = static_cast<int32_t*>(d_bad.p);
@@ -236,9 +236,9 @@ void udf_destroy(void* h) { delete static_cast<Udf*>(h); }
 
 uint64_t udf_bytes(const void* h) { return static_cast<const Udf*>(h)->bytes; }
 
-double udf_sample_tet_labels(const void* h, const double* V, size_t nv, const int32_t* d_tets, size_t N,
-                             const double* bmin, const double* bmax, double padding, double threshold,
-                             uint8_t* labels, bool labels_on_device, void* stream_v) {
+double udf_sample_tet_labels(const void* h, const double* V, size_t nv, bool vertices_on_device,
+                             const int32_t* d_tets, size_t N, const double* bmin, const double* bmax, double padding,
+                             double threshold, uint8_t* labels, bool labels_on_device, void* stream_v) {
     if (!h) fail(WTV_EINVAL, std::string(kPrefix) + "SparseUDF.sample_tet_labels: grid is empty");
     if (!bmin || !bmax) fail(WTV_EINVAL, std::string(kPrefix) + "SparseUDF.sample_tet_labels: bbox must have length 3");
     const double sx = bmax[0] - bmin[0], sy = bmax[1] - bmin[1], sz = bmax[2] - bmin[2];
@@ -250,25 +250,29 @@ double udf_sample_tet_labels(const void* h, const double* V, size_t nv, const in
     auto s = static_cast<hipStream_t>(stream_v);
     const auto t0 = std::chrono::steady_clock::now();
     DevBuf dV, dL, d_bad;
-    cku(hipMalloc(&dV.p, std::max<size_t>(nv, 1) * 24u), "vertices alloc");
+    const double* vdev = V;
+    if (!vertices_on_device) {
+        cku(hipMalloc(&dV.p, std::max<size_t>(nv, 1) * 24u), "vertices alloc");
+        vdev = static_cast<const double*>(dV.p);
+    }
     cku(hipMalloc(&d_bad.p, sizeof(int32_t)), "flag alloc");
     uint8_t* out = labels;
     if (!labels_on_device) {
         cku(hipMalloc(&dL.p, N), "labels alloc");
         out = static_cast<uint8_t*>(dL.p);
     }
-    if (nv) cku(hipMemcpyAsync(dV.p, V, nv * 24u, hipMemcpyHostToDevice, s), "vertices upload");
+    if (!vertices_on_device) pinned_to_device(V, nv * 24u, dV.p, 0, s);
     auto* bad = static_cast<int32_t*>(d_bad.p);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kB), 0, s, bad, 1ull, 0);
     hipLaunchKernelGGL(k_sample_labels, dim3(grid_for(N)), dim3(kB), 0, s, static_cast<uint64_t>(N),
-                       reinterpret_cast<const int4*>(d_tets), static_cast<const double*>(dV.p),
+                       reinterpret_cast<const int4*>(d_tets), vdev,
                        static_cast<uint64_t>(nv), u->g, bmin[0], bmin[1], bmin[2], sx, sy, sz, 1.0 - 2.0 * padding,
                        padding, threshold, out, bad);
     cku(hipGetLastError(), "k_sample_labels launch");
     int32_t h_bad = 0;
     cku(hipMemcpyAsync(&h_bad, bad, sizeof(h_bad), hipMemcpyDeviceToHost, s), "flag read");
-    if (!labels_on_device) cku(hipMemcpyAsync(labels, out, N, hipMemcpyDeviceToHost, s), "labels download");
     cku(hipStreamSynchronize(s), "sample sync");
+    if (!labels_on_device) pinned_to_host(out, N, labels, 0, s);
     if (h_bad) fail(WTV_EINVAL, std::string(kPrefix) + "SparseUDF.sample_tet_labels: tet index out of range");
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }

@@ -140,9 +140,9 @@ def lib():
         L.wtv_udf_destroy.restype = C.c_int
         L.wtv_udf_destroy.argtypes = [C.c_void_p]
         L.wtv_udf_sample_tet_labels.restype = C.c_int
-        L.wtv_udf_sample_tet_labels.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
-                                                C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p,
-                                                C.c_int, C.POINTER(C.c_double), C.c_void_p]
+        L.wtv_udf_sample_tet_labels.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p,
+                                                C.c_size_t, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                                C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_void_p]
         L.wtv_last_stats.restype = None
         L.wtv_last_stats.argtypes = [C.POINTER(WtvStats)]
         L.wtv_last_error.restype = C.c_char_p

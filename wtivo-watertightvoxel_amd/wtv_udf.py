@@ -64,7 +64,7 @@ class SparseUDFGrid:
         N = int(d_tets.size(0)) if d_tets is not None else 0
         labels = np.empty(N, np.uint8)
         sec = C.c_double(0.0)
-        W.check(W.lib().wtv_udf_sample_tet_labels(self._h, V.ctypes.data, V.shape[0],
+        W.check(W.lib().wtv_udf_sample_tet_labels(self._h, V.ctypes.data, V.shape[0], 0,
                                                   d_tets.data_ptr() if N else None, N, bmin.ctypes.data,
                                                   bmax.ctypes.data, float(padding), float(threshold),
                                                   labels.ctypes.data if N else None, 0, C.byref(sec),
