@@ -146,6 +146,21 @@ def test_stress_integer_shell(orc, torch_cuda):
     assert_same(orc, g.nbr, g.res, g.term, gpu, a.raw_flow, a.partition)
 
 
+def test_config4_full_size_integer_shell(orc, torch_cuda):
+    """The adversarial stress at full size: Kuhn 150^3 = 20,250,000 tets, thin shell
+    (wall 1.5 cells) with 40 tunnels, log-normal capacities quantised to [1, 2^20],
+    lambda 100.  Flow and labels bit-exact against Boykov-Kolmogorov, certificate."""
+    k = 150
+    m = S.kuhn_mesh(k, seed=20260004)
+    shape = S.thin_shell_with_tunnels(20260004, 1.0 / k, tunnels=40)
+    lab = S.labels_from_sdf(S.tet_centroids_sdf(m, shape, device="cuda"), 0.0)
+    g = S.lognormal_int_graph(m, lab, 100.0, seed=20260005, rev_slots=S.KUHN_REV_SLOT)
+    print(f"config 4: {m.N} tets", flush=True)
+    a = orc.bk_solve(g.nbr, g.res, g.term)
+    gpu = gpu_solve(torch_cuda, g.nbr, g.res, g.term, device_partition=True)
+    assert_same(orc, g.nbr, g.res, g.term, gpu, a.raw_flow, a.partition)
+
+
 @pytest.mark.parametrize("kw", [dict(local_steps=0), dict(local_steps=100), dict(global_relabel_period=-5),
                                 dict(global_relabel_period=1, local_steps=1), dict(grid_wgs=7),
                                 dict(max_rounds=0), dict(wg_size=512), dict(wg_size=1024),
