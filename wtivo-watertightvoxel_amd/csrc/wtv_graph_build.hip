@@ -24,6 +24,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <sstream>
@@ -584,9 +586,22 @@ void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t
         return;
     }
     hipStream_t s = static_cast<hipStream_t>(stream_v);
+    // WTV_TRACE=1: one stderr line per phase with its wall time (diagnostics)
+    const char* trace_env = std::getenv("WTV_TRACE");
+    const bool trace = trace_env && *trace_env && *trace_env != '0';
+    auto mark = all0;
+    auto phase = [&](const char* what) {
+        if (!trace) return;
+        (void)hipStreamSynchronize(s);
+        const auto now = Clock::now();
+        std::fprintf(stderr, "[wtv] graph_cut_fast %-18s %8.3f ms\n", what,
+                     std::chrono::duration<double, std::milli>(now - mark).count());
+        mark = now;
+    };
     Dev dLab;
     ckg(hipMalloc(&dLab.p, N), "malloc labels");
     pinned_to_device(lp, N, dLab.p, 0, s);
+    phase("labels upload");
 
     // VRAM precheck with the reference's estimate (:200-212) plus this
     // solver's workspace and the build's own buffers, for the worst case n = N
@@ -596,16 +611,19 @@ void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t
                           static_cast<uint64_t>(nv) * 24u + (256ull << 20);
     if (need > static_cast<uint64_t>(gpu_free)) fail(WTV_ENOMEM, "insufficient VRAM for reduced graph + solver");
 
+    phase("vram precheck");
     const auto build0 = Clock::now();
     Dev gN, gR, gT, dFreeOld;
     ckg(hipMalloc(&gN.p, N * 16u), "malloc graph nbr");
     ckg(hipMalloc(&gR.p, N * 16u), "malloc graph res");
     ckg(hipMalloc(&gT.p, N * 4u), "malloc graph term");
     ckg(hipMalloc(&dFreeOld.p, N * 4u), "malloc free_old");
+    phase("graph malloc");
     BuildInfo bi{};
     const bool any = build_reduced(V, nv, dT, dN, N, dLab.as<uint8_t>(), fill_T, gN.as<int32_t>(), gR.as<uint32_t>(),
                                    gT.as<int32_t>(), dFreeOld.as<uint32_t>(), bi, s);
     const auto build1 = Clock::now();
+    phase("build");
     R.full_edges = bi.full_refs / 2u;
     R.fixed_source = bi.fixed_source;
     R.fixed_sink = bi.fixed_sink;
@@ -625,6 +643,7 @@ void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t
     ckg(hipMalloc(&dPart.p, free_N), "malloc partition");
     pr_solve(static_cast<uint32_t>(free_N), gN.as<int32_t>(), gR.as<uint32_t>(), gT.as<int32_t>(), dPart.as<uint8_t>(),
              true, prm, st, stream_v);
+    phase("solve");
 
     // labels (:357-364): out = labels; out[free_old[q]] = partition[q]
     ckg(hipMalloc(&dOut.p, N), "malloc out labels");
@@ -632,8 +651,10 @@ void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t
     k_scatter<<<static_cast<int>(std::min<uint64_t>((free_N + kB - 1) / kB, 8192)), kB, 0, s>>>(
         static_cast<uint32_t>(free_N), dFreeOld.as<uint32_t>(), dPart.as<uint8_t>(), dOut.as<uint8_t>());
     ckg(hipGetLastError(), "scatter");
+    phase("scatter");
     pinned_to_host(dOut.p, N, out_labels, 0, s);
     ckg(hipStreamSynchronize(s), "labels sync");
+    phase("labels download");
 
     R.adjusted_flow = st.raw_flow + bi.base_flow;
     R.raw_flow = st.raw_flow;

@@ -23,6 +23,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -106,6 +109,8 @@ void pack(const int32_t* src, int64_t rs, int64_t cs, size_t r0, size_t r1, int3
 template <class Fill, class Drain>
 void pipeline(bool to_device, size_t chunks, size_t total_bytes, int threads, char* dev, hipStream_t caller,
               Fill fill, Drain drain) {
+    using Clock = std::chrono::steady_clock;
+    const auto t0 = Clock::now();
     std::lock_guard<std::mutex> lk(g_mu);
     int device = 0;
     cku(hipGetDevice(&device), "hipGetDevice");
@@ -162,11 +167,24 @@ void pipeline(bool to_device, size_t chunks, size_t total_bytes, int threads, ch
             (void)hipStreamSynchronize(wk.stream);
         }
     };
+    const auto t1 = Clock::now();
     std::vector<std::thread> pool;
     pool.reserve(W - 1);
     for (int w = 1; w < W; ++w) pool.emplace_back(run, w);
     run(0);
+    const auto t2 = Clock::now();
     for (auto& t : pool) t.join();
+    const auto t3 = Clock::now();
+    static const bool trace = [] {
+        const char* e = std::getenv("WTV_TRACE");
+        return e && *e && *e != '0';
+    }();
+    if (trace)
+        std::fprintf(stderr, "[wtv] copy %s %zu B, %d workers: setup %.3f ms, worker 0 %.3f ms, join %.3f ms\n",
+                     to_device ? "H2D" : "D2H", total_bytes, W,
+                     std::chrono::duration<double, std::milli>(t1 - t0).count(),
+                     std::chrono::duration<double, std::milli>(t2 - t1).count(),
+                     std::chrono::duration<double, std::milli>(t3 - t2).count());
     for (int w = 0; w < W; ++w)
         if (status[w] != WTV_OK) fail(status[w], msg[w]);
 }
