@@ -1240,6 +1240,9 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     const double alpha = prm.relabel_alpha == 0.0 ? 2.0 : prm.relabel_alpha;
     uint64_t relabels_at_gr = stat_sum(*hctl, kStRelabels);
     uint64_t active_at_gr = active;
+    // share of the vertices the last global relabel's BFS expanded
+    uint64_t pops_at_gr = stat_sum(*hctl, kStBfsPops);
+    double last_gr_reach = n ? static_cast<double>(pops_at_gr) / static_cast<double>(n) : 0.0;
 
     // One macro round = local_steps sub-rounds (push + relabel).  Rounds are
     // replayed from a captured graph (one per starting parity) and pipelined:
@@ -1307,11 +1310,20 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         global_relabel();
         active = filter_active();
         relabels_now = relabels_at_gr = stat_sum(*hctl, kStRelabels);
+        const uint64_t pops = stat_sum(*hctl, kStBfsPops);
+        last_gr_reach = n ? static_cast<double>(pops - pops_at_gr) / static_cast<double>(n) : 0.0;
+        pops_at_gr = pops;
         active_at_gr = active;
     };
+    // A relabel whose BFS expands most of the graph costs a sweep of it
+    // however little of the heights the discharge has changed, so it must be
+    // paid for by a longer discharge interval: the cost factor rises from 1
+    // when the last BFS expanded three quarters of the vertices or fewer to 3
+    // when it expanded all of them.
     auto affordable = [&]() -> bool {
-        return std::chrono::duration<double, std::milli>(Clock::now() - last_gr_end).count() >=
-               kRelabelCost * last_gr_ms;
+        const double reach = std::min(1.0, last_gr_reach);
+        const double factor = kRelabelCost * (1.0 + 8.0 * std::max(0.0, reach - 0.75));
+        return std::chrono::duration<double, std::milli>(Clock::now() - last_gr_end).count() >= factor * last_gr_ms;
     };
     auto work_trigger = [&]() -> bool {
         return alpha > 0.0 &&
