@@ -183,6 +183,21 @@ int wtv_graph_cut_fast(const double* V, size_t nv, const int32_t* d_tets,
                        uint64_t max_rounds, int local_steps, uint8_t* labels_out,
                        wtv_cut_result* out, void* stream);
 
+/* The same cut with every array resident in HBM: d_vertices float64[nv*3],
+ * d_labels uint8[N] and d_labels_out uint8[N] are device buffers (d_labels_out
+ * may equal d_labels: the input labels are read before the output is written).
+ * No host copies are made, so a pipeline that keeps its mesh and labels on the
+ * device (tet labels from wtv_udf_sample_tet_labels, surface extraction after)
+ * skips the vertex upload and both label transfers.  Results and errors are
+ * those of wtv_graph_cut_fast; build_upload_seconds then covers the build
+ * alone.  No reference counterpart: the reference's interface takes host
+ * arrays. */
+int wtv_graph_cut_fast_resident(const double* d_vertices, size_t nv, const int32_t* d_tets,
+                                const int32_t* d_nbr, size_t N, const uint8_t* d_labels,
+                                double fill_T, int global_relabel_period, uint64_t max_rounds,
+                                int local_steps, uint8_t* d_labels_out, wtv_cut_result* out,
+                                void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* Serving: requests are graphs already resident on the device.  Worker     */
 /* slots (one host thread + one stream each) take every waiting request     */

@@ -118,3 +118,12 @@ def test_udf_entry_points_reject_bad_arguments_without_device():
     b1 = (C.c_double * 3)(1, 1, 1)
     assert L.wtv_udf_sample_tet_labels(None, None, 0, 0, None, 0, b0, b1, 0.0, 0.5, None, 0, None, None) == W.WTV_EINVAL
     assert "grid is empty" in L.wtv_last_error().decode()
+
+
+def test_resident_cut_rejects_null_arguments_without_device():
+    L = W.lib()
+    r = W.WtvCutResult()
+    assert L.wtv_graph_cut_fast_resident(None, 0, None, None, 10, None, 10.0, 1024, 0, 8, None, C.byref(r),
+                                         None) == W.WTV_EINVAL
+    assert L.wtv_graph_cut_fast_resident(None, 0, None, None, 0, None, 10.0, 1024, 0, 8, None, C.byref(r),
+                                         None) == W.WTV_OK          # N == 0: nothing to do

@@ -134,6 +134,58 @@ def test_config0_reference_model_graph_cut_fast(orc, torch_cuda):
     assert np.array_equal(lab_out, want) and res.adjusted_flow == out[1]
 
 
+def _resident_cut(torch, V, T, Nb, lab_d, out_d, fill_T=10.0):
+    res = W.WtvCutResult()
+    Vd = torch.from_numpy(np.ascontiguousarray(V)).cuda()
+    W.check(W.lib().wtv_graph_cut_fast_resident(Vd.data_ptr(), V.shape[0], T.data_ptr(), Nb.data_ptr(), T.shape[0],
+                                                lab_d.data_ptr(), fill_T, 1024, 2000000, 8, out_d.data_ptr(),
+                                                C.byref(res), torch.cuda.current_stream().cuda_stream))
+    return res
+
+
+@pytest.mark.parametrize("k", [32, 60])
+def test_graph_cut_fast_resident_equals_host_call(orc, torch_cuda, k):
+    """Every array in HBM gives the host call's labels, flows and counters, also in place."""
+    torch = torch_cuda
+    m, lab, _ = _c0_soft(k=k)
+    T = torch.from_numpy(m.tets).cuda()
+    Nb = torch.from_numpy(m.nbrs).cuda()
+    host_out = np.empty(m.N, np.uint8)
+    hres = W.WtvCutResult()
+    W.check(W.lib().wtv_graph_cut_fast(m.V.ctypes.data, m.V.shape[0], T.data_ptr(), Nb.data_ptr(), m.N,
+                                       lab.ctypes.data, 10.0, 4, 1024, 2000000, 8, host_out.ctypes.data,
+                                       C.byref(hres), torch.cuda.current_stream().cuda_stream))
+    lab_d = torch.from_numpy(lab).cuda()
+    out_d = torch.empty_like(lab_d)
+    rres = _resident_cut(torch, m.V, T, Nb, lab_d, out_d)
+    assert np.array_equal(out_d.cpu().numpy(), host_out)
+    for f in ("adjusted_flow", "raw_flow", "graph_bytes", "full_edges", "free_edges", "fixed_source", "fixed_sink",
+              "terminal_edges", "map_bytes"):
+        assert getattr(rres, f) == getattr(hres, f), f
+    assert np.array_equal(lab_d.cpu().numpy(), lab)          # input untouched
+    inplace = lab_d.clone()
+    _resident_cut(torch, m.V, T, Nb, inplace, inplace)
+    assert np.array_equal(inplace.cpu().numpy(), host_out)
+    if k == 32:
+        r = orc.reduce(m.V, m.tets, m.nbrs, lab, 10.0)
+        sol = orc.pr_solve(r.nbr, r.res, r.term)
+        assert np.array_equal(host_out, orc.scatter(lab, r.free_old, sol.partition))
+
+
+def test_graph_cut_fast_resident_no_free_tets(torch_cuda):
+    torch = torch_cuda
+    m, _, _ = _c0_soft(k=8)
+    lab = np.zeros(m.N, np.uint8)                           # every tet fixed to the source
+    T = torch.from_numpy(m.tets).cuda()
+    Nb = torch.from_numpy(m.nbrs).cuda()
+    lab_d = torch.from_numpy(lab).cuda()
+    out_d = torch.full_like(lab_d, 7)
+    res = _resident_cut(torch, m.V, T, Nb, lab_d, out_d)
+    assert (out_d.cpu().numpy() == 0).all() and res.raw_flow == 0 and res.fixed_source == m.N
+    assert W.lib().wtv_graph_cut_fast_resident(None, 0, None, None, 10, None, 10.0, 1024, 0, 8, None,
+                                               None, None) == W.WTV_EINVAL
+
+
 def test_stress_integer_shell(orc, torch_cuda):
     """Scaled-down adversarial stress: thin shell + tunnels, log-normal caps in [1,2^20], lambda 100."""
     k = 40

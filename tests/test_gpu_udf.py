@@ -84,9 +84,9 @@ def test_labels_on_device_and_graph_cut(orc, torch_cuda):
     L = W.lib()
     oo, vv = np.ascontiguousarray(o), np.ascontiguousarray(v)
     W.check(L.wtv_udf_create(oo.ctypes.data, vv.ctypes.data, oo.shape[0], float(bg), 1 / 128, C.byref(h), None))
+    dV = torch.from_numpy(np.ascontiguousarray(m.V)).cuda()         # vertices resident in HBM
     try:
         bmin, bmax = np.zeros(3), np.ones(3)
-        dV = torch.from_numpy(np.ascontiguousarray(m.V)).cuda()     # vertices resident in HBM
         W.check(L.wtv_udf_sample_tet_labels(h, dV.data_ptr(), m.V.shape[0], 1, dt.data_ptr(),
                                             m.tets.shape[0], bmin.ctypes.data, bmax.ctypes.data, 0.0, thr,
                                             lab_d.data_ptr(), 1, None, torch.cuda.current_stream().cuda_stream))
@@ -97,6 +97,13 @@ def test_labels_on_device_and_graph_cut(orc, torch_cuda):
     a = G.graph_cut_fast(m.V, dt, dn, got, 10.0)
     b = G.graph_cut_fast(m.V, dt, dn, want, 10.0)
     assert np.array_equal(a[0], b[0]) and a[1] == b[1]
+    # the resident pipeline: device labels straight into the resident cut
+    out_d = torch.empty_like(lab_d)
+    res = W.WtvCutResult()
+    W.check(L.wtv_graph_cut_fast_resident(dV.data_ptr(), m.V.shape[0], dt.data_ptr(), dn.data_ptr(),
+                                          m.tets.shape[0], lab_d.data_ptr(), 10.0, 1024, 2000000, 8,
+                                          out_d.data_ptr(), C.byref(res), torch.cuda.current_stream().cuda_stream))
+    assert np.array_equal(out_d.cpu().numpy(), a[0]) and res.adjusted_flow == a[1]
 
 
 def test_udf_error_paths(torch_cuda):

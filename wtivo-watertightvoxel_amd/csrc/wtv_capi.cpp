@@ -105,6 +105,20 @@ int wtv_graph_cut_fast(const double* V, size_t nv, const int32_t* d_tets, const 
     });
 }
 
+int wtv_graph_cut_fast_resident(const double* d_vertices, size_t nv, const int32_t* d_tets, const int32_t* d_nbr,
+                                size_t N, const uint8_t* d_labels, double fill_T, int global_relabel_period,
+                                uint64_t max_rounds, int local_steps, uint8_t* d_labels_out, wtv_cut_result* out,
+                                void* stream) {
+    return guarded([&]() {
+        wtv_cut_result r{};
+        wtv_stats st{};
+        wtv::graph_cut_fast(d_vertices, nv, d_tets, d_nbr, N, d_labels, fill_T, 1, global_relabel_period,
+                            max_rounds, local_steps, d_labels_out, r, st, stream, true);
+        wtv::set_last_stats(st);
+        if (out) *out = r;
+    });
+}
+
 int wtv_build_reduced_graph(const double* V, size_t nv, const int32_t* d_tets, const int32_t* d_nbr, size_t N,
                             const uint8_t* labels, double fill_T, int32_t* d_nbr_out, uint32_t* d_res_out,
                             int32_t* d_term_out, uint32_t* d_free_old, wtv_build_info* info, void* stream) {

@@ -518,9 +518,9 @@ BuildSums read_build_ctl(const BuildCtl* d, hipStream_t s, const char* where) {
 // for N nodes (only the first free_n entries are written); their contents
 // match the reference's host build exactly.  Returns false when there are no
 // free nodes (the caller then returns the input labels).
-bool build_reduced(const double* V, size_t nv, const int32_t* dT, const int32_t* dN, size_t N, const uint8_t* d_lab,
-                   double fill_T, int32_t* gN, uint32_t* gR, int32_t* gT, uint32_t* free_old, BuildInfo& info,
-                   hipStream_t s) {
+bool build_reduced(const double* V, bool v_on_device, size_t nv, const int32_t* dT, const int32_t* dN, size_t N,
+                   const uint8_t* d_lab, double fill_T, int32_t* gN, uint32_t* gR, int32_t* gT, uint32_t* free_old,
+                   BuildInfo& info, hipStream_t s) {
     std::memset(&info, 0, sizeof(info));
     const uint32_t n32 = static_cast<uint32_t>(N);
     const uint32_t tiles = static_cast<uint32_t>((N + kTile - 1) / kTile);
@@ -547,12 +547,16 @@ bool build_reduced(const double* V, size_t nv, const int32_t* dT, const int32_t*
     if (h.free_n == 0) return false;
     if (h.free_n > static_cast<uint64_t>(std::numeric_limits<int32_t>::max()))
         fail(WTV_EINVAL, ">INT32_MAX reduced nodes unsupported");
-    ckg(hipMalloc(&dV.p, std::max<size_t>(nv, 1) * 24u), "malloc vertices");
-    pinned_to_device(V, nv * 24u, dV.p, 0, s);
+    const double* vdev = V;
+    if (!v_on_device) {
+        ckg(hipMalloc(&dV.p, std::max<size_t>(nv, 1) * 24u), "malloc vertices");
+        pinned_to_device(V, nv * 24u, dV.p, 0, s);
+        vdev = dV.as<double>();
+    }
     k_tile_scan<<<1, kB, 0, s>>>(tiles, dTiles.as<uint32_t>());
     k_assign<<<tiles, kB, 0, s>>>(n32, dCls.as<int32_t>(), dTiles.as<uint32_t>(), free_old);
     const int grid = static_cast<int>(std::min<uint64_t>((N + kB - 1) / kB, 8192));
-    k_build<<<grid, kB, 0, s>>>(n32, dV.as<double>(), nv, tets4, nbr4, d_lab, dCls.as<int32_t>(), fill_T,
+    k_build<<<grid, kB, 0, s>>>(n32, vdev, nv, tets4, nbr4, d_lab, dCls.as<int32_t>(), fill_T,
                                 reinterpret_cast<int4*>(gN), reinterpret_cast<uint4*>(gR), gT, ctl);
     ckg(hipGetLastError(), "build");
     h = read_build_ctl(ctl, s, "build D2H");
@@ -570,7 +574,7 @@ bool build_reduced(const double* V, size_t nv, const int32_t* dT, const int32_t*
 
 void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t* dN, size_t N, const uint8_t* lp,
                     double fill_T, int requested_threads, int period, uint64_t max_rounds, int local_steps,
-                    uint8_t* out_labels, wtv_cut_result& R, wtv_stats& st, void* stream_v) {
+                    uint8_t* out_labels, wtv_cut_result& R, wtv_stats& st, void* stream_v, bool resident) {
     using Clock = std::chrono::steady_clock;
     const auto all0 = Clock::now();
     std::memset(&R, 0, sizeof(R));
@@ -598,9 +602,14 @@ void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t
                      std::chrono::duration<double, std::milli>(now - mark).count());
         mark = now;
     };
+    // resident: vertices, labels and output labels are device buffers
     Dev dLab;
-    ckg(hipMalloc(&dLab.p, N), "malloc labels");
-    pinned_to_device(lp, N, dLab.p, 0, s);
+    const uint8_t* d_lab = lp;
+    if (!resident) {
+        ckg(hipMalloc(&dLab.p, N), "malloc labels");
+        pinned_to_device(lp, N, dLab.p, 0, s);
+        d_lab = dLab.as<uint8_t>();
+    }
     phase("labels upload");
 
     // VRAM precheck with the reference's estimate (:200-212) plus this
@@ -608,7 +617,7 @@ void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t
     size_t gpu_free = 0, gpu_total = 0;
     ckg(hipMemGetInfo(&gpu_free, &gpu_total), "hipMemGetInfo");
     const uint64_t need = static_cast<uint64_t>(N) * (36ull + 29ull + 4ull + 4ull + 1ull) +
-                          static_cast<uint64_t>(nv) * 24u + (256ull << 20);
+                          (resident ? 0ull : static_cast<uint64_t>(nv) * 24u) + (256ull << 20);
     if (need > static_cast<uint64_t>(gpu_free)) fail(WTV_ENOMEM, "insufficient VRAM for reduced graph + solver");
 
     phase("vram precheck");
@@ -620,7 +629,7 @@ void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t
     ckg(hipMalloc(&dFreeOld.p, N * 4u), "malloc free_old");
     phase("graph malloc");
     BuildInfo bi{};
-    const bool any = build_reduced(V, nv, dT, dN, N, dLab.as<uint8_t>(), fill_T, gN.as<int32_t>(), gR.as<uint32_t>(),
+    const bool any = build_reduced(V, resident, nv, dT, dN, N, d_lab, fill_T, gN.as<int32_t>(), gR.as<uint32_t>(),
                                    gT.as<int32_t>(), dFreeOld.as<uint32_t>(), bi, s);
     const auto build1 = Clock::now();
     phase("build");
@@ -628,7 +637,12 @@ void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t
     R.fixed_source = bi.fixed_source;
     R.fixed_sink = bi.fixed_sink;
     if (!any) {
-        std::memcpy(out_labels, lp, N);
+        if (resident) {
+            ckg(hipMemcpyAsync(out_labels, lp, N, hipMemcpyDeviceToDevice, s), "labels copy");
+            ckg(hipStreamSynchronize(s), "labels copy");
+        } else {
+            std::memcpy(out_labels, lp, N);
+        }
         R.native_total_seconds = std::chrono::duration<double>(Clock::now() - all0).count();
         return;
     }
@@ -646,13 +660,17 @@ void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t
     phase("solve");
 
     // labels (:357-364): out = labels; out[free_old[q]] = partition[q]
-    ckg(hipMalloc(&dOut.p, N), "malloc out labels");
-    ckg(hipMemcpyAsync(dOut.p, dLab.p, N, hipMemcpyDeviceToDevice, s), "labels copy");
+    uint8_t* d_out = out_labels;
+    if (!resident) {
+        ckg(hipMalloc(&dOut.p, N), "malloc out labels");
+        d_out = dOut.as<uint8_t>();
+    }
+    if (d_out != d_lab) ckg(hipMemcpyAsync(d_out, d_lab, N, hipMemcpyDeviceToDevice, s), "labels copy");
     k_scatter<<<static_cast<int>(std::min<uint64_t>((free_N + kB - 1) / kB, 8192)), kB, 0, s>>>(
-        static_cast<uint32_t>(free_N), dFreeOld.as<uint32_t>(), dPart.as<uint8_t>(), dOut.as<uint8_t>());
+        static_cast<uint32_t>(free_N), dFreeOld.as<uint32_t>(), dPart.as<uint8_t>(), d_out);
     ckg(hipGetLastError(), "scatter");
     phase("scatter");
-    pinned_to_host(dOut.p, N, out_labels, 0, s);
+    if (!resident) pinned_to_host(d_out, N, out_labels, 0, s);
     ckg(hipStreamSynchronize(s), "labels sync");
     phase("labels download");
 
@@ -682,7 +700,7 @@ void build_reduced_graph(const double* V, size_t nv, const int32_t* dT, const in
     Dev dLab;
     ckg(hipMalloc(&dLab.p, N), "malloc labels");
     pinned_to_device(labels, N, dLab.p, 0, s);
-    build_reduced(V, nv, dT, dN, N, dLab.as<uint8_t>(), fill_T, gN, gR, gT, free_old, info, s);
+    build_reduced(V, false, nv, dT, dN, N, dLab.as<uint8_t>(), fill_T, gN, gR, gT, free_old, info, s);
     ckg(hipStreamSynchronize(s), "build");
 }
 

@@ -45,10 +45,12 @@ void build_reduced_graph(const double* V, size_t nv, const int32_t* dT, const in
                          const uint8_t* labels, double fill_T, int32_t* gN, uint32_t* gR, int32_t* gT,
                          uint32_t* free_old, BuildInfo& info, void* stream);
 
-// Full cut on the device (wtv_graph_build.hip).
+// Full cut on the device (wtv_graph_build.hip).  resident = vertices, labels
+// and output labels are device buffers (else host buffers, as the reference).
 void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t* dN, size_t N,
                     const uint8_t* lp, double fill_T, int requested_threads, int period, uint64_t max_rounds,
-                    int local_steps, uint8_t* out_labels, wtv_cut_result& R, wtv_stats& st, void* stream_v);
+                    int local_steps, uint8_t* out_labels, wtv_cut_result& R, wtv_stats& st, void* stream_v,
+                    bool resident = false);
 
 // surface_extraction_topology_cuda on the device (wtv_graph_build.hip).
 void surface_extract(const uint8_t* labels, const double* V, size_t nv, const int32_t* dT, const int32_t* dN,

@@ -14,7 +14,8 @@ LIB_PATH = os.path.join(HERE, "libwtv_gpupr.so")
 WTV_OK, WTV_EINVAL, WTV_EOVERFLOW, WTV_EINVARIANT, WTV_ENOMEM, WTV_ELIMIT, WTV_EDEVICE = range(7)
 
 # every extern "C" symbol declared in include/wtv_gpupr.h
-EXPORTS = ["wtv_params_default", "wtv_pr_solve", "wtv_batch_solve", "wtv_graph_cut_fast", "wtv_last_stats",
+EXPORTS = ["wtv_params_default", "wtv_pr_solve", "wtv_batch_solve", "wtv_graph_cut_fast",
+           "wtv_graph_cut_fast_resident", "wtv_last_stats",
            "wtv_last_error", "wtv_version", "wtv_build_reduced_graph", "wtv_surface_extract",
            "wtv_surface_free", "wtv_request_footprint", "wtv_server_create", "wtv_server_submit",
            "wtv_server_wait", "wtv_server_destroy", "wtv_tet_neighbors", "wtv_topology_upload",
@@ -143,6 +144,10 @@ def lib():
         L.wtv_udf_sample_tet_labels.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p,
                                                 C.c_size_t, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
                                                 C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_void_p]
+        L.wtv_graph_cut_fast_resident.restype = C.c_int
+        L.wtv_graph_cut_fast_resident.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_size_t,
+                                                  C.c_void_p, C.c_double, C.c_int, C.c_uint64, C.c_int, C.c_void_p,
+                                                  C.POINTER(WtvCutResult), C.c_void_p]
         L.wtv_last_stats.restype = None
         L.wtv_last_stats.argtypes = [C.POINTER(WtvStats)]
         L.wtv_last_error.restype = C.c_char_p
