@@ -1361,6 +1361,9 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             const uint64_t rounds_left = max_rounds - st.rounds;
             const uint32_t max_sub = static_cast<uint32_t>(
                 std::min<uint64_t>(4096ull, rounds_left * static_cast<uint64_t>(local_steps)));
+            const auto tail0 = Clock::now();
+            const uint32_t tail_in = active;
+            const uint64_t tail_rel0 = relabels_now;
             kt.begin(KernelTimer::kTail);
             k_discharge_tail<kB><<<1, kTailTPB, 0, s>>>(n, static_cast<uint32_t>(pos & 1u), tail_cap, max_sub, budget, wl[0],
                                                    wl[1], rl, nbr4, rslot, d_res, d_term, excess, height, mark, gid, gn,
@@ -1378,6 +1381,10 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
                                                       static_cast<uint64_t>(local_steps));
             relabels_now = stat_sum(*hctl, kStRelabels);
             active = hctl->wl_cnt[pos & 1u];
+            if (trace)
+                std::fprintf(stderr, "[wtv] tail: active %u -> %u, %u sub-rounds, %llu height increases, %.3f ms\n",
+                             tail_in, active, ts, static_cast<unsigned long long>(relabels_now - tail_rel0),
+                             std::chrono::duration<double, std::milli>(Clock::now() - tail0).count());
             const bool by_period = macro_period > 0 && st.rounds / static_cast<uint64_t>(macro_period) !=
                                                            before / static_cast<uint64_t>(macro_period);
             if (active && (by_period || work_trigger())) do_global_relabel();
