@@ -168,6 +168,28 @@ def main():
     assert cut == raw_flow, f"cut capacity {cut} != flow {raw_flow}"
     part_h = part.cpu().numpy()
 
+    # -------------------------------------------------- maximum flow with conservation
+    res.copy_(res0)
+    term.copy_(term0)
+    fpart = torch.empty_like(part)
+    torch.cuda.synchronize()
+    tf = time.perf_counter()
+    fst, returned = W.pr_solve_flow(N, nbr.data_ptr(), res.data_ptr(), term.data_ptr(), fpart.data_ptr(), 1,
+                                    prm, stream)
+    torch.cuda.synchronize()
+    flow_s = time.perf_counter() - tf
+    assert fst.raw_flow == raw_flow and torch.equal(fpart, part), "flow mode changed the flow or the labels"
+    flow_mode = {"s": round(flow_s, 4), "returned_excess": returned, "raw_flow": fst.raw_flow,
+                 "labels": "equal to the solve's"}
+    if not a.no_cpu:
+        import oracle
+        oracle.build()
+        fc = oracle.flow_certificate(nbr.cpu().numpy(), res0.cpu().numpy().view(np.uint32), term0.cpu().numpy(),
+                                     res.cpu().numpy().view(np.uint32), term.cpu().numpy(), part_h, raw_flow)
+        assert fc == 0, f"flow certificate failed with code {fc}"
+        flow_mode["certificate"] = "maximum flow, conservation at every vertex (oracle flow certificate)"
+    log(f"[bench] flow mode: {flow_s * 1e3:.1f} ms, {returned} units of excess returned to the source")
+
     # -------------------------------------------------- profiled solve
     prof = None
     if not a.no_profile:
@@ -318,6 +340,7 @@ def main():
         "roofline": roofline,
         "kernels_ms": {kk: round(vv[0], 3) for kk, vv in kernels.items()} if kernels else None,
         "cpu_baseline": cpu,
+        "flow_mode": flow_mode,
         "atomics": atomics,
         "graph_cut_fast": cutleg,
         "topology": topology,

@@ -14,6 +14,8 @@
  *                          wtivo_gpupr.graph_cut_fast and
  *                          wtivo_gpupr.graph_cut_gpupr_fast_v630
  *                          (native/gpupr/wtivo_gpupr_bindings.cpp:100-397,581-596)
+ *   wtv_pr_solve_flow   <- cellocut_gpu_push_relabel_fast plus the second phase
+ *                          of push-relabel, which the reference leaves out
  *   wtv_batch_solve     <- no single reference function: G independent calls
  *                          of cellocut_gpu_push_relabel_fast solved together
  *   wtv_surface_extract <- surface_extraction_topology_cuda
@@ -197,6 +199,21 @@ int wtv_graph_cut_fast_resident(const double* d_vertices, size_t nv, const int32
                                 double fill_T, int global_relabel_period, uint64_t max_rounds,
                                 int local_steps, uint8_t* d_labels_out, wtv_cut_result* out,
                                 void* stream);
+
+/*
+ * wtv_pr_solve followed by an excess return (the second phase of
+ * push-relabel, which the reference leaves out: it stops at a maximum
+ * preflow).  On return d_res and d_term describe a maximum flow: every vertex
+ * conserves, the source side's term is the flow the source still sends into
+ * the vertex (>= 0), the sink side's term is as after wtv_pr_solve.  The sink
+ * flow (stats->raw_flow), the partition and the cut are exactly
+ * wtv_pr_solve's.  *returned (may be NULL) receives the excess sent back to
+ * the source.  Uses 25 B per node of extra device memory and about a second
+ * solve's time.  WTV_EOVERFLOW when one vertex's excess exceeds int32.
+ */
+int wtv_pr_solve_flow(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
+                      uint8_t* partition, int partition_on_device, const wtv_params* params,
+                      wtv_stats* stats, int64_t* returned, void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* Serving: requests are graphs already resident on the device.  Worker     */

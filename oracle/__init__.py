@@ -339,3 +339,21 @@ def sample_labels(origins, values, background, voxel_size, V, tets, bmin, bmax, 
     if rc != OK:
         raise OracleError(rc, "sample_labels")
     return labels[:N], vals[:N]
+
+
+def flow_certificate(nbr, res0, term0, res, term, partition, raw_flow) -> int:
+    """0 when (res, term) is a maximum flow -- every vertex conserves -- with its minimal sink set."""
+    L = lib()
+    if not hasattr(L, "_flowcert_ready"):
+        L.wtvo_flow_certificate.restype = C.c_int
+        L.wtvo_flow_certificate.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_int64]
+        L._flowcert_ready = True
+    nbr = np.ascontiguousarray(nbr, dtype=np.int32).reshape(-1, 4)
+    res0 = np.ascontiguousarray(res0, dtype=np.uint32).reshape(-1, 4)
+    res = np.ascontiguousarray(res, dtype=np.uint32).reshape(-1, 4)
+    term0 = np.ascontiguousarray(term0, dtype=np.int32).reshape(-1)
+    term = np.ascontiguousarray(term, dtype=np.int32).reshape(-1)
+    part = np.ascontiguousarray(partition, dtype=np.uint8).reshape(-1)
+    return int(L.wtvo_flow_certificate(term0.shape[0], nbr.ctypes.data, res0.ctypes.data, term0.ctypes.data,
+                                       res.ctypes.data, term.ctypes.data, part.ctypes.data, int(raw_flow)))

@@ -20,6 +20,9 @@
  *                      graph-cut solver, used as the CPU baseline.
  *   wtvo_certificate   max-flow/min-cut certificate (no reference counterpart;
  *                      the reference ships no checker).
+ *   wtvo_flow_certificate  the same for a maximum flow (conservation at every
+ *                      vertex), as left by the excess return; pinned by
+ *                      Boykov-Kolmogorov's results, which are flows.
  *   wtvo_sample_labels native/vdb/wtivo_vdb.cpp:1519-1584 with OpenVDB's
  *                      BoxSampler (trilinear, openvdb/tools/Interpolation.h)
  *                      restated from its published source; OpenVDB is not in
@@ -806,6 +809,68 @@ int wtvo_certificate(uint32_t n, const int32_t* nbr, const uint32_t* res0,
         const int s_side = !partition[v];
         if (s_side) {
             if (term[v] < 0) return 5;
+            if (term0[v] < 0) cut += -(int64_t)term0[v];
+            for (int k = 0; k < 4; ++k) {
+                const int32_t uu = nbr[v * 4u + (size_t)k];
+                if (uu < 0 || !partition[uu]) continue;
+                if (res[v * 4u + (size_t)k]) return 4;
+                cut += res0[v * 4u + (size_t)k];
+            }
+        } else if (term0[v] > 0) {
+            cut += term0[v];
+        }
+    }
+    if (cut != raw_flow) return 7;
+    uint8_t* p2 = (uint8_t*)malloc(N + 1u);
+    if (!p2) return WTVO_ENOMEM;
+    int rc = wtvo_partition(n, nbr, res, term, p2);
+    if (rc == WTVO_OK && memcmp(p2, partition, N) != 0) rc = 8;
+    else if (rc != WTVO_OK) rc = 9;
+    free(p2);
+    return rc;
+}
+
+/* Flow certificate: 0 when (res, term) is a maximum FLOW -- every vertex
+ * conserves -- with sink flow raw_flow and `partition` its minimal sink set.
+ * term0 > 0: term is the source flow still sent into v, in [0, term0];
+ * term0 <= 0: term - term0 is the sink flow out of v (term <= 0).
+ * 1 broken residual pair, 2 a vertex that does not conserve, 3 terminal out
+ * of range or sink total != raw_flow, 4 residual arc from the source side to
+ * the sink side, 7 cut capacity != raw_flow, 8 partition not the minimal sink
+ * set, 9 missing reverse arc. */
+int wtvo_flow_certificate(uint32_t n, const int32_t* nbr, const uint32_t* res0,
+                          const int32_t* term0, const uint32_t* res,
+                          const int32_t* term, const uint8_t* partition,
+                          int64_t raw_flow)
+{
+    const size_t N = n;
+    int64_t sink_total = 0;
+    for (size_t v = 0; v < N; ++v) {
+        int64_t bal = 0;
+        if (term0[v] > 0) {
+            if (term[v] < 0 || term[v] > term0[v]) return 3;
+            bal += term[v];
+        } else {
+            const int64_t used = (int64_t)term[v] - (int64_t)term0[v];
+            if (used < 0 || term[v] > 0) return 3;
+            sink_total += used;
+            bal -= used;
+        }
+        for (int k = 0; k < 4; ++k) {
+            const int32_t uu = nbr[v * 4u + (size_t)k];
+            if (uu < 0) continue;
+            const int rk = reverse_slot(nbr, (uint32_t)uu, (uint32_t)v);
+            if (rk < 0) return 9;
+            const size_t e = v * 4u + (size_t)k, r = (size_t)uu * 4u + (size_t)rk;
+            if ((uint64_t)res[e] + res[r] != (uint64_t)res0[e] + res0[r]) return 1;
+            bal -= (int64_t)res0[e] - (int64_t)res[e];
+        }
+        if (bal != 0) return 2;
+    }
+    if (sink_total != raw_flow) return 3;
+    int64_t cut = 0;
+    for (size_t v = 0; v < N; ++v) {
+        if (!partition[v]) {
             if (term0[v] < 0) cut += -(int64_t)term0[v];
             for (int k = 0; k < 4; ++k) {
                 const int32_t uu = nbr[v * 4u + (size_t)k];

@@ -127,3 +127,11 @@ def test_resident_cut_rejects_null_arguments_without_device():
                                          None) == W.WTV_EINVAL
     assert L.wtv_graph_cut_fast_resident(None, 0, None, None, 0, None, 10.0, 1024, 0, 8, None, C.byref(r),
                                          None) == W.WTV_OK          # N == 0: nothing to do
+
+
+def test_flow_mode_zero_nodes_and_null_pointers_without_device():
+    st, ret = W.pr_solve_flow(0, 0, 0, 0, 0, 0)
+    assert st.raw_flow == 0 and ret == 0
+    with pytest.raises(W.WtvError) as e:
+        W.pr_solve_flow(5, 0, 0, 0, 0, 0)
+    assert e.value.code == W.WTV_EINVAL

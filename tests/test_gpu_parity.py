@@ -186,6 +186,72 @@ def test_graph_cut_fast_resident_no_free_tets(torch_cuda):
                                                None, None) == W.WTV_EINVAL
 
 
+def gpu_solve_flow(torch, nbr, res, term, **kw):
+    """wtv_pr_solve_flow through ctypes: (partition, stats, returned excess, res, term) on the host."""
+    dn = torch.from_numpy(np.ascontiguousarray(nbr, np.int32)).cuda()
+    dr = torch.from_numpy(np.ascontiguousarray(res, np.uint32).view(np.int32)).cuda()
+    dt = torch.from_numpy(np.ascontiguousarray(term, np.int32)).cuda()
+    n = int(dt.numel())
+    part = np.empty(max(n, 1), np.uint8)
+    st, ret = W.pr_solve_flow(n, dn.data_ptr(), dr.data_ptr(), dt.data_ptr(), part.ctypes.data, 0, W.params(**kw),
+                              torch.cuda.current_stream().cuda_stream)
+    return part[:n], st.as_dict(), ret, dr.cpu().numpy().view(np.uint32), dt.cpu().numpy()
+
+
+def preflow_excess(res0, term0, res, term):
+    """Per-vertex excess of a solver state (zero everywhere for a flow)."""
+    src = np.where(term0 > 0, term.astype(np.int64), 0)
+    sink = np.where(term0 <= 0, term.astype(np.int64) - term0, 0)
+    return src - sink - (res0.astype(np.int64) - res.astype(np.int64)).reshape(-1, 4).sum(axis=1)
+
+
+def _check_flow_mode(orc, torch, nbr, res, term):
+    b = orc.bk_solve(nbr, res, term)
+    part, st, ret, r_out, t_out = gpu_solve_flow(torch, nbr, res, term)
+    assert st["raw_flow"] == b.raw_flow and np.array_equal(part, b.partition)
+    assert orc.flow_certificate(nbr, res, term, r_out, t_out, part, st["raw_flow"]) == 0
+    # what came back is the excess the plain solve leaves on the source side
+    p1, st1, r1, t1 = gpu_solve(torch, nbr, res, term)
+    ex = preflow_excess(np.asarray(res, np.uint32).reshape(-1, 4), np.asarray(term, np.int32), r1, t1)
+    assert ret == int(ex.sum()) and (ex >= 0).all()
+    return ret
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_flow_mode_random_graphs(orc, torch_cuda, seed):
+    rng = np.random.default_rng(seed + 40)
+    nbr, res, term = S.random_graph(int(rng.integers(50, 3000)), seed + 40, cap_max=int(5 + 9 * seed))
+    _check_flow_mode(orc, torch_cuda, nbr, res, term)
+
+
+def test_flow_mode_config0_soft_seed(orc, torch_cuda):
+    _, _, g = _c0_soft()
+    assert _check_flow_mode(orc, torch_cuda, g.nbr, g.res, g.term) > 0
+
+
+def test_flow_mode_config4_full_size(orc, torch_cuda):
+    k = 150
+    m = S.kuhn_mesh(k, seed=20260004)
+    shape = S.thin_shell_with_tunnels(20260004, 1.0 / k, tunnels=40)
+    lab = S.labels_from_sdf(S.tet_centroids_sdf(m, shape, device="cuda"), 0.0)
+    g = S.lognormal_int_graph(m, lab, 100.0, seed=20260005, rev_slots=S.KUHN_REV_SLOT)
+    print(f"config 4 flow mode: {m.N} tets", flush=True)
+    _check_flow_mode(orc, torch_cuda, g.nbr, g.res, g.term)
+
+
+def test_flow_mode_empty_and_no_excess(orc, torch_cuda):
+    st, ret = W.pr_solve_flow(0, 0, 0, 0, 0, 0)
+    assert st.raw_flow == 0 and ret == 0
+    # everything reaches the sink; two isolated vertices keep the path below the
+    # source height n (the reference's quirk blocks it at n = 2)
+    nbr = np.array([[1, -1, -1, -1], [0, -1, -1, -1], [-1] * 4, [-1] * 4], np.int32)
+    res = np.array([[5, 0, 0, 0], [5, 0, 0, 0], [0] * 4, [0] * 4], np.uint32)
+    term = np.array([5, -5, 0, 0], np.int32)
+    part, st, ret, r_out, t_out = gpu_solve_flow(torch_cuda, nbr, res, term)
+    assert st["raw_flow"] == 5 and ret == 0
+    assert orc.flow_certificate(nbr, res, term, r_out, t_out, part, 5) == 0
+
+
 def test_stress_integer_shell(orc, torch_cuda):
     """Scaled-down adversarial stress: thin shell + tunnels, log-normal caps in [1,2^20], lambda 100."""
     k = 40

@@ -240,6 +240,69 @@ def test_certificate_catches_planted_mistakes(orc):
     assert orc.certificate(nbr, res, term, r2, b.term, b.partition, b.raw_flow) != 0
 
 
+def scipy_flow_state(nbr, res, term):
+    """scipy's maximum flow in the solver's representation: residuals after the
+    net arc flows, source terminals = flow still sent, sink terminals used."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import maximum_flow
+    n = term.shape[0]
+    s, t = n, n + 1
+    rows, cols, caps = [], [], []
+    for v in range(n):
+        for k in range(4):
+            u = int(nbr[v, k])
+            if u >= 0:
+                rows.append(v); cols.append(u); caps.append(int(res[v, k]))
+        if term[v] > 0:
+            rows.append(s); cols.append(v); caps.append(int(term[v]))
+        if term[v] < 0:
+            rows.append(v); cols.append(t); caps.append(int(-term[v]))
+    M = csr_matrix((np.array(caps, np.int32), (rows, cols)), shape=(n + 2, n + 2))
+    r = maximum_flow(M, s, t)
+    F = r.flow.toarray().astype(np.int64)
+    res_f = res.astype(np.int64).copy()
+    for v in range(n):
+        for k in range(4):
+            u = int(nbr[v, k])
+            if u >= 0:
+                res_f[v, k] -= F[v, u]
+    term_f = term.astype(np.int64).copy()
+    term_f[term > 0] = F[s, np.nonzero(term > 0)[0]]
+    neg = np.nonzero(term < 0)[0]
+    term_f[neg] = term[neg] + F[neg, t]
+    assert (res_f >= 0).all()
+    return int(r.flow_value), res_f.astype(np.uint32), term_f.astype(np.int32)
+
+
+def flow_excess(nbr, res0, term0, res, term):
+    """Per-vertex imbalance of a preflow state (the excess a flow would not have)."""
+    src = np.where(term0 > 0, term.astype(np.int64), 0)
+    sink = np.where(term0 <= 0, term.astype(np.int64) - term0, 0)
+    out = (res0.astype(np.int64) - res.astype(np.int64)).sum(axis=1)
+    return src - sink - out
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_flow_certificate_pinned_by_scipy_flows(orc, seed):
+    n = int(np.random.default_rng(seed + 100).integers(6, 100))
+    nbr, res, term = S.random_graph(n, seed + 100, cap_max=int(3 + seed * 3))
+    val, res_f, term_f = scipy_flow_state(nbr, res, term)
+    b = orc.bk_solve(nbr, res, term)
+    assert b.raw_flow == val
+    assert (flow_excess(nbr, res, term, res_f, term_f) == 0).all()
+    assert orc.flow_certificate(nbr, res, term, res_f, term_f, b.partition, val) == 0
+    # a preflow with excess left on the source side is not a flow
+    ex = flow_excess(nbr, res, term, b.res, b.term)
+    if (ex != 0).any():
+        assert orc.flow_certificate(nbr, res, term, b.res, b.term, b.partition, b.raw_flow) == 2
+    assert orc.flow_certificate(nbr, res, term, res_f, term_f, b.partition, val + 1) != 0
+    pos = np.nonzero(term > 0)[0]
+    if len(pos):
+        t2 = term_f.copy()
+        t2[pos[0]] = term[pos[0]] + 1              # more source flow than the arc holds
+        assert orc.flow_certificate(nbr, res, term, res_f, t2, b.partition, val) == 3
+
+
 def test_soft_graph_c0_bk_equals_pr(orc):
     m = S.kuhn_mesh(16, seed=3)
     shape = S.sphere_with_holes(3)

@@ -105,6 +105,22 @@ int wtv_graph_cut_fast(const double* V, size_t nv, const int32_t* d_tets, const 
     });
 }
 
+int wtv_pr_solve_flow(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,
+                      int partition_on_device, const wtv_params* params, wtv_stats* stats, int64_t* returned,
+                      void* stream) {
+    return guarded([&]() {
+        wtv_params p{};
+        if (params) p = *params;
+        else wtv_params_default(&p);
+        wtv_stats st{};
+        int64_t r = 0;
+        wtv::pr_solve_flow(n, d_nbr, d_res, d_term, partition, partition_on_device != 0, p, st, r, stream);
+        wtv::set_last_stats(st);
+        if (stats) *stats = st;
+        if (returned) *returned = r;
+    });
+}
+
 int wtv_graph_cut_fast_resident(const double* d_vertices, size_t nv, const int32_t* d_tets, const int32_t* d_nbr,
                                 size_t N, const uint8_t* d_labels, double fill_T, int global_relabel_period,
                                 uint64_t max_rounds, int local_steps, uint8_t* d_labels_out, wtv_cut_result* out,

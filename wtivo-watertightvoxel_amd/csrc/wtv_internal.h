@@ -29,6 +29,11 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
               uint8_t* partition, bool partition_on_device, const wtv_params& p,
               wtv_stats& st, void* stream);
 
+// The solve followed by an excess return: residuals and terminals end as a
+// maximum flow that conserves at every vertex (wtv_flow.hip).
+void pr_solve_flow(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,
+                   bool partition_on_device, const wtv_params& p, wtv_stats& st, int64_t& returned, void* stream);
+
 // Solve G independent graphs as one disjoint union (wtv_solver.hip).
 void batch_solve(uint32_t G, const uint64_t* h_off, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
                  uint8_t* partition, bool partition_on_device, int64_t* h_flows, const wtv_params& p,
