@@ -65,8 +65,9 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--batch-graphs", type=int, default=256, help="batched leg: number of graphs (0 = skip)")
     ap.add_argument("--batch-max", type=int, default=4_000_000, help="batched leg: largest graph (tets)")
-    ap.add_argument("--serve-requests", type=int, default=48, help="serving leg: number of requests (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=96, help="serving leg: number of requests (0 = skip)")
     ap.add_argument("--serve-load", type=float, default=0.8, help="offered load vs looped-solve capacity")
+    ap.add_argument("--serve-slots", type=int, default=4, help="serving leg: concurrent batches (worker slots)")
     ap.add_argument("--delaunay-points", type=int, default=400_000,
                     help="Delaunay leg: genus-3 point-cloud size (0 = skip)")
     ap.add_argument("--no-tune", action="store_true",
@@ -887,7 +888,7 @@ def serve_leg(a, torch, np, W, S, dev, stream):
     qps = a.serve_load * len(reqs) / t_alone
     rng = np.random.Generator(np.random.PCG64(a.seed + 4))
     arrivals = np.cumsum(rng.exponential(1.0 / qps, size=len(reqs)))
-    with W.Server(slots=2, max_batch=16, hbm_budget_bytes=200 * 10**9, **prm) as srv:
+    with W.Server(slots=a.serve_slots, max_batch=16, hbm_budget_bytes=200 * 10**9, **prm) as srv:
         t0 = time.perf_counter()
         tickets = []
         for r, at in zip(reqs, arrivals):
@@ -907,7 +908,7 @@ def serve_leg(a, torch, np, W, S, dev, stream):
     log(f"[bench] serve: {len(reqs)} requests at {qps:.1f} qps, wall {wall:.2f}s, p50 {np.percentile(lat, 50):.1f} ms, "
         f"p99 {np.percentile(lat, 99):.1f} ms, mean batch {np.mean(bs):.2f}")
     return {"workload": f"{len(reqs)} Poisson-arriving soft-seed requests, tets log-uniform [100k, 20M], lambda 10, "
-                        "2 slots, 200 GB admission budget",
+                        f"{a.serve_slots} slots, 200 GB admission budget",
             "requests": len(reqs), "tets": int(sum(sizes)), "max_tets": int(max(sizes)),
             "qps_offered": round(qps, 3), "load_vs_alone_capacity": a.serve_load,
             "graphs_per_s": round(len(reqs) / wall, 3), "alone_graphs_per_s": round(len(reqs) / t_alone, 3),

@@ -222,7 +222,7 @@ int wtv_pr_solve_flow(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_te
 /* requires the HBM footprint of the batch to fit the budget.               */
 /* ---------------------------------------------------------------------- */
 typedef struct wtv_server_config {
-    int32_t slots;             /* concurrent batches on the device (default 2)   */
+    int32_t slots;             /* concurrent batches on the device (default 4)   */
     int32_t max_batch;         /* requests per batch (default 64)                */
     uint64_t max_batch_nodes;  /* nodes per batch (default 2^30)                 */
     uint64_t hbm_budget_bytes; /* footprint budget (default 200 GB)              */

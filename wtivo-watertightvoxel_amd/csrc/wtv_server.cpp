@@ -66,7 +66,9 @@ uint64_t request_footprint(uint32_t n) {
 class Server {
   public:
     explicit Server(const wtv_server_config& c) : cfg_(c) {
-        if (cfg_.slots <= 0) cfg_.slots = 2;
+        // four slots: concurrent latency-bound solves overlap on the device; more
+        // than the four hardware queues of a process did not lower latency further
+        if (cfg_.slots <= 0) cfg_.slots = 4;
         if (cfg_.max_batch <= 0) cfg_.max_batch = 64;
         if (cfg_.max_batch_nodes == 0) cfg_.max_batch_nodes = 1ull << 30;
         if (cfg_.max_batch_nodes > 0x7FFFFFFFull) cfg_.max_batch_nodes = 0x7FFFFFFFull;

@@ -211,7 +211,7 @@ def batch_solve(offsets, d_nbr, d_res, d_term, d_or_h_partition, partition_on_de
 class Server:
     """Thin wrapper of the native serving loop (wtv_server_*)."""
 
-    def __init__(self, slots=2, max_batch=64, max_batch_nodes=0, hbm_budget_bytes=0, **solver_kw):
+    def __init__(self, slots=4, max_batch=64, max_batch_nodes=0, hbm_budget_bytes=0, **solver_kw):
         cfg = WtvServerConfig()
         cfg.slots, cfg.max_batch = slots, max_batch
         cfg.max_batch_nodes, cfg.hbm_budget_bytes = max_batch_nodes, hbm_budget_bytes
