@@ -841,6 +841,31 @@ def batch_leg(a, torch, np, W, S, dev, stream):
     assert list(map(int, f_b)) == list(map(int, f_loop)), "batched flows differ from looped solves"
     assert torch.equal(part_loop, part_b), "batched labels differ from looped solves"
     log(f"[bench] batch: looped {t_loop*1e3:.1f} ms, batched {t_b*1e3:.1f} ms, x{t_loop / t_b:.2f}")
+
+    # the same graphs through the serving runtime, all submitted at once: four
+    # slots each take a quarter of the graphs as one union on their own stream
+    part_s = torch.empty(total, dtype=torch.uint8, device=dev)
+
+    def four_slots():
+        res.copy_(res0)
+        term.copy_(term0)
+        torch.cuda.synchronize()
+        with W.Server(slots=4, max_batch=(len(sizes) + 3) // 4, hbm_budget_bytes=200 * 10**9,
+                      global_relabel_period=a.period, **best) as srv:
+            t0 = time.perf_counter()
+            tickets = [srv.submit(sizes[g], nbr.data_ptr() + 16 * int(off[g]), res.data_ptr() + 16 * int(off[g]),
+                                  term.data_ptr() + 4 * int(off[g]), part_s.data_ptr() + int(off[g]))
+                       for g in range(len(sizes))]
+            out = [srv.wait(tk) for tk in tickets]
+            dt = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        return dt, out
+
+    four_slots()
+    t_4, out4 = four_slots()
+    assert [int(x.raw_flow) for x in out4] == [int(x) for x in f_loop], "four-slot flows differ from looped solves"
+    assert all(x.status == 0 for x in out4) and torch.equal(part_s, part_loop), "four-slot labels differ"
+    log(f"[bench] batch: four slots {t_4 * 1e3:.1f} ms, x{t_loop / t_4:.2f}")
     return {"workload": f"{len(sizes)} uneven soft-seed Kuhn graphs, tets log-uniform [50k, {a.batch_max}], "
                         "random shapes, lambda U[1,30]",
             "graphs": len(sizes), "tets": total, "min_tets": min(sizes), "max_tets": max(sizes),
@@ -848,6 +873,9 @@ def batch_leg(a, torch, np, W, S, dev, stream):
             "graphs_per_s_looped": round(len(sizes) / t_loop, 2),
             "graphs_per_s_batched": round(len(sizes) / t_b, 2),
             "speedup": round(t_loop / t_b, 2), "identical_to_looped": True,
+            "four_slots_s": round(t_4, 4), "four_slots_speedup": round(t_loop / t_4, 2),
+            "four_slots": "the serving runtime with every graph submitted at once: 4 slots, a quarter "
+                          "of the graphs per slot as one union, gathered and scattered back; identical results",
             "autotuned": best, "autotune_trials": len(trials), "autotune_seconds": round(t_tune, 2),
             "looped_params": "defaults (alpha 2, local_steps 8, tail_cap 512, grid 1024, wg 256)",
             "batched_work": {"sub_rounds": st.sub_rounds, "global_relabels": st.global_relabels,
