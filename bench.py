@@ -200,19 +200,21 @@ def main():
         prof = pst
     s0 = stats[len(stats) // 2]
     n = N
-    # Byte model of the k_bfs_level launches: 61 B per top-down frontier
-    # expansion; a bottom-up level costs a 4 B height sweep over all n vertices
-    # plus 48 B (nbr 16 + res 16 + 4 neighbour heights 16) per unreached vertex
-    # it examines.
+    # Byte model of the k_bfs_level launches: 49 B per top-down frontier
+    # expansion (entry 4, nbr 16, rslot 1, 4 neighbour heights 16, their 4
+    # residual masks 4, CAS and append 8); a bottom-up level costs a 4 B height
+    # sweep over all n vertices plus 33 B (nbr 16 + mask 1 + 4 neighbour
+    # heights 16) per unreached vertex it examines.
     def bfs_bytes(x):
-        return 61 * x["bfs_pops"] + 4 * n * x["bfs_bu_levels"] + 48 * x["bfs_bu_checks"]
+        return 49 * x["bfs_pops"] + 4 * n * x["bfs_bu_levels"] + 33 * x["bfs_bu_checks"]
 
     # Whole solve: discharge + BFS levels + the level-1/2 sweep of every relabel
     # (term read 4 B per vertex; height write 4 B per vertex in the first, then
     # only for the term >= 0 vertices it examines, each costing 48 B of nbr,
-    # res and neighbour terms + 4 B of height) + 2 B/vertex for the partition.
+    # res and neighbour terms + 4 B of height + 1 B of residual mask) + 2 B/vertex
+    # for the partition.
     bytes_alg = (69 * s0["visits"] + 32 * s0["pushes"] + 4 * s0["relabels"] + bfs_bytes(s0)
-                 + 4 * n * s0["global_relabels"] + 4 * n + 52 * s0["bfs_seed_checks"] + 2 * n)
+                 + 4 * n * s0["global_relabels"] + 4 * n + 53 * s0["bfs_seed_checks"] + 2 * n)
     t_solve_s = s0["t_solve_ms"] / 1e3
     roofline = None
     kernels = {}

@@ -320,12 +320,21 @@ __global__ void k_bfs_begin(Ctl* ctl, uint32_t p) {
 // per iteration (coalesced, one stride of kTPB apart), so the workgroup queue
 // flushes once per kSeedUnroll * kTPB vertices.  Level 2 goes to `fout` with
 // its count in bfs_cnt[2] (zeroed by k_bfs_begin).
+//
+// The sweep also writes nzm[v], bit k set when res[v][k] != 0, for every
+// vertex with term >= 0 -- every vertex the levels can still claim.  The
+// residuals do not change during a BFS, so the levels test an arc into the
+// frontier with one byte from an n-byte array instead of a word from the
+// 16n-byte residual rows: a random top-down probe then hits a line that
+// holds 64 vertices' masks (and stays in cache far better) instead of 4
+// vertices' residual rows, and a bottom-up level reads 1 byte per vertex
+// where it read 16.
 constexpr int kSeedUnroll = 4;
 static_assert(kSeedUnroll * kTPB <= WgQueue<kTPB>::kCap, "queue holds one append per vertex per iteration");
 __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __restrict__ term,
                                                    const int4* __restrict__ nbr4, const uint32_t* __restrict__ res,
-                                                   uint32_t* __restrict__ height, uint32_t* __restrict__ fout,
-                                                   uint32_t first, Ctl* ctl) {
+                                                   uint32_t* __restrict__ height, uint8_t* __restrict__ nzm,
+                                                   uint32_t* __restrict__ fout, uint32_t first, Ctl* ctl) {
     __shared__ WgQueue<kTPB> q;
     q_init(q);
     const uint32_t span = kTPB * kSeedUnroll;
@@ -348,6 +357,7 @@ __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __
                 ++work;
                 const int4 nb = nbr4[v[j]];
                 const uint4 r = reinterpret_cast<const uint4*>(res)[v[j]];
+                nzm[v[j]] = static_cast<uint8_t>((r.x != 0u) | (r.y != 0u) << 1 | (r.z != 0u) << 2 | (r.w != 0u) << 3);
                 int32_t tu[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
@@ -375,11 +385,12 @@ __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __
 }
 
 // Top-down claims of frontier vertex u (valid lanes only): each unreached
-// neighbour v with residual v->u > 0 gets height d+1 by CAS.  The height of v
-// and the reverse residual are loaded together (both addresses are known once
-// nbr[u] and rslot[u] are in) to keep the dependent-latency chain short.
+// neighbour v with residual v->u > 0 (bit rslot of nzm[v]) gets height d+1 by
+// CAS.  The height of v and its mask are loaded together (both addresses are
+// known once nbr[u] and rslot[u] are in) to keep the dependent-latency chain
+// short.
 __device__ __forceinline__ void td_claims(bool valid, uint32_t u, uint32_t d, const int4* __restrict__ nbr4,
-                                          const uint8_t* __restrict__ rslot, const uint32_t* __restrict__ res,
+                                          const uint8_t* __restrict__ rslot, const uint8_t* __restrict__ nzm,
                                           uint32_t* height, int4& nb, bool claim[4]) {
     nb = make_int4(-1, -1, -1, -1);
     uint32_t rs = 0;
@@ -395,7 +406,7 @@ __device__ __forceinline__ void td_claims(bool valid, uint32_t u, uint32_t d, co
         rv[k] = 0u;
         if (v >= 0) {
             hv[k] = height[v];
-            rv[k] = res[static_cast<size_t>(v) * 4u + ((rs >> (2 * k)) & 3u)];
+            rv[k] = (static_cast<uint32_t>(nzm[v]) >> ((rs >> (2 * k)) & 3u)) & 1u;
         }
     }
 #pragma unroll
@@ -414,7 +425,7 @@ template <int TPB>
 __global__ void __launch_bounds__(TPB) k_bfs_level(uint32_t n, uint32_t d, uint32_t bu_threshold,
                                                     const uint32_t* __restrict__ fin, uint32_t* __restrict__ fout,
                                                     const int4* __restrict__ nbr4, const uint8_t* __restrict__ rslot,
-                                                    const uint32_t* __restrict__ res, uint32_t* height, Ctl* ctl) {
+                                                    const uint8_t* __restrict__ nzm, uint32_t* height, Ctl* ctl) {
     __shared__ WgQueue<TPB> q;
     // The first frontier entry of this thread is loaded before the count is
     // known (the buffer holds n entries; entries past the count are ignored),
@@ -441,12 +452,12 @@ __global__ void __launch_bounds__(TPB) k_bfs_level(uint32_t n, uint32_t d, uint3
             if (v < n && height[v] == kSent) {
                 ++work;
                 const int4 nb = nbr4[v];
-                const uint4 r = reinterpret_cast<const uint4*>(res)[v];
+                const uint32_t m = nzm[v];
                 uint32_t hu[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const int u = slot_of(nb, k);
-                    hu[k] = (u >= 0 && slot_of(r, k) != 0u) ? height[u] : kSent;
+                    hu[k] = (u >= 0 && ((m >> k) & 1u)) ? height[u] : kSent;
                 }
 #pragma unroll
                 for (int k = 0; k < 4; ++k) found |= hu[k] == d;
@@ -464,7 +475,7 @@ __global__ void __launch_bounds__(TPB) k_bfs_level(uint32_t n, uint32_t d, uint3
             const uint32_t u = !valid ? 0u : (base == blockIdx.x * TPB ? u0 : fin[i]);
             int4 nb;
             bool claim[4];
-            td_claims(valid, u, d, nbr4, rslot, res, height, nb, claim);
+            td_claims(valid, u, d, nbr4, rslot, nzm, height, nb, claim);
 #pragma unroll
             for (int k = 0; k < 4; ++k) q_push(q, claim[k], static_cast<uint32_t>(slot_of(nb, k)));
             q_flush_if(q, fout, out_cnt, 4 * TPB);
@@ -485,7 +496,7 @@ constexpr double kRelabelCost = 1.0;
 constexpr int kTailTPB = 1024;
 __global__ void __launch_bounds__(kTailTPB) k_bfs_tail(uint32_t d0, uint32_t max_levels, uint32_t* f0, uint32_t* f1,
                                                    const int4* __restrict__ nbr4, const uint8_t* __restrict__ rslot,
-                                                   const uint32_t* __restrict__ res, uint32_t* height, Ctl* ctl) {
+                                                   const uint8_t* __restrict__ nzm, uint32_t* height, Ctl* ctl) {
     __shared__ uint32_t s_cnt;
     uint32_t* f[2] = {f0, f1};
     uint32_t cur = ctl->bfs_cnt[d0 % 3u];
@@ -503,7 +514,7 @@ __global__ void __launch_bounds__(kTailTPB) k_bfs_tail(uint32_t d0, uint32_t max
             pops += valid;
             int4 nb;
             bool claim[4];
-            td_claims(valid, u, d, nbr4, rslot, res, height, nb, claim);
+            td_claims(valid, u, d, nbr4, rslot, nzm, height, nb, claim);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const uint64_t m = __ballot(claim[k]);
@@ -979,7 +990,7 @@ struct GraphExec {
 };
 
 struct Layout {
-    size_t height, excess, mark, rslot, wl0, wl1, rl, ctl, gid, nbrg, gn, goff, gflow, total;
+    size_t height, excess, mark, rslot, nzm, wl0, wl1, rl, ctl, gid, nbrg, gn, goff, gflow, total;
 };
 
 // One allocation carved into 256-byte aligned arrays.  Batched mode adds the
@@ -998,6 +1009,7 @@ Layout layout_for(uint32_t n, uint32_t G) {
     L.wl1 = take(N * 4u);
     L.rl = take(N * 4u);
     L.rslot = take(N);
+    L.nzm = take(N);
     if (G) {
         L.gid = take(N * 4u);
         L.nbrg = take(N * 16u);
@@ -1072,6 +1084,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     uint32_t* wl[2] = {reinterpret_cast<uint32_t*>(base + L.wl0), reinterpret_cast<uint32_t*>(base + L.wl1)};
     uint32_t* rl = reinterpret_cast<uint32_t*>(base + L.rl);
     uint8_t* rslot = reinterpret_cast<uint8_t*>(base + L.rslot);
+    uint8_t* nzm = reinterpret_cast<uint8_t*>(base + L.nzm);
     const int4* nbr4 = reinterpret_cast<const int4*>(d_nbr);
     uint32_t* gid = kB ? reinterpret_cast<uint32_t*>(base + L.gid) : nullptr;
     uint32_t* gn = kB ? reinterpret_cast<uint32_t*>(base + L.gn) : nullptr;
@@ -1147,8 +1160,8 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         uint32_t* bf[2] = {rl, wl[p ^ 1u]};
         k_bfs_begin<<<1, 1, 0, s>>>(ctl, p);
         ck(hipGetLastError(), "bfs begin");
-        k_bfs_seed<<<sweep_grid, tpb, 0, s>>>(n, d_term, nbr4, d_res, height, bf[0], st.global_relabels == 0 ? 1u : 0u,
-                                              ctl);
+        k_bfs_seed<<<sweep_grid, tpb, 0, s>>>(n, d_term, nbr4, d_res, height, nzm, bf[0],
+                                              st.global_relabels == 0 ? 1u : 0u, ctl);
         ck(hipGetLastError(), "bfs seed");
         // Levels run in batches of 8 launches.  Batches are pipelined: batch
         // b+1 is enqueued before the frontier count after batch b is read, and
@@ -1168,7 +1181,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
                 launch_wg(wg, [&](auto t) {
                     constexpr int T = decltype(t)::value;
                     k_bfs_level<T><<<grid, T, 0, s>>>(n, d, bu_threshold, bf[d & 1u], bf[(d + 1u) & 1u], nbr4,
-                                                      rslot, d_res, height, ctl);
+                                                      rslot, nzm, height, ctl);
                 });
                 kt.end();
             }
@@ -1192,7 +1205,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
                 // small frontier: one workgroup runs the levels back to back,
                 // from the level the speculative batch reached (it takes the
                 // frontier only while it holds at most kTailMax vertices)
-                k_bfs_tail<<<1, kTailTPB, 0, s>>>(cur.d, 4096u, bf[0], bf[1], nbr4, rslot, d_res, height, ctl);
+                k_bfs_tail<<<1, kTailTPB, 0, s>>>(cur.d, 4096u, bf[0], bf[1], nbr4, rslot, nzm, height, ctl);
                 ck(hipGetLastError(), "bfs tail");
                 read_ctl("bfs tail");
                 d = hctl->tail_d;
