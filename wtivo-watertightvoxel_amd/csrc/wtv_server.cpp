@@ -58,9 +58,9 @@ struct SyncOnExit {
 }  // namespace
 
 uint64_t request_footprint(uint32_t n) {
-    // graph 36 B + plain-solver workspace 30 B per node + batched extras
+    // graph 36 B + plain-solver workspace 26 B per node + batched extras
     // (graph id 4, union ids 16, gathered nbr/res/term/partition 37) + slack
-    return static_cast<uint64_t>(n) * (36ull + 30ull + 20ull + 37ull) + (1ull << 20);
+    return static_cast<uint64_t>(n) * (36ull + 26ull + 20ull + 37ull) + (1ull << 20);
 }
 
 class Server {

@@ -258,11 +258,11 @@ __global__ void __launch_bounds__(kTPB) k_globalize(uint32_t n, uint32_t G, cons
     }
 }
 
-// Reverse slots, preflow init (gpu_push_relabel_fast.cu:69-83) and mark reset.
+// Reverse slots and preflow init (gpu_push_relabel_fast.cu:69-83).
 __global__ void __launch_bounds__(kTPB) k_prepare(uint32_t n, const int4* __restrict__ nbr4,
                                                   const int32_t* __restrict__ term, uint8_t* __restrict__ rslot,
-                                                  long long* __restrict__ excess, uint32_t* __restrict__ mark,
-                                                  uint32_t* __restrict__ wl0, Ctl* ctl) {
+                                                  long long* __restrict__ excess, uint32_t* __restrict__ wl0,
+                                                  Ctl* ctl) {
     __shared__ WgQueue<kTPB> q;
     q_init(q);
     const uint32_t stride = gridDim.x * kTPB;
@@ -289,7 +289,6 @@ __global__ void __launch_bounds__(kTPB) k_prepare(uint32_t n, const int4* __rest
             rslot[v] = static_cast<uint8_t>(code);
             const int32_t t = term[v];
             excess[v] = t > 0 ? static_cast<long long>(t) : 0LL;
-            mark[v] = 0u;
             cand = t > 0;  // every vertex with excess joins the first candidate list
         }
         q_push(q, cand, v);
@@ -599,19 +598,29 @@ __device__ __forceinline__ void flush_graph_flow(bool valid, uint32_t v, unsigne
 }
 
 // Push step of one worklist vertex v (heights read-only during the phase).
-// first: v claimed its relabel-list slot; cand[k]: neighbour k received its
-// first excess of the phase and claimed a slot.
+// keep: v still holds excess after its pushes; cand[k]: neighbour k received
+// its first excess.  Both come from the excess atomics' returned values, so
+// each vertex with excess at the end of the phase enters the relabel list
+// exactly once, with no membership flag:
+//  - excess never goes negative and only v's own thread lowers excess[v];
+//  - the add that finds a receiver at exactly 0 is unique (adds are positive),
+//    and that receiver is then not in the worklist or has just pushed all of
+//    its excess out, in which case its own subtraction returned exactly what
+//    it pushed and it did not keep itself;
+//  - a receiver whose excess was positive already is kept by its own visit
+//    (it is in the worklist, as is every vertex with excess and a height
+//    below the source height), or lies at or above the source height and
+//    stays inactive.
 template <bool kB>
 __device__ __forceinline__ void push_vertex(bool valid, uint32_t v, uint32_t n, const int4* __restrict__ nbr4,
                                             const uint8_t* __restrict__ rslot, uint32_t* res,
                                             int32_t* __restrict__ term, long long* excess,
-                                            const uint32_t* __restrict__ height, uint32_t* mark,
+                                            const uint32_t* __restrict__ height,
                                             const uint32_t* __restrict__ gid, const uint32_t* __restrict__ gn,
-                                            bool& first, bool cand[4], int4& nb, unsigned long long& visits,
+                                            bool& keep, bool cand[4], int4& nb, unsigned long long& visits,
                                             unsigned long long& pushes, unsigned long long& sinkf,
                                             unsigned long long& sdlt) {
     sdlt = 0ull;
-    first = valid && atom_xchg(&mark[v], 1u) == 0u;
     long long e = valid ? atom_load(&excess[v]) : 0LL;
     const uint32_t h = valid ? height[v] : kSent;
     const bool act = e > 0 && h < (valid ? src_h<kB>(v, n, gid, gn) : 0u);
@@ -653,17 +662,13 @@ __device__ __forceinline__ void push_vertex(bool valid, uint32_t v, uint32_t n, 
             res[static_cast<size_t>(v) * 4u + k] = rk - dlt;
             const uint32_t back = (rs >> (2 * k)) & 3u;
             atom_add(&res[static_cast<size_t>(u) * 4u + back], dlt);
-            atom_add(&excess[u], static_cast<long long>(dlt));
+            cand[k] = atom_add(&excess[u], static_cast<long long>(dlt)) == 0ll;
             e -= dlt;
             pushed += dlt;
             ++pushes;
-            // every receiver joins the relabel list once (the exchange dedups it
-            // against its own entry and other pushers), so the excess add needs
-            // no returned value
-            cand[k] = atom_xchg(&mark[u], 1u) == 0u;
         }
     }
-    if (pushed) atom_add(&excess[v], -pushed);
+    keep = act && (pushed == 0 || atom_add(&excess[v], -pushed) != pushed);
 }
 
 // Relabel step of one relabel-list vertex: height = 1 + min over residual
@@ -673,10 +678,9 @@ template <bool kB>
 __device__ __forceinline__ bool relabel_vertex(uint32_t v, uint32_t n, const int4* __restrict__ nbr4,
                                                const uint32_t* __restrict__ res, const int32_t* __restrict__ term,
                                                const long long* __restrict__ excess, uint32_t* height,
-                                               uint32_t* __restrict__ mark, const uint32_t* __restrict__ gid,
+                                               const uint32_t* __restrict__ gid,
                                                const uint32_t* __restrict__ gn, Ctl* ctl,
                                                unsigned long long& relabels) {
-    mark[v] = 0u;
     const long long e = excess[v];
     uint32_t h = height[v];
     const uint32_t sh = src_h<kB>(v, n, gid, gn);
@@ -715,7 +719,7 @@ __global__ void __launch_bounds__(TPB) k_push(uint32_t n, uint32_t p, const uint
                                                uint32_t* __restrict__ rl, const int4* __restrict__ nbr4,
                                                const uint8_t* __restrict__ rslot, uint32_t* res,
                                                int32_t* __restrict__ term, long long* excess,
-                                               const uint32_t* __restrict__ height, uint32_t* mark,
+                                               const uint32_t* __restrict__ height,
                                                const uint32_t* __restrict__ gid, const uint32_t* __restrict__ gn,
                                                unsigned long long* __restrict__ gflow, Ctl* ctl) {
     __shared__ WgQueue<TPB> q;
@@ -729,14 +733,14 @@ __global__ void __launch_bounds__(TPB) k_push(uint32_t n, uint32_t p, const uint
         const uint32_t i = base + threadIdx.x;
         const bool valid = i < cnt;
         const uint32_t v = valid ? wl[i] : 0u;
-        bool first, cand[4];
+        bool keep, cand[4];
         int4 nb;
         unsigned long long sdlt;
-        push_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, mark, gid, gn, first, cand, nb, visits,
+        push_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, gid, gn, keep, cand, nb, visits,
                         pushes, sinkf, sdlt);
         flush_graph_flow<kB>(valid, v, sdlt, gid, gflow);
-        // every worklist vertex is relabel-checked exactly once this sub-round
-        q_push(q, first, v);
+        // every vertex left with excess is relabel-checked exactly once
+        q_push(q, keep, v);
 #pragma unroll
         for (int k = 0; k < 4; ++k) q_push(q, cand[k], static_cast<uint32_t>(slot_of(nb, k)));
         q_flush_if(q, rl, rl_cnt, 5 * TPB);
@@ -754,7 +758,7 @@ __global__ void __launch_bounds__(TPB) k_relabel(uint32_t n, uint32_t p, const u
                                                   uint32_t* __restrict__ wl_next, const int4* __restrict__ nbr4,
                                                   const uint32_t* __restrict__ res, const int32_t* __restrict__ term,
                                                   const long long* __restrict__ excess, uint32_t* height,
-                                                  uint32_t* __restrict__ mark, const uint32_t* __restrict__ gid,
+                                                  const uint32_t* __restrict__ gid,
                                                   const uint32_t* __restrict__ gn, Ctl* ctl) {
     __shared__ WgQueue<TPB> q;
     q_init(q);
@@ -769,7 +773,7 @@ __global__ void __launch_bounds__(TPB) k_relabel(uint32_t n, uint32_t p, const u
         uint32_t v = 0;
         if (i < cnt) {
             v = rl[i];
-            act = relabel_vertex<kB>(v, n, nbr4, res, term, excess, height, mark, gid, gn, ctl, relabels);
+            act = relabel_vertex<kB>(v, n, nbr4, res, term, excess, height, gid, gn, ctl, relabels);
         }
         q_push(q, act, v);
         q_flush_if(q, wl_next, wl_cnt, TPB);
@@ -803,7 +807,7 @@ __global__ void __launch_bounds__(kTailTPB) k_discharge_tail(uint32_t n, uint32_
                                                          uint32_t* rl, const int4* __restrict__ nbr4,
                                                          const uint8_t* __restrict__ rslot, uint32_t* res,
                                                          int32_t* __restrict__ term, long long* excess,
-                                                         uint32_t* height, uint32_t* mark,
+                                                         uint32_t* height,
                                                          const uint32_t* __restrict__ gid,
                                                          const uint32_t* __restrict__ gn,
                                                          unsigned long long* __restrict__ gflow, Ctl* ctl) {
@@ -830,13 +834,13 @@ __global__ void __launch_bounds__(kTailTPB) k_discharge_tail(uint32_t n, uint32_
             const uint32_t i = base + threadIdx.x;
             const bool valid = i < cur;
             const uint32_t v = valid ? wl[p][i] : 0u;
-            bool first, cand[4];
+            bool keep, cand[4];
             int4 nb;
             unsigned long long sdlt;
-            push_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, mark, gid, gn, first, cand, nb,
+            push_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, gid, gn, keep, cand, nb,
                             visits, pushes, sinkf, sdlt);
             flush_graph_flow<kB>(valid, v, sdlt, gid, gflow);
-            lds_append(first, v, rl, &s_cnt);
+            lds_append(keep, v, rl, &s_cnt);
 #pragma unroll
             for (int k = 0; k < 4; ++k) lds_append(cand[k], static_cast<uint32_t>(slot_of(nb, k)), rl, &s_cnt);
         }
@@ -853,7 +857,7 @@ __global__ void __launch_bounds__(kTailTPB) k_discharge_tail(uint32_t n, uint32_
             uint32_t v = 0;
             if (i < rcnt) {
                 v = rl[i];
-                act = relabel_vertex<kB>(v, n, nbr4, res, term, excess, height, mark, gid, gn, ctl, rel);
+                act = relabel_vertex<kB>(v, n, nbr4, res, term, excess, height, gid, gn, ctl, rel);
             }
             lds_append(act, v, wl[p ^ 1u], &s_cnt);
         }
@@ -990,7 +994,7 @@ struct GraphExec {
 };
 
 struct Layout {
-    size_t height, excess, mark, rslot, nzm, wl0, wl1, rl, ctl, gid, nbrg, gn, goff, gflow, total;
+    size_t height, excess, rslot, nzm, wl0, wl1, rl, ctl, gid, nbrg, gn, goff, gflow, total;
 };
 
 // One allocation carved into 256-byte aligned arrays.  Batched mode adds the
@@ -1004,7 +1008,6 @@ Layout layout_for(uint32_t n, uint32_t G) {
     L.ctl = take(sizeof(Ctl));
     L.excess = take(N * 8u);
     L.height = take(N * 4u);
-    L.mark = take(N * 4u);
     L.wl0 = take(N * 4u);
     L.wl1 = take(N * 4u);
     L.rl = take(N * 4u);
@@ -1080,7 +1083,6 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     Ctl* ctl = reinterpret_cast<Ctl*>(base + L.ctl);
     long long* excess = reinterpret_cast<long long*>(base + L.excess);
     uint32_t* height = reinterpret_cast<uint32_t*>(base + L.height);
-    uint32_t* mark = reinterpret_cast<uint32_t*>(base + L.mark);
     uint32_t* wl[2] = {reinterpret_cast<uint32_t*>(base + L.wl0), reinterpret_cast<uint32_t*>(base + L.wl1)};
     uint32_t* rl = reinterpret_cast<uint32_t*>(base + L.rl);
     uint8_t* rslot = reinterpret_cast<uint8_t*>(base + L.rslot);
@@ -1143,7 +1145,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         ck(hipGetLastError(), "globalize");
         nbr4 = nbrg;
     }
-    k_prepare<<<sweep_grid, tpb, 0, s>>>(n, nbr4, d_term, rslot, excess, mark, wl[0], ctl);
+    k_prepare<<<sweep_grid, tpb, 0, s>>>(n, nbr4, d_term, rslot, excess, wl[0], ctl);
     ck(hipGetLastError(), "prepare");
     check_err("prepare");
 
@@ -1271,14 +1273,14 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
                 launch_wg(wg, [&](auto t) {
                     constexpr int T = decltype(t)::value;
                     k_push<kB, T><<<grid, T, 0, s>>>(n, p, wl[p], rl, nbr4, rslot, d_res, d_term, excess, height,
-                                                     mark, gid, gn, gflow, ctl);
+                                                     gid, gn, gflow, ctl);
                 });
                 kt.end();
                 kt.begin(KernelTimer::kRelabel);
                 launch_wg(wg, [&](auto t) {
                     constexpr int T = decltype(t)::value;
                     k_relabel<kB, T><<<grid, T, 0, s>>>(n, p, rl, wl[p ^ 1u], nbr4, d_res, d_term, excess, height,
-                                                        mark, gid, gn, ctl);
+                                                        gid, gn, ctl);
                 });
                 kt.end();
             }
@@ -1379,7 +1381,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             const uint64_t tail_rel0 = relabels_now;
             kt.begin(KernelTimer::kTail);
             k_discharge_tail<kB><<<1, kTailTPB, 0, s>>>(n, static_cast<uint32_t>(pos & 1u), tail_cap, max_sub, budget, wl[0],
-                                                   wl[1], rl, nbr4, rslot, d_res, d_term, excess, height, mark, gid, gn,
+                                                   wl[1], rl, nbr4, rslot, d_res, d_term, excess, height, gid, gn,
                                                    gflow, ctl);
             kt.end();
             ck(hipGetLastError(), "discharge tail");
