@@ -78,6 +78,8 @@ def main():
                     help="skip the topology leg (host upload and device neighbour construction at --k)")
     ap.add_argument("--no-udf-leg", action="store_true",
                     help="skip the sparse-grid tet label leg (SparseUDF.sample_tet_labels at --k)")
+    ap.add_argument("--no-flow-leg", action="store_true",
+                    help="skip the maximum-flow-with-conservation leg (wtv_pr_solve_flow on the same graph)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy three times (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     ap.add_argument("--atomic-probe", action="store_true",
@@ -170,26 +172,9 @@ def main():
     part_h = part.cpu().numpy()
 
     # -------------------------------------------------- maximum flow with conservation
-    res.copy_(res0)
-    term.copy_(term0)
-    fpart = torch.empty_like(part)
-    torch.cuda.synchronize()
-    tf = time.perf_counter()
-    fst, returned = W.pr_solve_flow(N, nbr.data_ptr(), res.data_ptr(), term.data_ptr(), fpart.data_ptr(), 1,
-                                    prm, stream)
-    torch.cuda.synchronize()
-    flow_s = time.perf_counter() - tf
-    assert fst.raw_flow == raw_flow and torch.equal(fpart, part), "flow mode changed the flow or the labels"
-    flow_mode = {"s": round(flow_s, 4), "returned_excess": returned, "raw_flow": fst.raw_flow,
-                 "labels": "equal to the solve's"}
-    if not a.no_cpu:
-        import oracle
-        oracle.build()
-        fc = oracle.flow_certificate(nbr.cpu().numpy(), res0.cpu().numpy().view(np.uint32), term0.cpu().numpy(),
-                                     res.cpu().numpy().view(np.uint32), term.cpu().numpy(), part_h, raw_flow)
-        assert fc == 0, f"flow certificate failed with code {fc}"
-        flow_mode["certificate"] = "maximum flow, conservation at every vertex (oracle flow certificate)"
-    log(f"[bench] flow mode: {flow_s * 1e3:.1f} ms, {returned} units of excess returned to the source")
+    flow_mode = None
+    if not a.no_flow_leg:
+        flow_mode = flow_leg(a, torch, np, W, N, nbr, res, term, res0, term0, part, part_h, prm, stream, raw_flow)
 
     # -------------------------------------------------- profiled solve
     prof = None
@@ -653,6 +638,34 @@ def topology_leg(a, torch, np, W, S, dev, stream):
         f"({nbytes / up_f / 1e9:.1f} GB/s), C {up_c * 1e3:.1f} ms, reference loop "
         f"{out_d.get('reference_upload_s', float('nan')) * 1e3:.1f} ms, one-stream pinned copy {pcie:.1f} GB/s")
     return out_d
+
+
+def flow_leg(a, torch, np, W, N, nbr, res, term, res0, term0, part, part_h, prm, stream, raw_flow):
+    """wtv_pr_solve_flow on the headline graph: the solve plus the excess
+    return, timed wall-clock.  The flow and labels must equal the solve's, and
+    with the CPU leg on, the oracle's flow certificate (maximum flow,
+    conservation at every vertex) must hold for the returned arrays."""
+    res.copy_(res0)
+    term.copy_(term0)
+    fpart = torch.empty_like(part)
+    torch.cuda.synchronize()
+    tf = time.perf_counter()
+    fst, returned = W.pr_solve_flow(N, nbr.data_ptr(), res.data_ptr(), term.data_ptr(), fpart.data_ptr(), 1,
+                                    prm, stream)
+    torch.cuda.synchronize()
+    flow_s = time.perf_counter() - tf
+    assert fst.raw_flow == raw_flow and torch.equal(fpart, part), "flow mode changed the flow or the labels"
+    out = {"s": round(flow_s, 4), "returned_excess": returned, "raw_flow": fst.raw_flow,
+           "labels": "equal to the solve's"}
+    if not a.no_cpu:
+        import oracle
+        oracle.build()
+        fc = oracle.flow_certificate(nbr.cpu().numpy(), res0.cpu().numpy().view(np.uint32), term0.cpu().numpy(),
+                                     res.cpu().numpy().view(np.uint32), term.cpu().numpy(), part_h, raw_flow)
+        assert fc == 0, f"flow certificate failed with code {fc}"
+        out["certificate"] = "maximum flow, conservation at every vertex (oracle flow certificate)"
+    log(f"[bench] flow mode: {flow_s * 1e3:.1f} ms, {returned} units of excess returned to the source")
+    return out
 
 
 def udf_leg(a, torch, np, W, S, dev, stream):
