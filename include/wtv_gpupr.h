@@ -84,6 +84,11 @@ typedef struct wtv_stats {
     uint64_t tail_sub_rounds; /* sub-rounds run by the one-workgroup kernel  */
     uint64_t bfs_seed_checks; /* term >= 0 vertices examined by the sweep that
                                  expands BFS levels 1 and 2, all relabels    */
+    double t_event_overhead_us; /* profile: what the event pair around one launch
+                                   adds to it (dispatch and completion latency),
+                                   measured on the solver's stream with an empty
+                                   kernel of the push launch shape; subtract it
+                                   from t_push_ms / n_push for the kernel time  */
 } wtv_stats;
 
 /* Solver knobs.  global_relabel_period, max_rounds and local_steps keep the

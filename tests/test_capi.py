@@ -34,10 +34,29 @@ def test_params_defaults_match_reference():
     assert W.lib().wtv_version().startswith(b"wtv_gpupr")
 
 
-def test_struct_sizes_match_c_layout():
-    # 8 doubles + 17 uint64/int64 in wtv_stats
-    assert C.sizeof(W.WtvStats) == 26 * 8
-    assert C.sizeof(W.WtvCutResult) == 17 * 8
+def test_struct_layouts_match_c_header(tmp_path):
+    """Every ctypes mirror has the size and field offsets that gcc gives the
+    structs of include/wtv_gpupr.h (a field missing from either side fails to
+    compile or to match)."""
+    import subprocess
+    pairs = [(W.WtvStats, "wtv_stats"), (W.WtvParams, "wtv_params"), (W.WtvCutResult, "wtv_cut_result"),
+             (W.WtvServerConfig, "wtv_server_config"), (W.WtvRequest, "wtv_request"), (W.WtvResult, "wtv_result"),
+             (W.WtvBuildInfo, "wtv_build_info")]
+    lines, want = [], []
+    for cls, cname in pairs:
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        want.append(C.sizeof(cls))
+        for f in cls._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {f[0]}));')
+            want.append(getattr(cls, f[0]).offset)
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stddef.h>\n#include <stdio.h>\n#include \"wtv_gpupr.h\"\nint main(void) {\n"
+                   + "\n".join(lines) + "\nreturn 0;\n}\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-std=c99", "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == want
 
 
 def test_zero_nodes_and_null_pointers_without_device():

@@ -181,6 +181,7 @@ py::dict stats_dict(const wtv_stats& s) {
     d["n_tail"] = s.n_tail;
     d["tail_sub_rounds"] = s.tail_sub_rounds;
     d["bfs_seed_checks"] = s.bfs_seed_checks;
+    d["t_event_overhead_us"] = s.t_event_overhead_us;
     return d;
 }
 

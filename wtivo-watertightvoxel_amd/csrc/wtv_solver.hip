@@ -973,6 +973,42 @@ struct KernelTimer {
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+__global__ void k_noop() {}
+
+// Cost of the event pair the profile mode puts around every launch, on
+// stream s: an empty kernel of the push launch shape timed kCal times with a
+// pair of its own, against kCal back-to-back launches in one pair.  The pair
+// adds the dispatch and completion latency that back-to-back launches hide,
+// which a kernel trace does not count as the kernel's duration.
+double event_pair_overhead_us(int grid, int wg, hipStream_t s) {
+    constexpr int kCal = 32;
+    std::vector<hipEvent_t> ev(2 * kCal + 2, nullptr);
+    struct Guard {
+        std::vector<hipEvent_t>& e;
+        ~Guard() { for (auto x : e) if (x) (void)hipEventDestroy(x); }
+    } guard{ev};
+    for (auto& e : ev) ck(hipEventCreate(&e), "event create");
+    for (int i = 0; i < kCal; ++i) {
+        ck(hipEventRecord(ev[2 * i], s), "event");
+        k_noop<<<grid, wg, 0, s>>>();
+        ck(hipEventRecord(ev[2 * i + 1], s), "event");
+    }
+    ck(hipEventRecord(ev[2 * kCal], s), "event");
+    for (int i = 0; i < kCal; ++i) k_noop<<<grid, wg, 0, s>>>();
+    ck(hipEventRecord(ev[2 * kCal + 1], s), "event");
+    ck(hipGetLastError(), "noop launch");
+    ck(hipEventSynchronize(ev[2 * kCal + 1]), "event sync");
+    double pairs = 0.0;
+    for (int i = 0; i < kCal; ++i) {
+        float t = 0.f;
+        ck(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]), "event time");
+        pairs += t;
+    }
+    float batch = 0.f;
+    ck(hipEventElapsedTime(&batch, ev[2 * kCal], ev[2 * kCal + 1]), "event time");
+    return std::max(0.0, (pairs - static_cast<double>(batch)) / kCal * 1e3);
+}
+
 // One non-blocking stream per host thread and device for the solver's own
 // work (default priority).  Graph capture needs a stream other than the null
 // stream; the caller's stream is ordered before and after through events.
@@ -1519,6 +1555,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     st.n_level = kt.count[KernelTimer::kLevel];
     st.t_tail_ms = kt.ms[KernelTimer::kTail];
     st.n_tail = kt.count[KernelTimer::kTail];
+    st.t_event_overhead_us = kt.on ? event_pair_overhead_us(grid, wg, s) : 0.0;
 }
 
 void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,

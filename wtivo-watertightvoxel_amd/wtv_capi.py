@@ -34,7 +34,7 @@ class WtvStats(C.Structure):
         ("n_push", C.c_uint64), ("n_relabel", C.c_uint64), ("n_level", C.c_uint64),
         ("bfs_bu_levels", C.c_uint64), ("bfs_bu_checks", C.c_uint64),
         ("t_tail_ms", C.c_double), ("n_tail", C.c_uint64), ("tail_sub_rounds", C.c_uint64),
-        ("bfs_seed_checks", C.c_uint64),
+        ("bfs_seed_checks", C.c_uint64), ("t_event_overhead_us", C.c_double),
     ]
 
     def as_dict(self):
