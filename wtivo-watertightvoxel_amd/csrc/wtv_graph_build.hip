@@ -616,7 +616,7 @@ void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t
     // solver's workspace and the build's own buffers, for the worst case n = N
     size_t gpu_free = 0, gpu_total = 0;
     ckg(hipMemGetInfo(&gpu_free, &gpu_total), "hipMemGetInfo");
-    const uint64_t need = static_cast<uint64_t>(N) * (36ull + 29ull + 4ull + 4ull + 1ull) +
+    const uint64_t need = static_cast<uint64_t>(N) * (36ull + 26ull + 4ull + 4ull + 1ull) +
                           (resident ? 0ull : static_cast<uint64_t>(nv) * 24u) + (256ull << 20);
     if (need > static_cast<uint64_t>(gpu_free)) fail(WTV_ENOMEM, "insufficient VRAM for reduced graph + solver");
 
