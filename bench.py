@@ -23,6 +23,13 @@ Further legs, each reported in its own object of the line:
   * graph_cut_fast: the drop-in call (build + solve + scatter) on the same mesh
     with the reference capacity model, certificate-checked, with the CPU
     restatement of the build timed and compared array for array;
+  * flow_mode: the maximum flow with conservation (excess returned to the
+    source), checked by the oracle's flow certificate;
+  * topology: the chunked topology upload and the device neighbour build;
+  * udf_labels: tet labels sampled from a sparse distance grid, checked
+    against the oracle;
+  * surface: surface extraction of the label-0 region, checked against the
+    oracle, beside the reference's chunked topology download;
   * delaunay: the Qhull mesh of a 400k-point genus-3 cloud, checked against BK;
   * batched: 256 uneven graphs, looped vs one autotuned batched solve;
   * serving: Poisson-arriving requests into the native server, p50/p99;
@@ -78,6 +85,8 @@ def main():
                     help="skip the topology leg (host upload and device neighbour construction at --k)")
     ap.add_argument("--no-udf-leg", action="store_true",
                     help="skip the sparse-grid tet label leg (SparseUDF.sample_tet_labels at --k)")
+    ap.add_argument("--no-surface-leg", action="store_true",
+                    help="skip the surface extraction leg (surface_extraction_topology_cuda at --k)")
     ap.add_argument("--no-flow-leg", action="store_true",
                     help="skip the maximum-flow-with-conservation leg (wtv_pr_solve_flow on the same graph)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -288,6 +297,10 @@ def main():
     if not a.no_udf_leg:
         udf = udf_leg(a, torch, np, W, S, dev, stream)
 
+    surface = None
+    if not a.no_surface_leg:
+        surface = surface_leg(a, torch, np, W, S, dev, stream)
+
     delaunay = None
     if a.delaunay_points > 0:
         delaunay = delaunay_leg(a, torch, np, W, S, dev, stream)
@@ -342,6 +355,7 @@ def main():
         "graph_cut_fast": cutleg,
         "topology": topology,
         "udf_labels": udf,
+        "surface": surface,
         "delaunay": delaunay,
         "batched": batched,
         "serving": serving,
@@ -674,6 +688,72 @@ def flow_leg(a, torch, np, W, N, nbr, res, term, res0, term0, part, part_h, prm,
         assert fc == 0, f"flow certificate failed with code {fc}"
         out["certificate"] = "maximum flow, conservation at every vertex (oracle flow certificate)"
     log(f"[bench] flow mode: {flow_s * 1e3:.1f} ms, {returned} units of excess returned to the source")
+    return out
+
+
+def surface_leg(a, torch, np, W, S, dev, stream):
+    """surface_extraction_topology_cuda through the drop-in module at the Kuhn k^3
+    size: the oriented boundary of a genus-3 solid (its tets labelled 0), with
+    host labels and vertices and the topology in HBM, as the reference's caller
+    runs it after the cut (wtivo.py:857).  Timed per call.  The oracle's
+    restatement runs on the same arrays on one core and must return identical
+    vertices and faces.  Beside it, the reference's own first pass pulls the
+    neighbour tensor to the host in 262,144-row chunks, one synchronous copy each
+    (wtivo_gpupr_bindings.cpp:431-440); that transfer is restated with torch
+    copies and timed (a lower bound for the reference: its second pass copies
+    the tets and neighbours of every chunk holding a face again)."""
+    import wtivo_gpupr as G
+    k = a.k
+    t = time.time()
+    V, T, Nb = S.kuhn_mesh_t(k, seed=a.seed, device=dev)
+    lab = (S.tet_sdf_t(V, T, S.genus3(a.seed)) >= 0).to(torch.uint8)
+    Vh = np.ascontiguousarray(V.cpu().numpy())
+    labh = np.ascontiguousarray(lab.cpu().numpy())
+    del V, lab
+    N = int(T.shape[0])
+    gen_s = time.time() - t
+
+    def call():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = G.surface_extraction_topology_cuda(labh, Vh, T, Nb, 16)
+        return time.perf_counter() - t0, out
+
+    call()
+    runs = [call() for _ in range(3)]
+    call_s = min(r[0] for r in runs)
+    OV, OF = runs[-1][1]
+    # the reference's first pass: the neighbour rows, chunk by chunk, to pageable memory
+    CH = 262144
+    hn = torch.empty((CH, 4), dtype=torch.int32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i0 in range(0, N, CH):
+        c = min(CH, N - i0)
+        hn[:c].copy_(Nb[i0:i0 + c])
+    ref_d2h_s = time.perf_counter() - t0
+    out = {"workload": f"surface_extraction_topology_cuda on Kuhn {k}^3 ({N} tets): boundary of the genus-3 "
+                       f"solid's label-0 tets; host labels and vertices, topology in HBM",
+           "tets": N, "label0_tets": int((labh == 0).sum()), "faces": int(OF.shape[0]),
+           "vertices": int(OV.shape[0]), "call_s": round(call_s, 4), "tets_per_s": round(N / call_s, 1),
+           "reference_first_pass_d2h_s": round(ref_d2h_s, 4),
+           "reference_first_pass_d2h": "the neighbour tensor copied to the host in 262,144-row chunks, one "
+                                       "synchronous copy each, as the reference's first pass does",
+           "gen_seconds": round(gen_s, 1)}
+    if not a.no_cpu:
+        import oracle
+        oracle.build()
+        Th, Nh = T.cpu().numpy(), Nb.cpu().numpy()
+        t0 = time.perf_counter()
+        want_v, want_f = oracle.surface(labh, Vh, Th, Nh)
+        cpu_s = time.perf_counter() - t0
+        assert np.array_equal(OF, want_f) and np.array_equal(OV, want_v), "surface leg: output differs from the oracle's"
+        out["cpu_baseline"] = {"value": round(cpu_s, 4), "unit": "s", "cores": 1, "kind": "port",
+                               "sample": "full workload: the oracle's restatement (oracle/wtv_oracle.c) on host "
+                                         "copies of the same arrays, topology already on the host; vertices and "
+                                         "faces identical to the device's"}
+    log(f"[bench] surface: {N} tets, {OF.shape[0]} faces, {OV.shape[0]} vertices, call {call_s * 1e3:.1f} ms; "
+        f"reference first-pass D2H {ref_d2h_s * 1e3:.1f} ms")
     return out
 
 

@@ -776,6 +776,11 @@ def test_profile_mode_counts_kernels(torch_cuda):
     assert st["n_push"] == st["n_relabel"] and (st["n_tail"] > 0) == (st["tail_sub_rounds"] > 0)
     _, st2, _, _ = gpu_solve(torch_cuda, g.nbr, g.res, g.term, profile=1, tail_cap=-1)
     assert st2["n_tail"] == 0 and st2["n_push"] >= st2["sub_rounds"] and st2["t_push_ms"] > 0
+    # the event-pair cost is measured in profile mode only, and is a part of
+    # the event-timed launches it is subtracted from
+    assert 0.0 <= st2["t_event_overhead_us"] < 1e3 * st2["t_push_ms"] / st2["n_push"]
+    _, st3, _, _ = gpu_solve(torch_cuda, g.nbr, g.res, g.term)
+    assert st3["t_event_overhead_us"] == 0.0 and st3["n_push"] == 0
 
 
 @pytest.mark.parametrize("tail_cap", [-1, 16, 2048, 1 << 20])
