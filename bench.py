@@ -194,13 +194,13 @@ def main():
         prof = pst
     s0 = stats[len(stats) // 2]
     n = N
-    # Byte model of the k_bfs_level launches: 49 B per top-down frontier
-    # expansion (entry 4, nbr 16, rslot 1, 4 neighbour heights 16, their 4
-    # residual masks 4, CAS and append 8); a bottom-up level costs a 4 B height
-    # sweep over all n vertices plus 33 B (nbr 16 + mask 1 + 4 neighbour
-    # heights 16) per unreached vertex it examines.
+    # Byte model of the k_bfs_level launches: 34 B per top-down frontier
+    # expansion (entry 4, nbr 16, rslot 1, the 4 neighbours' mask bytes 4, and
+    # the claim that reached the vertex: CAS 4, mask write 1, append 4); a
+    # bottom-up level costs a 4 B height sweep over all n vertices plus 33 B
+    # (nbr 16 + mask 1 + 4 neighbour heights 16) per unreached vertex it examines.
     def bfs_bytes(x):
-        return 49 * x["bfs_pops"] + 4 * n * x["bfs_bu_levels"] + 33 * x["bfs_bu_checks"]
+        return 34 * x["bfs_pops"] + 4 * n * x["bfs_bu_levels"] + 33 * x["bfs_bu_checks"]
 
     # Whole solve: discharge + BFS levels + the level-1/2 sweep of every relabel
     # (term read 4 B per vertex; height write 4 B per vertex in the first, then

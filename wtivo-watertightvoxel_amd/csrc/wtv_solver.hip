@@ -54,6 +54,8 @@ namespace wtv {
 namespace {
 
 constexpr uint32_t kSent = 0xFFFFFFFFu;
+// bit 4 of a vertex's residual-mask byte during a BFS: not reached yet
+constexpr uint32_t kUnreached = 16u;
 constexpr int kTPB = 256;
 
 // Device control block: worklist counters, flow, work counters, error word.
@@ -327,7 +329,9 @@ __global__ void k_bfs_begin(Ctl* ctl, uint32_t p) {
 // 16n-byte residual rows: a random top-down probe then hits a line that
 // holds 64 vertices' masks (and stays in cache far better) instead of 4
 // vertices' residual rows, and a bottom-up level reads 1 byte per vertex
-// where it read 16.
+// where it read 16.  Bit 4 (kUnreached) marks a vertex the BFS has not
+// reached: a top-down probe reads only the neighbour's byte, and touches its
+// height only for the claim itself.
 constexpr int kSeedUnroll = 4;
 static_assert(kSeedUnroll * kTPB <= WgQueue<kTPB>::kCap, "queue holds one append per vertex per iteration");
 __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __restrict__ term,
@@ -356,7 +360,7 @@ __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __
                 ++work;
                 const int4 nb = nbr4[v[j]];
                 const uint4 r = reinterpret_cast<const uint4*>(res)[v[j]];
-                nzm[v[j]] = static_cast<uint8_t>((r.x != 0u) | (r.y != 0u) << 1 | (r.z != 0u) << 2 | (r.w != 0u) << 3);
+                const uint32_t mask = (r.x != 0u) | (r.y != 0u) << 1 | (r.z != 0u) << 2 | (r.w != 0u) << 3;
                 int32_t tu[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
@@ -365,6 +369,7 @@ __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __
                 }
 #pragma unroll
                 for (int k = 0; k < 4; ++k) found[j] |= tu[k] < 0;
+                nzm[v[j]] = static_cast<uint8_t>(mask | (found[j] ? 0u : kUnreached));
             }
         }
 #pragma unroll
@@ -373,8 +378,10 @@ __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __
             // valid labels give it h <= 1, and term < 0 never returns once
             // the sink arc is saturated, so later relabels skip those writes
             if (static_cast<uint64_t>(base) + static_cast<uint64_t>(j) * kTPB + threadIdx.x < n &&
-                (t[j] >= 0 || first))
+                (t[j] >= 0 || first)) {
                 height[v[j]] = t[j] < 0 ? 1u : (found[j] ? 2u : kSent);
+                if (t[j] < 0) nzm[v[j]] = 0u;  // height 1 for good: never unreached again
+            }
             q_push(q, found[j], v[j]);
         }
         q_flush_if(q, fout, &ctl->bfs_cnt[2], kSeedUnroll * kTPB);
@@ -389,7 +396,7 @@ __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __
 // known once nbr[u] and rslot[u] are in) to keep the dependent-latency chain
 // short.
 __device__ __forceinline__ void td_claims(bool valid, uint32_t u, uint32_t d, const int4* __restrict__ nbr4,
-                                          const uint8_t* __restrict__ rslot, const uint8_t* __restrict__ nzm,
+                                          const uint8_t* __restrict__ rslot, uint8_t* nzm,
                                           uint32_t* height, int4& nb, bool claim[4]) {
     nb = make_int4(-1, -1, -1, -1);
     uint32_t rs = 0;
@@ -397,21 +404,20 @@ __device__ __forceinline__ void td_claims(bool valid, uint32_t u, uint32_t d, co
         nb = nbr4[u];
         rs = rslot[u];
     }
-    uint32_t hv[4], rv[4];
+    uint32_t mv[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int v = slot_of(nb, k);
-        hv[k] = 0u;
-        rv[k] = 0u;
-        if (v >= 0) {
-            hv[k] = height[v];
-            rv[k] = (static_cast<uint32_t>(nzm[v]) >> ((rs >> (2 * k)) & 3u)) & 1u;
-        }
+        mv[k] = v >= 0 ? static_cast<uint32_t>(nzm[v]) : 0u;
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         claim[k] = false;
-        if (hv[k] == kSent && rv[k] != 0u) claim[k] = atom_cas(&height[slot_of(nb, k)], kSent, d + 1u);
+        // unreached with a residual arc into u; a stale bit only costs a failed CAS
+        if ((mv[k] & kUnreached) != 0u && ((mv[k] >> ((rs >> (2 * k)) & 3u)) & 1u) != 0u) {
+            claim[k] = atom_cas(&height[slot_of(nb, k)], kSent, d + 1u);
+            if (claim[k]) nzm[slot_of(nb, k)] = static_cast<uint8_t>(mv[k] & (kUnreached - 1u));
+        }
     }
 }
 
@@ -424,7 +430,7 @@ template <int TPB>
 __global__ void __launch_bounds__(TPB) k_bfs_level(uint32_t n, uint32_t d, uint32_t bu_threshold,
                                                     const uint32_t* __restrict__ fin, uint32_t* __restrict__ fout,
                                                     const int4* __restrict__ nbr4, const uint8_t* __restrict__ rslot,
-                                                    const uint8_t* __restrict__ nzm, uint32_t* height, Ctl* ctl) {
+                                                    uint8_t* nzm, uint32_t* height, Ctl* ctl) {
     __shared__ WgQueue<TPB> q;
     // The first frontier entry of this thread is loaded before the count is
     // known (the buffer holds n entries; entries past the count are ignored),
@@ -460,7 +466,10 @@ __global__ void __launch_bounds__(TPB) k_bfs_level(uint32_t n, uint32_t d, uint3
                 }
 #pragma unroll
                 for (int k = 0; k < 4; ++k) found |= hu[k] == d;
-                if (found) height[v] = d + 1u;
+                if (found) {
+                    height[v] = d + 1u;
+                    nzm[v] = static_cast<uint8_t>(m & (kUnreached - 1u));
+                }
             }
             q_push(q, found, v);
             q_flush_if(q, fout, out_cnt, TPB);
@@ -495,7 +504,7 @@ constexpr double kRelabelCost = 1.0;
 constexpr int kTailTPB = 1024;
 __global__ void __launch_bounds__(kTailTPB) k_bfs_tail(uint32_t d0, uint32_t max_levels, uint32_t* f0, uint32_t* f1,
                                                    const int4* __restrict__ nbr4, const uint8_t* __restrict__ rslot,
-                                                   const uint8_t* __restrict__ nzm, uint32_t* height, Ctl* ctl) {
+                                                   uint8_t* nzm, uint32_t* height, Ctl* ctl) {
     __shared__ uint32_t s_cnt;
     uint32_t* f[2] = {f0, f1};
     uint32_t cur = ctl->bfs_cnt[d0 % 3u];
@@ -1369,11 +1378,11 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     // A relabel whose BFS expands most of the graph costs a sweep of it
     // however little of the heights the discharge has changed, so it must be
     // paid for by a longer discharge interval: the cost factor rises from 1
-    // when the last BFS expanded three quarters of the vertices or fewer to 3
+    // when the last BFS expanded three quarters of the vertices or fewer to 4
     // when it expanded all of them.
     auto affordable = [&]() -> bool {
         const double reach = std::min(1.0, last_gr_reach);
-        const double factor = kRelabelCost * (1.0 + 8.0 * std::max(0.0, reach - 0.75));
+        const double factor = kRelabelCost * (1.0 + 12.0 * std::max(0.0, reach - 0.75));
         return std::chrono::duration<double, std::milli>(Clock::now() - last_gr_end).count() >= factor * last_gr_ms;
     };
     auto work_trigger = [&]() -> bool {
