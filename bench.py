@@ -30,6 +30,8 @@ Further legs, each reported in its own object of the line:
     against the oracle;
   * surface: surface extraction of the label-0 region, checked against the
     oracle, beside the reference's chunked topology download;
+  * shards: the headline graph's 8 z-slabs solved one by one, each the block
+    one GPU of an 8-way partition would hold (certificate-checked);
   * delaunay: the Qhull mesh of a 400k-point genus-3 cloud, checked against BK;
   * batched: 256 uneven graphs, looped vs one autotuned batched solve;
   * serving: Poisson-arriving requests into the native server, p50/p99;
@@ -87,6 +89,8 @@ def main():
                     help="skip the sparse-grid tet label leg (SparseUDF.sample_tet_labels at --k)")
     ap.add_argument("--no-surface-leg", action="store_true",
                     help="skip the surface extraction leg (surface_extraction_topology_cuda at --k)")
+    ap.add_argument("--no-shard-leg", action="store_true",
+                    help="skip the leg that solves the headline graph's 8 z-slabs one by one")
     ap.add_argument("--no-flow-leg", action="store_true",
                     help="skip the maximum-flow-with-conservation leg (wtv_pr_solve_flow on the same graph)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -184,6 +188,11 @@ def main():
     flow_mode = None
     if not a.no_flow_leg:
         flow_mode = flow_leg(a, torch, np, W, N, nbr, res, term, res0, term0, part, part_h, prm, stream, raw_flow)
+
+    # -------------------------------------------------- one GPU's share of an 8-way partition
+    shards = None
+    if not a.no_shard_leg:
+        shards = shard_leg(a, torch, W, k, nbr, res0, term0, stream)
 
     # -------------------------------------------------- profiled solve
     prof = None
@@ -356,6 +365,7 @@ def main():
         "topology": topology,
         "udf_labels": udf,
         "surface": surface,
+        "shards": shards,
         "delaunay": delaunay,
         "batched": batched,
         "serving": serving,
@@ -688,6 +698,60 @@ def flow_leg(a, torch, np, W, N, nbr, res, term, res0, term0, part, part_h, prm,
         assert fc == 0, f"flow certificate failed with code {fc}"
         out["certificate"] = "maximum flow, conservation at every vertex (oracle flow certificate)"
     log(f"[bench] flow mode: {flow_s * 1e3:.1f} ms, {returned} units of excess returned to the source")
+    return out
+
+
+def shard_leg(a, torch, W, k, nbr, res0, term0, stream, parts=8):
+    """The headline graph cut into `parts` slabs of whole cell layers, along z
+    and along x, each slab solved on its own with the arcs that leave it
+    dropped: the block each GPU of a `parts`-way partition would hold, at its
+    real shape.  A distributed solve adds the ghost exchange per macro round
+    and a distributed BFS per relabel on top, and the slabs' flows do not add
+    up to the whole graph's.  Each slab's flow must equal the integer capacity
+    of its cut.  Tet t lies in cell t // 6, cells numbered x-fastest."""
+    n = nbr.shape[0]
+    cell = torch.arange(n, device=nbr.device, dtype=torch.int64) // 6
+    coord = {"z": cell // (k * k), "x": cell % k}
+    edges = [round(i * k / parts) for i in range(parts + 1)]
+    prm = W.params(global_relabel_period=a.period, local_steps=a.local_steps, relabel_alpha=a.alpha)
+    out = {"workload": f"the headline graph cut into {parts} slabs of whole cell layers along z and along x, each "
+                       "solved alone with its outgoing arcs dropped: one GPU's block of an 8-way partition, "
+                       "without the exchange a distributed solve adds",
+           "parts": parts, "certificate": "each slab's flow == the integer capacity of its cut"}
+    for axis, cc in coord.items():
+        rows = []
+        for i in range(parts):
+            ids = torch.nonzero((cc >= edges[i]) & (cc < edges[i + 1])).flatten()
+            m = int(ids.numel())
+            remap = torch.full((n,), -1, dtype=torch.int64, device=nbr.device)
+            remap[ids] = torch.arange(m, device=nbr.device, dtype=torch.int64)
+            nb = nbr[ids].to(torch.int64)
+            snbr = torch.where(nb >= 0, remap[nb.clamp(min=0)], torch.full_like(nb, -1))
+            keep = snbr >= 0
+            snbr = snbr.to(torch.int32).contiguous()
+            sres0 = torch.where(keep, res0[ids], torch.zeros_like(res0[ids])).contiguous()
+            sterm0 = term0[ids].contiguous()
+            del remap, nb, keep
+            sres, sterm = sres0.clone(), sterm0.clone()
+            spart = torch.empty(m, dtype=torch.uint8, device=nbr.device)
+            times, st = [], None
+            for _ in range(4):
+                sres.copy_(sres0)
+                sterm.copy_(sterm0)
+                st = W.pr_solve(m, snbr.data_ptr(), sres.data_ptr(), sterm.data_ptr(), spart.data_ptr(), 1, prm,
+                                stream)
+                times.append(st.t_solve_ms)
+            torch.cuda.synchronize()
+            cut, _ = cut_capacity(torch, snbr, sres0, sterm0, spart)
+            assert cut == st.raw_flow, f"{axis} slab {i}: cut capacity {cut} != flow {st.raw_flow}"
+            rows.append({"slab": i, "cell_layers": [edges[i], edges[i + 1]], "tets": m,
+                         "t_solve_ms_device": round(min(times[1:]), 3), "global_relabels": st.global_relabels,
+                         "raw_flow": st.raw_flow})
+        ts = [r["t_solve_ms_device"] for r in rows]
+        out[axis] = {"max_ms": max(ts), "mean_ms": round(sum(ts) / parts, 3), "slabs": rows}
+        log(f"[bench] shards along {axis}: {parts} slabs of {min(r['tets'] for r in rows)}-"
+            f"{max(r['tets'] for r in rows)} tets, solve max {max(ts):.2f} ms, mean {out[axis]['mean_ms']:.2f} ms")
+    del cell, coord
     return out
 
 

@@ -23,6 +23,6 @@ def test_every_leg_has_a_switch():
     r = _bench("--help")
     assert r.returncode == 0
     for flag in ["--no-cpu", "--no-profile", "--batch-graphs", "--serve-requests", "--delaunay-points", "--no-tune",
-                 "--no-cut-leg", "--no-topology-leg", "--no-udf-leg", "--no-surface-leg", "--no-flow-leg",
+                 "--no-cut-leg", "--no-topology-leg", "--no-udf-leg", "--no-surface-leg", "--no-flow-leg", "--no-shard-leg",
                  "--copy-probe", "--atomic-probe", "--steps", "--warmup"]:
         assert flag in r.stdout, flag
