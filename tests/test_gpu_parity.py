@@ -88,16 +88,16 @@ def test_golden(orc, torch_cuda, name):
         assert_same(orc, z["nbr"], z["res"], z["term"], gpu, int(z["raw_flow"]), z["partition"])
 
 
-def _c0_soft(k=32, seed=20260000):
+def _kuhn_soft(k=32, seed=20260000):
     m = S.kuhn_mesh(k, seed=seed)
     shape = S.sphere_with_holes(seed)
     lab = S.labels_from_sdf(S.tet_centroids_sdf(m, shape), 0.02, seed=seed + 1)
     return m, lab, S.soft_graph(m, lab, shape, 10.0, 1.0 / k)
 
 
-def test_config0_soft_seed(orc, torch_cuda):
+def test_kuhn32_soft_seed(orc, torch_cuda):
     """196,608-tet jittered Kuhn grid, sphere with 3 holes, 2% flips, soft seeds."""
-    m, lab, g = _c0_soft()
+    m, lab, g = _kuhn_soft()
     a = orc.pr_solve(g.nbr, g.res, g.term)
     gpu = gpu_solve(torch_cuda, g.nbr, g.res, g.term, device_partition=True)
     assert_same(orc, g.nbr, g.res, g.term, gpu, a.raw_flow, a.partition)
@@ -105,10 +105,10 @@ def test_config0_soft_seed(orc, torch_cuda):
     assert S.cut_energy(g, gpu[0]) == S.cut_energy(g, a.partition)
 
 
-def test_config0_reference_model_graph_cut_fast(orc, torch_cuda):
-    """The drop-in entry point on config 0 with the reference capacity formula."""
+def test_kuhn32_reference_model_graph_cut_fast(orc, torch_cuda):
+    """The drop-in entry point on the Kuhn 32^3 mesh (196,608 tets) with the reference capacity formula."""
     import wtivo_gpupr as G
-    m, lab, _ = _c0_soft()
+    m, lab, _ = _kuhn_soft()
     r = orc.reduce(m.V, m.tets, m.nbrs, lab, 10.0)
     sol = orc.pr_solve(r.nbr, r.res, r.term)
     want = orc.scatter(lab, r.free_old, sol.partition)
@@ -147,7 +147,7 @@ def _resident_cut(torch, V, T, Nb, lab_d, out_d, fill_T=10.0):
 def test_graph_cut_fast_resident_equals_host_call(orc, torch_cuda, k):
     """Every array in HBM gives the host call's labels, flows and counters, also in place."""
     torch = torch_cuda
-    m, lab, _ = _c0_soft(k=k)
+    m, lab, _ = _kuhn_soft(k=k)
     T = torch.from_numpy(m.tets).cuda()
     Nb = torch.from_numpy(m.nbrs).cuda()
     host_out = np.empty(m.N, np.uint8)
@@ -174,7 +174,7 @@ def test_graph_cut_fast_resident_equals_host_call(orc, torch_cuda, k):
 
 def test_graph_cut_fast_resident_no_free_tets(torch_cuda):
     torch = torch_cuda
-    m, _, _ = _c0_soft(k=8)
+    m, _, _ = _kuhn_soft(k=8)
     lab = np.zeros(m.N, np.uint8)                           # every tet fixed to the source
     T = torch.from_numpy(m.tets).cuda()
     Nb = torch.from_numpy(m.nbrs).cuda()
@@ -224,18 +224,18 @@ def test_flow_mode_random_graphs(orc, torch_cuda, seed):
     _check_flow_mode(orc, torch_cuda, nbr, res, term)
 
 
-def test_flow_mode_config0_soft_seed(orc, torch_cuda):
-    _, _, g = _c0_soft()
+def test_flow_mode_kuhn32_soft_seed(orc, torch_cuda):
+    _, _, g = _kuhn_soft()
     assert _check_flow_mode(orc, torch_cuda, g.nbr, g.res, g.term) > 0
 
 
-def test_flow_mode_config4_full_size(orc, torch_cuda):
+def test_flow_mode_kuhn150_integer_shell(orc, torch_cuda):
     k = 150
     m = S.kuhn_mesh(k, seed=20260004)
     shape = S.thin_shell_with_tunnels(20260004, 1.0 / k, tunnels=40)
     lab = S.labels_from_sdf(S.tet_centroids_sdf(m, shape, device="cuda"), 0.0)
     g = S.lognormal_int_graph(m, lab, 100.0, seed=20260005, rev_slots=S.KUHN_REV_SLOT)
-    print(f"config 4 flow mode: {m.N} tets", flush=True)
+    print(f"Kuhn 150^3 integer shell, flow mode: {m.N} tets", flush=True)
     _check_flow_mode(orc, torch_cuda, g.nbr, g.res, g.term)
 
 
@@ -264,7 +264,7 @@ def test_stress_integer_shell(orc, torch_cuda):
     assert_same(orc, g.nbr, g.res, g.term, gpu, a.raw_flow, a.partition)
 
 
-def test_config4_full_size_integer_shell(orc, torch_cuda):
+def test_kuhn150_integer_shell_full_size(orc, torch_cuda):
     """The adversarial stress at full size: Kuhn 150^3 = 20,250,000 tets, thin shell
     (wall 1.5 cells) with 40 tunnels, log-normal capacities quantised to [1, 2^20],
     lambda 100.  Flow and labels bit-exact against Boykov-Kolmogorov, certificate."""
@@ -273,7 +273,7 @@ def test_config4_full_size_integer_shell(orc, torch_cuda):
     shape = S.thin_shell_with_tunnels(20260004, 1.0 / k, tunnels=40)
     lab = S.labels_from_sdf(S.tet_centroids_sdf(m, shape, device="cuda"), 0.0)
     g = S.lognormal_int_graph(m, lab, 100.0, seed=20260005, rev_slots=S.KUHN_REV_SLOT)
-    print(f"config 4: {m.N} tets", flush=True)
+    print(f"Kuhn 150^3 integer shell: {m.N} tets", flush=True)
     a = orc.bk_solve(g.nbr, g.res, g.term)
     gpu = gpu_solve(torch_cuda, g.nbr, g.res, g.term, device_partition=True)
     assert_same(orc, g.nbr, g.res, g.term, gpu, a.raw_flow, a.partition)
@@ -293,7 +293,7 @@ def test_clamped_parameters_same_answer(orc, torch_cuda, kw):
 def test_workgroup_size_same_answer_large(orc, torch_cuda, wg):
     """Every list kernel at 512 and 1024 threads per workgroup, in single and
     batched mode, on graphs large enough for the queue flushes to trigger."""
-    _, _, g = _c0_soft(48, 90 + wg)
+    _, _, g = _kuhn_soft(48, 90 + wg)
     a = orc.bk_solve(g.nbr, g.res, g.term)
     assert_same(orc, g.nbr, g.res, g.term, gpu_solve(torch_cuda, g.nbr, g.res, g.term, wg_size=wg, tail_cap=-1),
                 a.raw_flow, a.partition)
@@ -333,14 +333,14 @@ def test_single_vertex(orc, torch_cuda, t):
 
 
 def test_deterministic_labels_and_flow(torch_cuda):
-    _, _, g = _c0_soft(16, 5)
+    _, _, g = _kuhn_soft(16, 5)
     p1, s1, r1, t1 = gpu_solve(torch_cuda, g.nbr, g.res, g.term)
     p2, s2, r2, t2 = gpu_solve(torch_cuda, g.nbr, g.res, g.term)
     assert s1["raw_flow"] == s2["raw_flow"] and np.array_equal(p1, p2)
 
 
 def test_max_rounds_limit(torch_cuda):
-    _, _, g = _c0_soft(16, 6)
+    _, _, g = _kuhn_soft(16, 6)
     with pytest.raises(W.WtvError, match="max macro rounds exceeded") as e:
         gpu_solve(torch_cuda, g.nbr, g.res, g.term, max_rounds=1, local_steps=1)
     assert e.value.code == W.WTV_ELIMIT
@@ -450,7 +450,7 @@ def test_batch_matches_individual_solves(orc, torch_cuda):
     graphs.append((np.array([[1, -1, -1, -1], [0, -1, -1, -1]], np.int32),
                    np.array([[5, 0, 0, 0], [0, 0, 0, 0]], np.uint32), np.array([5, -5], np.int32)))
     graphs.append(tuple(np.zeros((0, 4), np.int32) if i < 2 else np.zeros(0, np.int32) for i in range(3)))
-    _, _, g = _c0_soft(16, 8)
+    _, _, g = _kuhn_soft(16, 8)
     graphs.append((g.nbr, g.res, g.term))
     off, flows, part, r_out, t_out, st = gpu_batch(torch_cuda, graphs)
     for i, (nbr, res, term) in enumerate(graphs):
@@ -493,7 +493,7 @@ def test_batch_delaunay_and_kuhn_graphs_vs_bk(orc, torch_cuda):
         lab = S.labels_from_sdf(S.tet_centroids_sdf(m, shape), 0.02, seed=71 + i)
         g = S.soft_graph(m, lab, shape, float(5 + 10 * i), 1.0 / res)
         graphs.append((g.nbr, g.res, g.term))
-    _, _, g = _c0_soft(20, 72)
+    _, _, g = _kuhn_soft(20, 72)
     graphs.append((g.nbr, g.res, g.term))
     off, flows, part, r_out, t_out, st = gpu_batch(torch_cuda, graphs)
     for i, (nbr, res, term) in enumerate(graphs):
@@ -531,7 +531,7 @@ def _dev_graph(torch, nbr, res, term):
 def test_server_results_equal_alone_and_in_the_mix(orc, torch_cuda):
     graphs = [S.random_graph(int(80 + 29 * i), 900 + i, cap_max=40) for i in range(10)]
     for k, seed in ((12, 1), (20, 2), (16, 3)):
-        _, _, g = _c0_soft(k, seed)
+        _, _, g = _kuhn_soft(k, seed)
         graphs.append((g.nbr, g.res, g.term))
     want = [orc.bk_solve(*g) for g in graphs]
     torch_cuda.cuda.synchronize()
@@ -557,7 +557,7 @@ def test_server_isolates_a_failed_request(orc, torch_cuda):
     """A request whose graph is invalid (a missing reverse arc) fails alone: the
     batch it was admitted with is re-solved request by request, so every other
     request still returns its lone-solve flow and labels."""
-    _, _, big = _c0_soft(24, 11)
+    _, _, big = _kuhn_soft(24, 11)
     good = [S.random_graph(int(200 + 50 * i), 960 + i, cap_max=40) for i in range(5)]
     bad = S.random_graph(300, 3)
     nbr = bad[0].copy()
@@ -701,7 +701,7 @@ def test_reference_model_cut_kuhn96(orc, torch_cuda):
     _reference_model_cut(orc, torch_cuda, m.V, m.tets, m.nbrs, lab)
 
 
-def test_config4_integer_stress_full_size(orc, torch_cuda):
+def test_kuhn150_integer_stress_full_size(orc, torch_cuda):
     """20,250,000-tet (150^3) thin shell with 40 tunnels, lambda 100, log-normal(2)
     capacities quantised to [1, 2^20]: labels and flow bit-exact against BK."""
     k = 150
@@ -740,7 +740,7 @@ def test_surface_extraction_matches_oracle(orc, torch_cuda, seed):
 def test_surface_after_graph_cut_and_errors(orc, torch_cuda):
     """The pipeline order of wtivo.py:842-859: cut, then surface of the new labels."""
     import wtivo_gpupr as G
-    m, lab, _ = _c0_soft(24, 60)
+    m, lab, _ = _kuhn_soft(24, 60)
     T = torch_cuda.from_numpy(m.tets).cuda()
     Nb = torch_cuda.from_numpy(m.nbrs).cuda()
     new_labels = G.graph_cut_fast(m.V, T, Nb, lab, 10.0)[0]
@@ -759,7 +759,7 @@ def test_surface_after_graph_cut_and_errors(orc, torch_cuda):
 def test_graph_replay_equals_plain_launches(orc, torch_cuda):
     """Macro rounds replayed from a captured hipGraph (default) give the same
     answer as kernel-by-kernel launches, over many replays and odd/even local_steps."""
-    _, _, g = _c0_soft(24, 70)
+    _, _, g = _kuhn_soft(24, 70)
     want = orc.bk_solve(g.nbr, g.res, g.term)
     for ls in (1, 3, 8):
         for no_graph in (0, 1):
@@ -769,7 +769,7 @@ def test_graph_replay_equals_plain_launches(orc, torch_cuda):
 
 
 def test_profile_mode_counts_kernels(torch_cuda):
-    _, _, g = _c0_soft(16, 71)
+    _, _, g = _kuhn_soft(16, 71)
     part, st, _, _ = gpu_solve(torch_cuda, g.nbr, g.res, g.term, profile=1)
     assert st["sub_rounds"] > 0 and st["n_level"] > 0 and st["t_level_ms"] > 0
     assert st["n_push"] + st["tail_sub_rounds"] >= st["sub_rounds"]
@@ -787,7 +787,7 @@ def test_profile_mode_counts_kernels(torch_cuda):
 def test_discharge_tail_mode_same_answer(orc, torch_cuda, tail_cap):
     """The one-workgroup discharge (small active lists) and the multi-workgroup
     rounds compute the same flow and labels, whichever handles which phase."""
-    _, _, g = _c0_soft(24, 80)
+    _, _, g = _kuhn_soft(24, 80)
     want = orc.bk_solve(g.nbr, g.res, g.term)
     gpu = gpu_solve(torch_cuda, g.nbr, g.res, g.term, tail_cap=tail_cap)
     assert_same(orc, g.nbr, g.res, g.term, gpu, want.raw_flow, want.partition)

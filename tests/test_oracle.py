@@ -303,7 +303,7 @@ def test_flow_certificate_pinned_by_scipy_flows(orc, seed):
         assert orc.flow_certificate(nbr, res, term, res_f, t2, b.partition, val) == 3
 
 
-def test_soft_graph_c0_bk_equals_pr(orc):
+def test_soft_graph_kuhn32_bk_equals_pr(orc):
     m = S.kuhn_mesh(16, seed=3)
     shape = S.sphere_with_holes(3)
     lab = S.labels_from_sdf(S.tet_centroids_sdf(m, shape), 0.02, seed=4)
