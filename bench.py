@@ -4,7 +4,7 @@
 Workload (the largest single-GPU configuration, "cut solve s @78M tets"):
   * mesh   : jittered Kuhn 6-tet subdivision of a 235^3 grid = 77,867,250 tets
              (stand-in for the 12M-point Delaunay mesh, which cannot be
-             generated on the box in bench time), int32 4-slot layout;
+             generated on the GPU host within a bench run), int32 4-slot layout;
   * labels : genus-3 solid (three fused tori), inside = 0, 2% seeded flips;
   * caps   : soft seeds, lambda_fill = 10: terminal = 10*vol on the labelled
              side, facet = area * exp(-|UDF(facet centroid)|/h), h = 1 cell, fp32,
@@ -99,7 +99,7 @@ def main():
                     help="only run the atomic-add probe (calibrates rocprofv3's atomic counters)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); "
+        print(f"bench.py: the benchmark measures one GPU only (got --gpus {a.gpus}); "
               "multi-GPU sharding is a design note, not an implemented path.", file=sys.stderr)
         sys.exit(2)
 

@@ -1,5 +1,5 @@
 """bench.py's command-line contract, checked on the CPU: more than one GPU is
-refused with a message (this build measures one GPU), and every leg can be
+refused with a message (the benchmark measures one GPU), and every leg can be
 switched off by its own flag."""
 import os
 import subprocess
