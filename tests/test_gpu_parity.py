@@ -134,6 +134,27 @@ def test_kuhn32_reference_model_graph_cut_fast(orc, torch_cuda):
     assert np.array_equal(lab_out, want) and res.adjusted_flow == out[1]
 
 
+@pytest.mark.parametrize("seed", [0, 1])
+def test_graph_cut_fast_three_label_values(orc, torch_cuda, seed):
+    """Labels 0, 1 and 2 through the drop-in call.  Any two different nonzero
+    labels count as different in the capacities (wtivo_gpupr_bindings.cpp:93),
+    a fixed sink tet keeps its input label in the output, and free tets get the
+    partition (:357-364): the whole 18-tuple's labels and flows equal the
+    oracle's reduce -> solve -> scatter."""
+    import wtivo_gpupr as G
+    rng = np.random.default_rng(300 + seed)
+    m = S.kuhn_mesh(12 + 6 * seed, jitter=0.3, seed=300 + seed)
+    lab = rng.choice(np.array([0, 1, 2], np.uint8), size=m.N, p=[0.25, 0.6, 0.15])
+    r = orc.reduce(m.V, m.tets, m.nbrs, lab, 7.5)
+    assert r.n > 0 and (lab[r.free_old] == 2).any()
+    sol = orc.pr_solve(r.nbr, r.res, r.term)
+    want = orc.scatter(lab, r.free_old, sol.partition)
+    out = G.graph_cut_fast(m.V, torch_cuda.from_numpy(m.tets).cuda(), torch_cuda.from_numpy(m.nbrs).cuda(), lab, 7.5)
+    assert np.array_equal(out[0], want)
+    assert out[2] == sol.raw_flow and out[1] == sol.raw_flow + r.info["base_flow"]
+    assert (out[0] == 2).any()  # fixed sink tets labelled 2 keep their label
+
+
 def _resident_cut(torch, V, T, Nb, lab_d, out_d, fill_T=10.0):
     res = W.WtvCutResult()
     Vd = torch.from_numpy(np.ascontiguousarray(V)).cuda()
