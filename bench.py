@@ -946,7 +946,9 @@ def delaunay_leg(a, torch, np, W, S, dev, stream):
 
 
 def batch_leg(a, torch, np, W, S, dev, stream):
-    """Config-2 style batch: G uneven soft-seed graphs, looped plain solves vs one batched solve."""
+    """A batch of G uneven soft-seed Kuhn graphs (50k to --batch-max tets): looped plain
+    solves against one autotuned batched solve, then the same graphs through the
+    serving runtime's four slots."""
     t = time.time()
     graphs = S.batch_workload(a.batch_graphs, a.seed + 2, 50_000, a.batch_max, device=dev)
     sizes = [g["N"] for g in graphs]
@@ -1051,7 +1053,7 @@ def batch_leg(a, torch, np, W, S, dev, stream):
 
 
 def serve_leg(a, torch, np, W, S, dev, stream):
-    """Config-3 style serving: Poisson arrivals of device-resident cut requests (tets
+    """Serving: Poisson arrivals of device-resident cut requests (tets
     log-uniform [100k, 20M], lambda 10) into the native server (continuous batching,
     HBM-footprint admission under 200 GB); p50/p99 latency beside graphs/s."""
     t = time.time()
