@@ -299,6 +299,20 @@ __global__ void __launch_bounds__(kTPB) k_prepare(uint32_t n, const int4* __rest
     q_flush(q, wl0, &ctl->wl_cnt[0]);
 }
 
+// Batched solves: the graphs that still hold an active vertex.  Graphs share
+// no arcs, so a graph without one stays finished; the global relabels before
+// the final one skip it (its heights no longer matter, and the final BFS
+// recomputes them all).
+__global__ void __launch_bounds__(kTPB) k_clear_alive(uint32_t G, uint8_t* __restrict__ galive) {
+    for (uint32_t g = blockIdx.x * kTPB + threadIdx.x; g < G; g += gridDim.x * kTPB) galive[g] = 0u;
+}
+__global__ void __launch_bounds__(kTPB) k_mark_alive(uint32_t p, const uint32_t* __restrict__ wl,
+                                                     const uint32_t* __restrict__ gid, uint8_t* __restrict__ galive,
+                                                     const Ctl* ctl) {
+    const uint32_t cnt = ctl->wl_cnt[p];
+    for (uint32_t i = blockIdx.x * kTPB + threadIdx.x; i < cnt; i += gridDim.x * kTPB) galive[gid[wl[i]]] = 1u;
+}
+
 // BFS seeds: vertices with remaining sink capacity get height 1
 // (gpu_push_relabel_fast.cu:85-101); every other vertex starts unreached.
 // No seed list is built: level 1 always expands bottom-up.
@@ -337,13 +351,23 @@ static_assert(kSeedUnroll * kTPB <= WgQueue<kTPB>::kCap, "queue holds one append
 __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __restrict__ term,
                                                    const int4* __restrict__ nbr4, const uint32_t* __restrict__ res,
                                                    uint32_t* __restrict__ height, uint8_t* __restrict__ nzm,
-                                                   uint32_t* __restrict__ fout, uint32_t first, Ctl* ctl) {
+                                                   uint32_t* __restrict__ fout, uint32_t first,
+                                                   const uint32_t* __restrict__ gid,
+                                                   const uint8_t* __restrict__ galive, Ctl* ctl) {
     __shared__ WgQueue<kTPB> q;
     q_init(q);
     const uint32_t span = kTPB * kSeedUnroll;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * span;
     unsigned long long work = 0;
     for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * span; base < n; base += stride) {
+        if (galive) {
+            // a tile whose graphs are all finished is skipped by the whole
+            // workgroup (graph ids ascend with the node id)
+            const uint64_t last = base + span < n ? base + span - 1u : static_cast<uint64_t>(n) - 1u;
+            bool any = false;
+            for (uint32_t g = gid[base]; g <= gid[last] && !any; ++g) any = galive[g] != 0u;
+            if (!any) continue;
+        }
         uint32_t v[kSeedUnroll];
         int32_t t[kSeedUnroll];
 #pragma unroll
@@ -1039,7 +1063,7 @@ struct GraphExec {
 };
 
 struct Layout {
-    size_t height, excess, rslot, nzm, wl0, wl1, rl, ctl, gid, nbrg, gn, goff, gflow, total;
+    size_t height, excess, rslot, nzm, wl0, wl1, rl, ctl, gid, nbrg, gn, goff, gflow, galive, total;
 };
 
 // One allocation carved into 256-byte aligned arrays.  Batched mode adds the
@@ -1064,6 +1088,7 @@ Layout layout_for(uint32_t n, uint32_t G) {
         L.gn = take(static_cast<size_t>(G) * 4u);
         L.goff = take((static_cast<size_t>(G) + 1u) * 8u);
         L.gflow = take(static_cast<size_t>(G) * 8u);
+        L.galive = take(G);
     }
     L.total = o;
     return L;
@@ -1137,6 +1162,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     uint32_t* gn = kB ? reinterpret_cast<uint32_t*>(base + L.gn) : nullptr;
     unsigned long long* goff = kB ? reinterpret_cast<unsigned long long*>(base + L.goff) : nullptr;
     unsigned long long* gflow = kB ? reinterpret_cast<unsigned long long*>(base + L.gflow) : nullptr;
+    uint8_t* galive = kB ? reinterpret_cast<uint8_t*>(base + L.galive) : nullptr;
 
     HostBuf hctl_buf;
     ck(hipHostMalloc(&hctl_buf.p, 5 * sizeof(Ctl), hipHostMallocDefault), "pinned ctl");
@@ -1201,14 +1227,23 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     // BFS from the sink.  Its frontiers ping-pong between the relabel list
     // (dead between sub-rounds) and the idle worklist, so the active list
     // wl[pos & 1] survives for the filter that follows.
-    auto global_relabel = [&]() {
+    // skip_done (batched solves, relabels between the first and the final
+    // one): the seed sweep passes over graphs without an active vertex
+    auto global_relabel = [&](bool skip_done) {
         ck(hipEventRecord(ev_b0.e, s), "event");
         const uint32_t p = static_cast<uint32_t>(pos & 1u);
         uint32_t* bf[2] = {rl, wl[p ^ 1u]};
         k_bfs_begin<<<1, 1, 0, s>>>(ctl, p);
         ck(hipGetLastError(), "bfs begin");
+        const bool skip = kB && skip_done && st.global_relabels > 0;
+        if (skip) {
+            k_clear_alive<<<std::max(1, static_cast<int>(std::min<uint64_t>((G + tpb - 1) / tpb, 1024))), tpb, 0,
+                            s>>>(G, galive);
+            k_mark_alive<<<grid, tpb, 0, s>>>(p, wl[p], gid, galive, ctl);
+            ck(hipGetLastError(), "mark alive");
+        }
         k_bfs_seed<<<sweep_grid, tpb, 0, s>>>(n, d_term, nbr4, d_res, height, nzm, bf[0],
-                                              st.global_relabels == 0 ? 1u : 0u, ctl);
+                                              st.global_relabels == 0 ? 1u : 0u, gid, skip ? galive : nullptr, ctl);
         ck(hipGetLastError(), "bfs seed");
         // Levels run in batches of 8 launches.  Batches are pipelined: batch
         // b+1 is enqueued before the frontier count after batch b is read, and
@@ -1289,7 +1324,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         return hctl->wl_cnt[p ^ 1u];
     };
 
-    global_relabel();
+    global_relabel(false);
     check_err("global relabel");
     uint32_t active = filter_active();
 
@@ -1367,7 +1402,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     const uint32_t tail_cap = prm.tail_cap == 0 ? 512u : (prm.tail_cap < 0 ? 0u : static_cast<uint32_t>(prm.tail_cap));
     uint64_t relabels_now = relabels_at_gr;
     auto do_global_relabel = [&]() {
-        global_relabel();
+        global_relabel(true);
         active = filter_active();
         relabels_now = relabels_at_gr = stat_sum(*hctl, kStRelabels);
         const uint64_t pops = stat_sum(*hctl, kStBfsPops);
@@ -1499,7 +1534,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     st.solve_seconds = std::chrono::duration<double>(Clock::now() - solve0).count();
 
     const auto fr0 = Clock::now();
-    global_relabel();
+    global_relabel(false);  // every graph: the labels come from this BFS
     st.final_relabel_seconds = std::chrono::duration<double>(Clock::now() - fr0).count();
 
     uint8_t* d_part = partition_on_device ? partition : reinterpret_cast<uint8_t*>(rl);
