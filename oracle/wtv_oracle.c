@@ -41,6 +41,14 @@
  *               on random graphs; pr_solve == bk_solve on every instance.
  *   partition - 2^n enumeration of all cuts on graphs with n <= 14 free nodes:
  *               the labels are the unique minimal sink set of the min cut.
+ *   scatter   - literal label arrays worked out by hand (labels 0, 1, 7 mixed,
+ *               free tets from the classification of a 3^3 Kuhn mesh); a scatter
+ *               indexed by reduced id, unnormalised bytes or a zero start fail.
+ *   surface   - a three-tet mesh whose faces and vertices are written out by
+ *               hand: tet-major / slot-minor order, orientation, vertices in
+ *               ascending original id; slot-major order, first-use compaction
+ *               or unoriented faces fail.  Closed-surface invariants on Kuhn
+ *               blobs (each directed edge once, enclosed volume).
  *
  * Graph representation (the reduced graph of the reference):
  *   n nodes; nbr[4v+k] = neighbour id or -1; res[4v+k] = residual capacity of

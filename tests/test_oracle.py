@@ -335,6 +335,50 @@ def test_golden_fixture_reproduces(orc, name):
         assert np.array_equal(sol.partition, z["partition"])
 
 
+# ------------------------------------------------------------ label scatter
+def test_scatter_known_answer(orc):
+    """Label reconstruction (wtivo_gpupr_bindings.cpp:357-364): out starts as a copy
+    of the input labels, then every free tet free_old[q] takes partition[q] != 0.
+    Labels 0, 1 and 7 are mixed; the expected array is written out by hand."""
+    labels = np.array([7, 0, 1, 7, 0, 1, 7, 1], np.uint8)
+    free_old = np.array([0, 3, 5, 7], np.uint32)   # reduced id q -> tet index
+    partition = np.array([0, 1, 0, 5], np.uint8)    # any nonzero byte means "reaches the sink"
+    want = np.array([0, 0, 1, 1, 0, 0, 7, 1], np.uint8)
+    out = orc.scatter(labels, free_old, partition)
+    assert out.tolist() == want.tolist()
+    # the fixed tets (label 0 sources; label 1/7 sinks) keep their input label
+    fixed = np.setdiff1d(np.arange(8), free_old)
+    assert np.array_equal(out[fixed], labels[fixed])
+    # plausible mistakes produce a different array, so the literal discriminates
+    by_reduced_id = labels.copy()
+    by_reduced_id[:4] = partition != 0                      # indexed by q, not free_old[q]
+    raw_bytes = labels.copy()
+    raw_bytes[free_old] = partition                         # no normalisation to 0/1
+    from_zero = np.zeros(8, np.uint8)
+    from_zero[free_old] = partition != 0                    # fixed tets not copied from the input
+    for wrong in (by_reduced_id, raw_bytes, from_zero):
+        assert wrong.tolist() != want.tolist()
+
+
+def test_scatter_through_reduce_known_answer(orc):
+    """Classify -> reduced ids -> scatter on a 3^3 Kuhn mesh (162 tets, tet id =
+    cell * 6 + perm).  Every tet has label 0 except the six of the centre cell
+    (cell 13: tets 78..83, labels 7 1 7 1 7 1), which have four neighbours each and
+    so are the free tets, in increasing tet order (wtivo_gpupr_bindings.cpp:157-188).
+    A hand-picked partition then lands on tets 78, 80 and 81 only
+    (wtivo_gpupr_bindings.cpp:357-364)."""
+    m = S.kuhn_mesh(3, jitter=0.1, seed=5)
+    lab = np.zeros(m.N, np.uint8)
+    lab[78:84] = [7, 1, 7, 1, 7, 1]
+    r = orc.reduce(m.V, m.tets, m.nbrs, lab, 10.0)
+    assert r.free_old.tolist() == [78, 79, 80, 81, 82, 83]
+    assert r.info["fixed_source"] == 156 and r.info["fixed_sink"] == 0
+    out = orc.scatter(lab, r.free_old, np.array([1, 0, 1, 1, 0, 0], np.uint8))
+    want = np.zeros(162, np.uint8)
+    want[[78, 80, 81]] = 1
+    assert out.tolist() == want.tolist()
+
+
 # ------------------------------------------------------------ surface extraction
 def test_surface_single_tet_outward(orc):
     V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
@@ -371,9 +415,56 @@ def test_surface_is_closed_and_encloses_the_region(orc, seed):
     T = m.V[m.tets[lab == 0]]
     vol_tets = np.abs(np.einsum("ij,ij->i", T[:, 1] - T[:, 0], np.cross(T[:, 2] - T[:, 0], T[:, 3] - T[:, 0]))).sum() / 6
     assert abs(vol_surf - vol_tets) < 1e-9 * max(1.0, vol_tets)
-    # vertices compacted in ascending original id
-    used = np.unique(np.concatenate([m.tets[lab == 0].ravel()]))
-    assert OV.shape[0] <= used.shape[0]
+    # vertices compacted in ascending original id: map every output vertex back to
+    # its original id through its (jittered, hence unique) coordinates
+    row_of = {tuple(v): i for i, v in enumerate(m.V.tolist())}
+    ids = np.array([row_of[tuple(v)] for v in OV.tolist()])
+    lut = np.array([[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]])
+    on_surface = set()
+    for i in np.flatnonzero(lab == 0):
+        for fi in range(4):
+            j = m.nbrs[i, fi]
+            if j < 0 or lab[j] != 0:
+                on_surface.update(m.tets[i, lut[fi]].tolist())
+    assert ids.tolist() == sorted(on_surface)
+
+
+def test_surface_known_answer(orc):
+    """Three tets, written out by hand (wtivo_gpupr_bindings.cpp:400-546):
+        T0 = [4,0,3,2] label 0, T1 = [2,3,4,5] label 7, T2 = [6,2,0,3] label 0,
+    slot fi = local face {012,013,023,123}[fi]; T0 meets T1 across {2,3,4} (T0
+    slot 2) and T2 across {0,2,3} (T0 slot 3, T2 slot 3).  Faces are taken from
+    label-0 tets towards the hull or a label != 0 tet, tet-major then slot-minor;
+    each face's ids are sorted and then oriented away from the tet's fourth vertex
+    by the sign of the FP64 volume; the used vertices {0,2,3,4,6} are renumbered in
+    ascending original id (vertices 1 and 5 are not used)."""
+    V = np.array([[0, 0, 0], [9, 9, 9], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 1], [0, 0, -1]], float)
+    tets = np.array([[4, 0, 3, 2], [2, 3, 4, 5], [6, 2, 0, 3]], np.int32)
+    nbrs = np.array([[-1, -1, 1, 2], [0, -1, -1, -1], [-1, -1, -1, 0]], np.int32)
+    labels = np.array([0, 7, 0], np.uint8)
+    OV, OF = orc.surface(labels, V, tets, nbrs)
+    # original ids, worked by hand: T0 slots 0,1,2 then T2 slots 0,1,2
+    raw = [(0, 4, 3), (0, 2, 4), (2, 3, 4), (0, 6, 2), (2, 6, 3), (0, 3, 6)]
+    remap = {0: 0, 2: 1, 3: 2, 4: 3, 6: 4}
+    want_f = [[remap[x] for x in f] for f in raw]
+    assert want_f == [[0, 3, 2], [0, 1, 3], [1, 2, 3], [0, 4, 1], [1, 4, 2], [0, 2, 4]]
+    assert OF.tolist() == want_f
+    assert OV.tolist() == V[[0, 2, 3, 4, 6]].tolist()
+    # plausible mistakes give a different answer
+    slot_major = [want_f[i] for i in (0, 3, 1, 4, 2, 5)]
+    first_use = {}
+    for f in raw:
+        for x in f:
+            first_use.setdefault(x, len(first_use))
+    first_use_f = [[first_use[x] for x in f] for f in raw]
+    unoriented = [sorted(f) for f in want_f]
+    for wrong in (slot_major, first_use_f, unoriented):
+        assert wrong != want_f
+    # each face normal points away from its tet's fourth vertex
+    fourth = [2, 3, 0, 3, 0, 2]
+    for f, q in zip(raw, fourth):
+        a, b, c = V[list(f)]
+        assert np.dot(np.cross(b - a, c - a), V[q] - a) < 0
 
 
 def test_extreme_capacities_bk_equals_pr(orc):
