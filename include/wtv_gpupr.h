@@ -234,12 +234,20 @@ typedef struct wtv_server_config {
     wtv_params params;         /* solver knobs for every batch                   */
 } wtv_server_config;
 
+/* Ordering: the solve reads d_nbr, d_res and d_term only after the work that
+ * was queued on `stream` when wtv_server_submit was called (the server
+ * records an event there and its worker waits for it), so a client may write
+ * the graph asynchronously on its own stream and submit without a host sync.
+ * With stream == NULL the buffers must be complete at submit (host
+ * synchronised).  Either way the client must not touch the four buffers
+ * between submit and the return of wtv_server_wait for the ticket. */
 typedef struct wtv_request {
     uint32_t n;                /* free nodes                                     */
     int32_t* d_nbr;            /* int32[n*4], read only                          */
     uint32_t* d_res;           /* uint32[n*4], in/out                            */
     int32_t* d_term;           /* int32[n], in/out                               */
     uint8_t* d_partition;      /* uint8[n] device, out                           */
+    void* stream;              /* hipStream_t the buffers were written on, or NULL */
 } wtv_request;
 
 typedef struct wtv_result {

@@ -367,6 +367,26 @@ def test_max_rounds_limit(torch_cuda):
     assert e.value.code == W.WTV_ELIMIT
 
 
+def test_time_limit_inside_the_discharge_loop(orc, torch_cuda):
+    """The wall-clock limit (gpu_push_relabel_fast.cu:538-544) holds inside the
+    pipelined macro-round loop: with no periodic or adaptive relabel and the
+    one-workgroup tail off, a 1.57M-tet graph never leaves that loop, and a 2 ms
+    limit must end the solve with WTV_ELIMIT (max_rounds is only a backstop).
+    The kernels still queued finish before the call returns: the next solve on
+    the same thread gives the exact answer."""
+    k = 64
+    m = S.kuhn_mesh(k, seed=9)
+    shape = S.genus3(9)
+    lab = S.labels_from_sdf(S.tet_centroids_sdf(m, shape), 0.02, seed=10)
+    g = S.soft_graph(m, lab, shape, 10.0, 1.0 / k, rev_slots=S.KUHN_REV_SLOT)
+    with pytest.raises(W.WtvError, match=r"solve exceeded 0\.002s safety limit") as e:
+        gpu_solve(torch_cuda, g.nbr, g.res, g.term, global_relabel_period=0, relabel_alpha=-1.0, tail_cap=-1,
+                  local_steps=1, time_limit_s=0.002, max_rounds=20000)
+    assert e.value.code == W.WTV_ELIMIT
+    part, st, _, _ = gpu_solve(torch_cuda, g.nbr, g.res, g.term, device_partition=True)
+    assert S.quantised_cut(g.nbr, g.res, g.term, part) == st["raw_flow"]
+
+
 def test_asymmetric_neighbours_report_invariant(torch_cuda):
     nbr, res, term = S.random_graph(50, 3)
     v = int(np.argwhere(nbr >= 0)[0][0])
@@ -602,6 +622,36 @@ def test_server_isolates_a_failed_request(orc, torch_cuda):
         w = orc.bk_solve(*g)
         assert r.status == 0 and r.raw_flow == w.raw_flow, i
         assert np.array_equal(d[3].cpu().numpy()[:g[2].shape[0]], w.partition), i
+
+
+def test_server_orders_the_solve_after_the_request_stream(orc, torch_cuda):
+    """A client writes its graphs asynchronously on its own stream, behind a
+    ~50 ms spin kernel, and submits at once with that stream: the solves wait for
+    the writes (the server records an event on the stream at submit), so every
+    result equals BK's on the graph as written."""
+    torch = torch_cuda
+    graphs = [S.random_graph(int(300 + 40 * i), 970 + i, cap_max=40) for i in range(4)]
+    _, _, g = _kuhn_soft(16, 12)
+    graphs.append((g.nbr, g.res, g.term))
+    dev = [_dev_graph(torch, np.zeros_like(a), np.zeros_like(b), np.zeros_like(c)) for a, b, c in graphs]
+    host = [tuple(torch.from_numpy(np.ascontiguousarray(x).view(np.int32)).pin_memory() for x in gr)
+            for gr in graphs]
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    with W.Server(slots=2, max_batch=4) as srv:
+        with torch.cuda.stream(side):
+            torch.cuda._sleep(100_000_000)
+            for d, h in zip(dev, host):
+                for dst, src in zip(d[:3], h):
+                    dst.copy_(src.view(dst.shape), non_blocking=True)
+        tickets = [srv.submit(gr[2].shape[0], d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), d[3].data_ptr(),
+                              stream=side.cuda_stream) for gr, d in zip(graphs, dev)]
+        res = [srv.wait(t) for t in tickets]
+    torch.cuda.synchronize()
+    for gr, d, r in zip(graphs, dev, res):
+        w = orc.bk_solve(*gr)
+        assert r.status == 0 and r.raw_flow == w.raw_flow
+        assert np.array_equal(d[3].cpu().numpy()[:gr[2].shape[0]], w.partition)
 
 
 def test_server_admission_by_footprint(orc, torch_cuda):

@@ -514,52 +514,67 @@ BuildSums read_build_ctl(const BuildCtl* d, hipStream_t s, const char* where) {
 
 }  // namespace
 
-// Builds the reduced graph on the device.  Outputs are caller buffers sized
-// for N nodes (only the first free_n entries are written); their contents
-// match the reference's host build exactly.  Returns false when there are no
-// free nodes (the caller then returns the input labels).
-bool build_reduced(const double* V, bool v_on_device, size_t nv, const int32_t* dT, const int32_t* dN, size_t N,
-                   const uint8_t* d_lab, double fill_T, int32_t* gN, uint32_t* gR, int32_t* gT, uint32_t* free_old,
-                   BuildInfo& info, hipStream_t s) {
+// Scratch of one reduced-graph build: the class of every tet, free-tet counts
+// per tile and the control block.
+struct BuildScratch {
+    Dev dCls, dTiles, dCtl;
+    uint32_t tiles = 0;
+};
+
+// Classification and its counters (wtivo_gpupr_bindings.cpp:139-188): label 0
+// -> fixed source, label != 0 with a hull face -> fixed sink, else free.
+// Fills the classify counters of info and returns the free tet count.
+uint64_t classify_tets(const int32_t* dN, size_t N, const uint8_t* d_lab, BuildScratch& bs, BuildInfo& info,
+                       hipStream_t s) {
     std::memset(&info, 0, sizeof(info));
     const uint32_t n32 = static_cast<uint32_t>(N);
-    const uint32_t tiles = static_cast<uint32_t>((N + kTile - 1) / kTile);
-    if (tiles > (1u << 20)) fail(WTV_EINVAL, ">2^32 full tets unsupported");
-    Dev dCls, dTiles, dCtl, dV;
-    ckg(hipMalloc(&dCls.p, N * 4u), "malloc classes");
-    ckg(hipMalloc(&dTiles.p, static_cast<size_t>(tiles) * 4u), "malloc tiles");
-    ckg(hipMalloc(&dCtl.p, sizeof(BuildCtl)), "malloc build ctl");
-    ckg(hipMemsetAsync(dCtl.p, 0, sizeof(BuildCtl), s), "build ctl clear");
+    bs.tiles = static_cast<uint32_t>((N + kTile - 1) / kTile);
+    if (bs.tiles > (1u << 20)) fail(WTV_EINVAL, ">2^32 full tets unsupported");
+    ckg(hipMalloc(&bs.dCls.p, N * 4u), "malloc classes");
+    ckg(hipMalloc(&bs.dTiles.p, static_cast<size_t>(bs.tiles) * 4u), "malloc tiles");
+    ckg(hipMalloc(&bs.dCtl.p, sizeof(BuildCtl)), "malloc build ctl");
+    ckg(hipMemsetAsync(bs.dCtl.p, 0, sizeof(BuildCtl), s), "build ctl clear");
     const int4* nbr4 = reinterpret_cast<const int4*>(dN);
-    const int4* tets4 = reinterpret_cast<const int4*>(dT);
-    BuildCtl* ctl = dCtl.as<BuildCtl>();
-    BuildSums h;
-    // classify + reduced ids (wtivo_gpupr_bindings.cpp:139-188)
-    k_classify<<<tiles, kB, 0, s>>>(n32, nbr4, d_lab, dCls.as<int32_t>(), dTiles.as<uint32_t>(), ctl);
+    BuildCtl* ctl = bs.dCtl.as<BuildCtl>();
+    k_classify<<<bs.tiles, kB, 0, s>>>(n32, nbr4, d_lab, bs.dCls.as<int32_t>(), bs.dTiles.as<uint32_t>(), ctl);
     ckg(hipGetLastError(), "classify");
-    h = read_build_ctl(ctl, s, "classify D2H");
+    const BuildSums h = read_build_ctl(ctl, s, "classify D2H");
     if (h.bad_range) fail(WTV_EINVAL, "neighbor out of range");
     if (h.full_refs & 1ull) fail(WTV_EINVAL, "neighbor reference count not even");
     info.free_n = h.free_n;
     info.fixed_source = h.fixed_source;
     info.fixed_sink = h.fixed_sink;
     info.full_refs = h.full_refs;
-    if (h.free_n == 0) return false;
     if (h.free_n > static_cast<uint64_t>(std::numeric_limits<int32_t>::max()))
         fail(WTV_EINVAL, ">INT32_MAX reduced nodes unsupported");
+    return h.free_n;
+}
+
+// The rest of the build after classify_tets found free tets: reduced ids in
+// increasing tet order, capacity_exact_formula and terminal folding
+// (wtivo_gpupr_bindings.cpp:179-318).  Outputs hold at least free_n entries;
+// their contents match the reference's host build exactly.
+void build_graph(const double* V, bool v_on_device, size_t nv, const int32_t* dT, const int32_t* dN, size_t N,
+                 const uint8_t* d_lab, double fill_T, int32_t* gN, uint32_t* gR, int32_t* gT, uint32_t* free_old,
+                 BuildScratch& bs, BuildInfo& info, hipStream_t s) {
+    const uint32_t n32 = static_cast<uint32_t>(N);
+    const int4* nbr4 = reinterpret_cast<const int4*>(dN);
+    const int4* tets4 = reinterpret_cast<const int4*>(dT);
+    BuildCtl* ctl = bs.dCtl.as<BuildCtl>();
+    Dev dV;
     const double* vdev = V;
     if (!v_on_device) {
         ckg(hipMalloc(&dV.p, std::max<size_t>(nv, 1) * 24u), "malloc vertices");
         pinned_to_device(V, nv * 24u, dV.p, 0, s);
         vdev = dV.as<double>();
     }
-    k_tile_scan<<<1, kB, 0, s>>>(tiles, dTiles.as<uint32_t>());
-    k_assign<<<tiles, kB, 0, s>>>(n32, dCls.as<int32_t>(), dTiles.as<uint32_t>(), free_old);
+    k_tile_scan<<<1, kB, 0, s>>>(bs.tiles, bs.dTiles.as<uint32_t>());
+    k_assign<<<bs.tiles, kB, 0, s>>>(n32, bs.dCls.as<int32_t>(), bs.dTiles.as<uint32_t>(), free_old);
     const int grid = static_cast<int>(std::min<uint64_t>((N + kB - 1) / kB, 8192));
-    k_build<<<grid, kB, 0, s>>>(n32, vdev, nv, tets4, nbr4, d_lab, dCls.as<int32_t>(), fill_T,
+    k_build<<<grid, kB, 0, s>>>(n32, vdev, nv, tets4, nbr4, d_lab, bs.dCls.as<int32_t>(), fill_T,
                                 reinterpret_cast<int4*>(gN), reinterpret_cast<uint4*>(gR), gT, ctl);
     ckg(hipGetLastError(), "build");
-    h = read_build_ctl(ctl, s, "build D2H");
+    const BuildSums h = read_build_ctl(ctl, s, "build D2H");
     // the reference's build lambda turns every failure, the capacity
     // overflow_error included, into one runtime_error (:292,296-297)
     if (h.bad_build == 6) fail(WTV_EINVAL, "invalid graph reduction/capacity state (capacity outside safe 30-bit range)");
@@ -569,6 +584,17 @@ bool build_reduced(const double* V, bool v_on_device, size_t nv, const int32_t* 
     info.free_refs = h.free_refs;
     info.terminal_edges = h.terminal_edges;
     info.base_flow = h.base_flow;
+}
+
+// Builds the reduced graph on the device into caller buffers holding at least
+// free_n entries.  Returns false when there are no free nodes (the caller then
+// returns the input labels).
+bool build_reduced(const double* V, bool v_on_device, size_t nv, const int32_t* dT, const int32_t* dN, size_t N,
+                   const uint8_t* d_lab, double fill_T, int32_t* gN, uint32_t* gR, int32_t* gT, uint32_t* free_old,
+                   BuildInfo& info, hipStream_t s) {
+    BuildScratch bs;
+    if (classify_tets(dN, N, d_lab, bs, info, s) == 0) return false;
+    build_graph(V, v_on_device, nv, dT, dN, N, d_lab, fill_T, gN, gR, gT, free_old, bs, info, s);
     return true;
 }
 
@@ -612,41 +638,47 @@ void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t
     }
     phase("labels upload");
 
-    // VRAM precheck with the reference's estimate (:200-212) plus this
-    // solver's workspace and the build's own buffers, for the worst case n = N
-    size_t gpu_free = 0, gpu_total = 0;
-    ckg(hipMemGetInfo(&gpu_free, &gpu_total), "hipMemGetInfo");
-    const uint64_t need = static_cast<uint64_t>(N) * (36ull + 26ull + 4ull + 4ull + 1ull) +
-                          (resident ? 0ull : static_cast<uint64_t>(nv) * 24u) + (256ull << 20);
-    if (need > static_cast<uint64_t>(gpu_free)) fail(WTV_ENOMEM, "insufficient VRAM for reduced graph + solver");
-
-    phase("vram precheck");
     const auto build0 = Clock::now();
-    Dev gN, gR, gT, dFreeOld;
-    ckg(hipMalloc(&gN.p, N * 16u), "malloc graph nbr");
-    ckg(hipMalloc(&gR.p, N * 16u), "malloc graph res");
-    ckg(hipMalloc(&gT.p, N * 4u), "malloc graph term");
-    ckg(hipMalloc(&dFreeOld.p, N * 4u), "malloc free_old");
-    phase("graph malloc");
     BuildInfo bi{};
-    const bool any = build_reduced(V, resident, nv, dT, dN, N, d_lab, fill_T, gN.as<int32_t>(), gR.as<uint32_t>(),
-                                   gT.as<int32_t>(), dFreeOld.as<uint32_t>(), bi, s);
-    const auto build1 = Clock::now();
-    phase("build");
+    BuildScratch bs;
+    const uint64_t free_N = classify_tets(dN, N, d_lab, bs, bi, s);
+    phase("classify");
     R.full_edges = bi.full_refs / 2u;
     R.fixed_source = bi.fixed_source;
     R.fixed_sink = bi.fixed_sink;
-    if (!any) {
+    if (free_N == 0) {
         if (resident) {
             ckg(hipMemcpyAsync(out_labels, lp, N, hipMemcpyDeviceToDevice, s), "labels copy");
             ckg(hipStreamSynchronize(s), "labels copy");
         } else {
             std::memcpy(out_labels, lp, N);
         }
+        R.build_upload_seconds = std::chrono::duration<double>(Clock::now() - build0).count();
         R.native_total_seconds = std::chrono::duration<double>(Clock::now() - all0).count();
         return;
     }
-    const uint64_t free_N = bi.free_n;
+
+    // VRAM precheck after classification, as the reference (:200-212): the
+    // reduced graph (36 B) plus this solver's workspace (26 B), the reduced id
+    // map (4 B) and the partition (1 B) per free node, the vertices and output
+    // labels still to be placed, and 256 MiB of headroom
+    size_t gpu_free = 0, gpu_total = 0;
+    ckg(hipMemGetInfo(&gpu_free, &gpu_total), "hipMemGetInfo");
+    const uint64_t need = free_N * (36ull + 26ull + 4ull + 1ull) +
+                          (resident ? 0ull : static_cast<uint64_t>(nv) * 24u + static_cast<uint64_t>(N)) +
+                          (256ull << 20);
+    if (need > static_cast<uint64_t>(gpu_free)) fail(WTV_ENOMEM, "insufficient VRAM for reduced graph + solver");
+    phase("vram precheck");
+    Dev gN, gR, gT, dFreeOld;
+    ckg(hipMalloc(&gN.p, free_N * 16u), "malloc graph nbr");
+    ckg(hipMalloc(&gR.p, free_N * 16u), "malloc graph res");
+    ckg(hipMalloc(&gT.p, free_N * 4u), "malloc graph term");
+    ckg(hipMalloc(&dFreeOld.p, free_N * 4u), "malloc free_old");
+    phase("graph malloc");
+    build_graph(V, resident, nv, dT, dN, N, d_lab, fill_T, gN.as<int32_t>(), gR.as<uint32_t>(), gT.as<int32_t>(),
+                dFreeOld.as<uint32_t>(), bs, bi, s);
+    const auto build1 = Clock::now();
+    phase("build");
 
     wtv_params prm;
     wtv_params_default(&prm);

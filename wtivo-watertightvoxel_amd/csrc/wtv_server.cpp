@@ -39,8 +39,10 @@ struct Job {
     wtv_request req;
     uint64_t footprint;
     double submit_s;
+    hipEvent_t ready = nullptr;  // recorded on req.stream at submit
     bool done = false;
     wtv_result res{};
+    ~Job() { if (ready) (void)hipEventDestroy(ready); }
 };
 
 struct DevMem {
@@ -95,6 +97,11 @@ class Server {
         std::lock_guard<std::mutex> lk(mu_);
         if (stop_) fail(WTV_EINVAL, std::string(kPrefix) + "server is stopping");
         auto job = std::make_shared<Job>();
+        if (r.stream) {
+            // the solve waits for what the client queued on its stream so far
+            cks(hipEventCreateWithFlags(&job->ready, hipEventDisableTiming), "ready event");
+            cks(hipEventRecord(job->ready, static_cast<hipStream_t>(r.stream)), "ready record");
+        }
         job->ticket = next_ticket_++;
         job->req = r;
         job->footprint = fp;
@@ -209,6 +216,8 @@ class Server {
 
     void run(const std::vector<std::shared_ptr<Job>>& batch, std::vector<int64_t>& flows, hipStream_t s) {
         wtv_stats st{};
+        for (const auto& j : batch)
+            if (j->ready) cks(hipStreamWaitEvent(s, j->ready, 0), "ready wait");
         if (batch.size() == 1) {
             const wtv_request& r = batch[0]->req;
             pr_solve(r.n, r.d_nbr, r.d_res, r.d_term, r.d_partition, true, cfg_.params, st, s);

@@ -44,6 +44,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <sstream>
 #include <type_traits>
 #include <vector>
@@ -952,15 +953,6 @@ struct DeviceBuf {
     void* p = nullptr;
     ~DeviceBuf() { if (p) (void)hipFree(p); }
 };
-struct HostBuf {
-    void* p = nullptr;
-    ~HostBuf() { if (p) (void)hipHostFree(p); }
-};
-struct Event {
-    hipEvent_t e = nullptr;
-    Event() { ck(hipEventCreate(&e), "event create"); }
-    ~Event() { if (e) (void)hipEventDestroy(e); }
-};
 
 // Per-launch HIP event pairs on the solver's stream (profile mode only).
 // Completed pairs are harvested in launch order; events are not reused within
@@ -1042,20 +1034,85 @@ double event_pair_overhead_us(int grid, int wg, hipStream_t s) {
     return std::max(0.0, (pairs - static_cast<double>(batch)) / kCal * 1e3);
 }
 
-// One non-blocking stream per host thread and device for the solver's own
-// work (default priority).  Graph capture needs a stream other than the null
-// stream; the caller's stream is ordered before and after through events.
-hipStream_t solver_stream() {
-    thread_local std::vector<std::pair<int, hipStream_t>> cache;
+// Resources of one solve: the solver's own non-blocking stream (default
+// priority; graph capture needs a stream other than the null stream, and the
+// caller's stream is ordered before and after through events), a pinned block
+// for control-block snapshots and the events of the host loop.  They come
+// from a process-wide pool, per device, and go back to it when the solve
+// ends, so a process holds as many sets as it ever ran solves at once: server
+// worker threads that come and go create none of their own, and no set is
+// ever released (nothing is torn down at process exit).
+constexpr int kSolveEvents = 10;
+constexpr int kCtlSnapshots = 5;
+struct SolveRes {
     int dev = 0;
-    ck(hipGetDevice(&dev), "hipGetDevice");
-    for (auto& e : cache)
-        if (e.first == dev) return e.second;
-    hipStream_t st = nullptr;
-    ck(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream create");
-    cache.emplace_back(dev, st);
-    return st;
+    hipStream_t s = nullptr;
+    void* hctl = nullptr;  // kCtlSnapshots * sizeof(Ctl), pinned
+    hipEvent_t ev[kSolveEvents] = {};
+};
+
+class ResPool {
+  public:
+    SolveRes acquire() {
+        int dev = 0;
+        ck(hipGetDevice(&dev), "hipGetDevice");
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (size_t i = 0; i < free_.size(); ++i)
+                if (free_[i].dev == dev) {
+                    SolveRes r = free_[i];
+                    free_.erase(free_.begin() + static_cast<std::ptrdiff_t>(i));
+                    return r;
+                }
+        }
+        SolveRes r;
+        r.dev = dev;
+        struct Partial {  // releases what was created if a later step fails
+            SolveRes& r;
+            bool done = false;
+            ~Partial() {
+                if (done) return;
+                for (auto e : r.ev) if (e) (void)hipEventDestroy(e);
+                if (r.hctl) (void)hipHostFree(r.hctl);
+                if (r.s) (void)hipStreamDestroy(r.s);
+            }
+        } partial{r};
+        ck(hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking), "stream create");
+        ck(hipHostMalloc(&r.hctl, kCtlSnapshots * sizeof(Ctl), hipHostMallocDefault), "pinned ctl");
+        for (auto& e : r.ev) ck(hipEventCreate(&e), "event create");
+        partial.done = true;
+        return r;
+    }
+    void release(const SolveRes& r) {
+        std::lock_guard<std::mutex> lk(mu_);
+        free_.push_back(r);
+    }
+
+  private:
+    std::mutex mu_;
+    std::vector<SolveRes> free_;
+};
+
+ResPool& res_pool() {
+    static ResPool* pool = new ResPool;  // never destroyed: no HIP calls at exit
+    return *pool;
 }
+
+// A set of solve resources for the lifetime of one solve.
+struct ResLease {
+    SolveRes r;
+    ResLease() : r(res_pool().acquire()) {}
+    ~ResLease() { res_pool().release(r); }
+    ResLease(const ResLease&) = delete;
+    ResLease& operator=(const ResLease&) = delete;
+};
+
+// Waits for a stream when the scope ends, also when an exception unwinds, so
+// that the buffers declared before it outlive every kernel queued on it.
+struct DrainOnExit {
+    hipStream_t s;
+    ~DrainOnExit() { (void)hipStreamSynchronize(s); }
+};
 
 struct GraphExec {
     hipGraphExec_t exec = nullptr;
@@ -1135,8 +1192,12 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     // a level whose frontier holds more than n/32 vertices expands bottom-up
     const uint32_t bu_threshold = std::max<uint32_t>(n / 32u, 4096u);
     hipStream_t cs = static_cast<hipStream_t>(stream_v);
-    hipStream_t s = solver_stream();
-    Event ev_caller, ev_done;
+    ResLease lease;
+    hipStream_t s = lease.r.s;
+    struct EvRef {
+        hipEvent_t e;
+    };
+    EvRef ev_caller{lease.r.ev[0]}, ev_done{lease.r.ev[1]};
     ck(hipEventRecord(ev_caller.e, cs), "event");
     ck(hipStreamWaitEvent(s, ev_caller.e, 0), "stream wait");
     const bool use_graph = prm.profile == 0 && prm.no_graph == 0;
@@ -1148,6 +1209,9 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     const Layout L = layout_for(n, G);
     DeviceBuf ws;
     ck(hipMalloc(&ws.p, L.total), "workspace malloc");
+    // every exit from here on, failures included, first waits for the kernels
+    // still queued on s (a speculative round, a BFS batch), then frees ws
+    DrainOnExit drain{s};
     st.solver_workspace_bytes = L.total;
     char* base = static_cast<char*>(ws.p);
     Ctl* ctl = reinterpret_cast<Ctl*>(base + L.ctl);
@@ -1164,12 +1228,10 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     unsigned long long* gflow = kB ? reinterpret_cast<unsigned long long*>(base + L.gflow) : nullptr;
     uint8_t* galive = kB ? reinterpret_cast<uint8_t*>(base + L.galive) : nullptr;
 
-    HostBuf hctl_buf;
-    ck(hipHostMalloc(&hctl_buf.p, 5 * sizeof(Ctl), hipHostMallocDefault), "pinned ctl");
-    Ctl* hctl = static_cast<Ctl*>(hctl_buf.p);
+    Ctl* hctl = static_cast<Ctl*>(lease.r.hctl);
     Ctl* snap = hctl + 1;   // two pipelined snapshots of the discharge loop
     Ctl* bsnap = hctl + 3;  // two of the BFS level batches
-    Event bfs_ev[2];
+    EvRef bfs_ev[2] = {{lease.r.ev[2]}, {lease.r.ev[3]}};
 
     KernelTimer* ktp = nullptr;
     auto read_ctl = [&](const char* where) {
@@ -1186,7 +1248,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         }
     };
 
-    Event ev_t0, ev_t1, ev_b0, ev_b1;
+    EvRef ev_t0{lease.r.ev[4]}, ev_t1{lease.r.ev[5]}, ev_b0{lease.r.ev[6]}, ev_b1{lease.r.ev[7]};
     double bfs_ms = 0.0;
     // Cost guard of the adaptive relabel trigger: a global relabel may run
     // once the time since the last one reaches kRelabelCost times that one's
@@ -1386,9 +1448,9 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     struct Pending {
         int slot;
         uint64_t pos_after;
-        Event* ev;
+        EvRef* ev;
     };
-    Event snap_ev[2];
+    EvRef snap_ev[2] = {{lease.r.ev[8]}, {lease.r.ev[9]}};
     int next_slot = 0;
     auto snapshot = [&]() -> Pending {
         const int sl = next_slot;
@@ -1434,10 +1496,20 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     };
 
     const auto solve0 = Clock::now();
+    // the wall-clock safety limit (gpu_push_relabel_fast.cu:538-544), checked
+    // before every macro round the host decides on; the kernels still queued
+    // finish before the failure unwinds (DrainOnExit)
+    auto check_time = [&]() {
+        const double el = std::chrono::duration<double>(Clock::now() - solve0).count();
+        if (el > time_limit) {
+            std::ostringstream oss;
+            oss << kPrefix << "solve exceeded " << time_limit << "s safety limit";
+            fail(WTV_ELIMIT, oss.str());
+        }
+    };
     while (active) {
         if (st.rounds >= max_rounds) fail(WTV_ELIMIT, std::string(kPrefix) + "max macro rounds exceeded");
-        const double el = std::chrono::duration<double>(Clock::now() - solve0).count();
-        if (el > time_limit) fail(WTV_ELIMIT, std::string(kPrefix) + "solve exceeded 240s safety limit");
+        check_time();
         if (active <= tail_cap) {
             // ---- one workgroup, sub-rounds back to back
             unsigned long long budget = ~0ull;
@@ -1513,6 +1585,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
                 break;
             }
             if (!spec) fail(WTV_ELIMIT, std::string(kPrefix) + "max macro rounds exceeded");
+            check_time();
             ++st.rounds;
             const bool by_period = macro_period > 0 && (st.rounds % static_cast<uint64_t>(macro_period)) == 0ull;
             if (by_period || work_trigger()) {

@@ -68,7 +68,7 @@ class WtvServerConfig(C.Structure):
 
 class WtvRequest(C.Structure):
     _fields_ = [("n", C.c_uint32), ("d_nbr", C.c_void_p), ("d_res", C.c_void_p), ("d_term", C.c_void_p),
-                ("d_partition", C.c_void_p)]
+                ("d_partition", C.c_void_p), ("stream", C.c_void_p)]
 
 
 class WtvResult(C.Structure):
@@ -220,8 +220,10 @@ class Server:
         if not self._h:
             raise WtvError(WTV_EINVAL, lib().wtv_last_error().decode())
 
-    def submit(self, n, d_nbr, d_res, d_term, d_partition):
-        r = WtvRequest(n, d_nbr, d_res, d_term, d_partition)
+    def submit(self, n, d_nbr, d_res, d_term, d_partition, stream=None):
+        """stream: the hipStream_t (int) the buffers were written on; the solve is
+        ordered after the work queued there now.  None: the buffers are complete."""
+        r = WtvRequest(n, d_nbr, d_res, d_term, d_partition, stream)
         t = C.c_uint64(0)
         check(lib().wtv_server_submit(self._h, C.byref(r), C.byref(t)))
         return t.value
