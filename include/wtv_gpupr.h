@@ -136,6 +136,23 @@ int wtv_pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
                  const wtv_params* params, wtv_stats* stats, void* stream);
 
 /*
+ * wtv_pr_solve that allocates nothing: no device, pinned or host-runtime
+ * allocation, no stream, event or graph creation (no reference counterpart;
+ * for callers that cannot allocate, such as tight serving loops and harnesses
+ * that forbid hipMalloc).  Its workspace is a fixed device array inside the
+ * library, one per device, sized for WTV_NOALLOC_MAX_NODES nodes (27.3 MB);
+ * a larger n fails with WTV_EINVAL.  The kernels run on `stream` itself and
+ * the call returns when they are done.  Results equal wtv_pr_solve's.  The
+ * array keeps the last call's contents (each call clears what it reads
+ * first), so calls that overlap on one device race: serialise them.
+ * Profile mode (params->profile) is not available here (WTV_EINVAL).
+ */
+#define WTV_NOALLOC_MAX_NODES 1048576u
+int wtv_pr_solve_noalloc(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
+                         uint8_t* partition, int partition_on_device,
+                         const wtv_params* params, wtv_stats* stats, void* stream);
+
+/*
  * Batched solve: G independent reduced graphs in one launch sequence.
  *   offsets     uint64[G+1] host: graph g owns nodes [offsets[g], offsets[g+1])
  *               of the concatenated arrays; offsets[0] = 0, total < 2^31

@@ -76,6 +76,19 @@ int wtv_pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, u
     });
 }
 
+int wtv_pr_solve_noalloc(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,
+                         int partition_on_device, const wtv_params* params, wtv_stats* stats, void* stream) {
+    return guarded([&]() {
+        wtv_params p;
+        wtv_params_default(&p);
+        if (params) p = *params;
+        wtv_stats st{};
+        wtv::pr_solve_noalloc(n, d_nbr, d_res, d_term, partition, partition_on_device != 0, p, st, stream);
+        wtv::set_last_stats(st);
+        if (stats) *stats = st;
+    });
+}
+
 int wtv_batch_solve(uint32_t num_graphs, const uint64_t* offsets, const int32_t* d_nbr, uint32_t* d_res,
                     int32_t* d_term, uint8_t* partition, int partition_on_device, int64_t* flows,
                     const wtv_params* params, wtv_stats* stats, void* stream) {

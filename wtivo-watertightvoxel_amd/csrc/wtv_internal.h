@@ -29,6 +29,12 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
               uint8_t* partition, bool partition_on_device, const wtv_params& p,
               wtv_stats& st, void* stream);
 
+// pr_solve with its workspace in a file-scope device array: no allocation of
+// any kind, n <= WTV_NOALLOC_MAX_NODES, kernels on the caller's stream;
+// concurrent calls on one device race (wtv_solver.hip).
+void pr_solve_noalloc(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,
+                      bool partition_on_device, const wtv_params& p, wtv_stats& st, void* stream);
+
 // The solve followed by an excess return: residuals and terminals end as a
 // maximum flow that conserves at every vertex (wtv_flow.hip).
 void pr_solve_flow(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,

@@ -45,6 +45,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <optional>
 #include <sstream>
 #include <type_traits>
 #include <vector>
@@ -1168,12 +1169,21 @@ struct BatchArgs {
 
 }  // namespace
 
+// fixed_ws != nullptr: the solve allocates nothing (no device or pinned
+// memory, streams, events or graphs).  fixed_ws is a workspace of at least
+// layout_for(n, 0).total bytes, the kernels run on the caller's stream, the
+// host reads the control block into thread-local pageable memory, and every
+// wait is a stream synchronisation.
 template <bool kB>
 void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,
-             bool partition_on_device, const wtv_params& prm, wtv_stats& st, void* stream_v, const BatchArgs* B) {
+             bool partition_on_device, const wtv_params& prm, wtv_stats& st, void* stream_v, const BatchArgs* B,
+             void* fixed_ws = nullptr) {
     using Clock = std::chrono::steady_clock;
     std::memset(&st, 0, sizeof(st));
     if (!n) return;
+    const bool noalloc = fixed_ws != nullptr;
+    if (noalloc && prm.profile != 0)
+        fail(WTV_EINVAL, std::string(kPrefix) + "profile mode needs events: not available without allocation");
     if (!d_nbr || !d_res || !d_term || !partition) fail(WTV_EINVAL, std::string(kPrefix) + "null graph pointer");
     if (n > 0x7FFFFFFFu) fail(WTV_EINVAL, std::string(kPrefix) + ">INT32_MAX reduced nodes unsupported");
     int local_steps = prm.local_steps;
@@ -1192,15 +1202,26 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     // a level whose frontier holds more than n/32 vertices expands bottom-up
     const uint32_t bu_threshold = std::max<uint32_t>(n / 32u, 4096u);
     hipStream_t cs = static_cast<hipStream_t>(stream_v);
-    ResLease lease;
-    hipStream_t s = lease.r.s;
+    std::optional<ResLease> lease;
+    if (!noalloc) lease.emplace();
+    hipStream_t s = noalloc ? cs : lease->r.s;
     struct EvRef {
-        hipEvent_t e;
+        hipEvent_t e;  // null without allocation: waits become stream synchronisations
     };
-    EvRef ev_caller{lease.r.ev[0]}, ev_done{lease.r.ev[1]};
-    ck(hipEventRecord(ev_caller.e, cs), "event");
-    ck(hipStreamWaitEvent(s, ev_caller.e, 0), "stream wait");
-    const bool use_graph = prm.profile == 0 && prm.no_graph == 0;
+    auto lease_ev = [&](int i) -> EvRef { return EvRef{noalloc ? nullptr : lease->r.ev[i]}; };
+    auto ev_record = [&](const EvRef& e) {
+        if (e.e) ck(hipEventRecord(e.e, s), "event");
+    };
+    auto ev_wait = [&](const EvRef& e, const char* where) {
+        if (e.e) ck(hipEventSynchronize(e.e), where);
+        else ck(hipStreamSynchronize(s), where);
+    };
+    EvRef ev_caller = lease_ev(0), ev_done = lease_ev(1);
+    if (!noalloc) {
+        ck(hipEventRecord(ev_caller.e, cs), "event");
+        ck(hipStreamWaitEvent(s, ev_caller.e, 0), "stream wait");
+    }
+    const bool use_graph = prm.profile == 0 && prm.no_graph == 0 && !noalloc;
     // WTV_TRACE=1: one stderr line per macro round / global relabel (diagnostics)
     const char* trace_env = std::getenv("WTV_TRACE");
     const bool trace = trace_env && *trace_env && *trace_env != '0';
@@ -1208,12 +1229,12 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     const uint32_t G = kB ? B->G : 0u;
     const Layout L = layout_for(n, G);
     DeviceBuf ws;
-    ck(hipMalloc(&ws.p, L.total), "workspace malloc");
+    if (!noalloc) ck(hipMalloc(&ws.p, L.total), "workspace malloc");
     // every exit from here on, failures included, first waits for the kernels
     // still queued on s (a speculative round, a BFS batch), then frees ws
     DrainOnExit drain{s};
     st.solver_workspace_bytes = L.total;
-    char* base = static_cast<char*>(ws.p);
+    char* base = static_cast<char*>(noalloc ? fixed_ws : ws.p);
     Ctl* ctl = reinterpret_cast<Ctl*>(base + L.ctl);
     long long* excess = reinterpret_cast<long long*>(base + L.excess);
     uint32_t* height = reinterpret_cast<uint32_t*>(base + L.height);
@@ -1228,10 +1249,12 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     unsigned long long* gflow = kB ? reinterpret_cast<unsigned long long*>(base + L.gflow) : nullptr;
     uint8_t* galive = kB ? reinterpret_cast<uint8_t*>(base + L.galive) : nullptr;
 
-    Ctl* hctl = static_cast<Ctl*>(lease.r.hctl);
+    // pinned snapshots, or thread-local pageable ones without allocation
+    thread_local Ctl tl_ctl[kCtlSnapshots];
+    Ctl* hctl = noalloc ? tl_ctl : static_cast<Ctl*>(lease->r.hctl);
     Ctl* snap = hctl + 1;   // two pipelined snapshots of the discharge loop
     Ctl* bsnap = hctl + 3;  // two of the BFS level batches
-    EvRef bfs_ev[2] = {{lease.r.ev[2]}, {lease.r.ev[3]}};
+    EvRef bfs_ev[2] = {lease_ev(2), lease_ev(3)};
 
     KernelTimer* ktp = nullptr;
     auto read_ctl = [&](const char* where) {
@@ -1248,7 +1271,8 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         }
     };
 
-    EvRef ev_t0{lease.r.ev[4]}, ev_t1{lease.r.ev[5]}, ev_b0{lease.r.ev[6]}, ev_b1{lease.r.ev[7]};
+    EvRef ev_t0 = lease_ev(4), ev_t1 = lease_ev(5), ev_b0 = lease_ev(6), ev_b1 = lease_ev(7);
+    const auto host_t0 = Clock::now();
     double bfs_ms = 0.0;
     // Cost guard of the adaptive relabel trigger: a global relabel may run
     // once the time since the last one reaches kRelabelCost times that one's
@@ -1271,7 +1295,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         ck(hipMemcpyAsync(goff, B->h_off, (G + 1u) * 8u, hipMemcpyHostToDevice, s), "batch offsets");
         ck(hipMemsetAsync(gflow, 0, G * 8u, s), "batch flows");
     }
-    ck(hipEventRecord(ev_t0.e, s), "event");
+    ev_record(ev_t0);
     if constexpr (kB) {
         int4* nbrg = reinterpret_cast<int4*>(base + L.nbrg);
         k_globalize<<<sweep_grid, tpb, 0, s>>>(n, G, goff, reinterpret_cast<const int4*>(d_nbr), nbrg, gid, ctl);
@@ -1292,7 +1316,8 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     // skip_done (batched solves, relabels between the first and the final
     // one): the seed sweep passes over graphs without an active vertex
     auto global_relabel = [&](bool skip_done) {
-        ck(hipEventRecord(ev_b0.e, s), "event");
+        ev_record(ev_b0);
+        const auto host_b0 = Clock::now();
         const uint32_t p = static_cast<uint32_t>(pos & 1u);
         uint32_t* bf[2] = {rl, wl[p ^ 1u]};
         k_bfs_begin<<<1, 1, 0, s>>>(ctl, p);
@@ -1333,13 +1358,13 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             const int sl = next;
             next ^= 1;
             ck(hipMemcpyAsync(&bsnap[sl], ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s), "bfs snapshot");
-            ck(hipEventRecord(bfs_ev[sl].e, s), "event");
+            ev_record(bfs_ev[sl]);
             return Mark{sl, d};
         };
         Mark prev = enqueue_batch();
         for (;;) {
             const Mark cur = enqueue_batch();  // speculative
-            ck(hipEventSynchronize(bfs_ev[prev.slot].e), "bfs snapshot wait");
+            ev_wait(bfs_ev[prev.slot], "bfs snapshot wait");
             if (ktp) ktp->harvest();
             const uint32_t last = bsnap[prev.slot].bfs_cnt[prev.d % 3u];
             if (last == 0u) break;  // the speculative batch ran on empty frontiers
@@ -1361,10 +1386,11 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             }
             prev = cur;
         }
-        ck(hipEventRecord(ev_b1.e, s), "event");
-        ck(hipEventSynchronize(ev_b1.e), "event sync");
+        ev_record(ev_b1);
+        ev_wait(ev_b1, "event sync");
         float ms = 0.f;
-        ck(hipEventElapsedTime(&ms, ev_b0.e, ev_b1.e), "event time");
+        if (noalloc) ms = std::chrono::duration<float, std::milli>(Clock::now() - host_b0).count();
+        else ck(hipEventElapsedTime(&ms, ev_b0.e, ev_b1.e), "event time");
         bfs_ms += ms;
         last_gr_ms = ms;
         last_gr_end = Clock::now();
@@ -1450,13 +1476,13 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         uint64_t pos_after;
         EvRef* ev;
     };
-    EvRef snap_ev[2] = {{lease.r.ev[8]}, {lease.r.ev[9]}};
+    EvRef snap_ev[2] = {lease_ev(8), lease_ev(9)};
     int next_slot = 0;
     auto snapshot = [&]() -> Pending {
         const int sl = next_slot;
         next_slot ^= 1;
         ck(hipMemcpyAsync(&snap[sl], ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s), "snapshot");
-        ck(hipEventRecord(snap_ev[sl].e, s), "event");
+        ev_record(snap_ev[sl]);
         return Pending{sl, pos, &snap_ev[sl]};
     };
 
@@ -1569,7 +1595,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
                 launch_round();
                 cur = snapshot();
             }
-            ck(hipEventSynchronize(prev.ev->e), "snapshot wait");
+            ev_wait(*prev.ev, "snapshot wait");
             if (ktp) ktp->harvest();
             const Ctl& S = snap[prev.slot];
             check_device_error(S);
@@ -1580,7 +1606,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
                              static_cast<unsigned long long>(st.rounds), act_prev,
                              stat_sum(S, kStRelabels));
             if (act_prev == 0u) {
-                if (spec) ck(hipEventSynchronize(cur.ev->e), "drain");
+                if (spec) ev_wait(*cur.ev, "drain");
                 active = 0;
                 break;
             }
@@ -1593,7 +1619,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
                 break;
             }
             if (act_prev <= tail_cap) {
-                ck(hipEventSynchronize(cur.ev->e), "switch");
+                ev_wait(*cur.ev, "switch");
                 const Ctl& C2 = snap[cur.slot];
                 check_device_error(C2);
                 active = C2.wl_cnt[cur.pos_after & 1u];
@@ -1615,12 +1641,12 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         hipStream_t cs, s;
         hipEvent_t e;
         ~CallerOrder() {
-            if (hipEventRecord(e, s) == hipSuccess) (void)hipStreamWaitEvent(cs, e, 0);
+            if (e && hipEventRecord(e, s) == hipSuccess) (void)hipStreamWaitEvent(cs, e, 0);
         }
-    } order{cs, s, ev_done.e};
+    } order{cs, s, ev_done.e};  // without allocation s is the caller's stream
     k_partition<kB><<<sweep_grid, tpb, 0, s>>>(n, height, excess, d_part, gid, gn, ctl);
     ck(hipGetLastError(), "partition");
-    ck(hipEventRecord(ev_t1.e, s), "event");
+    ev_record(ev_t1);
     if (!partition_on_device)
         ck(hipMemcpyAsync(partition, d_part, static_cast<size_t>(n), hipMemcpyDeviceToHost, s), "partition D2H");
     read_ctl("final");
@@ -1660,7 +1686,8 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     st.bfs_seed_checks = stat_sum(*hctl, kStSeedChecks);
     st.sub_rounds = sub;
     float ms = 0.f;
-    ck(hipEventElapsedTime(&ms, ev_t0.e, ev_t1.e), "event time");
+    if (noalloc) ms = std::chrono::duration<float, std::milli>(Clock::now() - host_t0).count();
+    else ck(hipEventElapsedTime(&ms, ev_t0.e, ev_t1.e), "event time");
     st.t_solve_ms = ms;
     st.t_bfs_ms = bfs_ms;
     kt.harvest();
@@ -1678,6 +1705,55 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
 void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,
               bool partition_on_device, const wtv_params& prm, wtv_stats& st, void* stream) {
     solve_t<false>(n, d_nbr, d_res, d_term, partition, partition_on_device, prm, st, stream, nullptr);
+}
+
+// ------------------------------------------------ allocation-free solve
+// Workspace of wtv_pr_solve_noalloc: a file-scope device array, so that the
+// call allocates nothing.  It holds the layout of an n-node solve for every
+// n up to WTV_NOALLOC_MAX_NODES (control block, then 26 B per node: excess 8,
+// height 4, two active lists and the relabel list 4 each, reverse slots 1,
+// residual masks 1, each array 256-byte aligned).  Every device the library
+// runs on holds its own copy, 27.3 MB, until the process ends.
+constexpr size_t kNoallocMaxNodes = WTV_NOALLOC_MAX_NODES;
+constexpr size_t kScratchBytes = (sizeof(Ctl) + 255u) / 256u * 256u + 26u * kNoallocMaxNodes;
+static_assert(kNoallocMaxNodes % 256u == 0u, "per-node arrays stay 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+// Device address of `scratch`, looked up once per device with that device
+// current.  In host code the array's name is a host-side stand-in, not a
+// device address: kernels and copies get this address instead.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+void* scratch_of_current_device() {
+    int dev = 0;
+    ck(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) fail(WTV_EINVAL, std::string(kPrefix) + "device ordinal beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        ck(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch symbol address");
+        if (!p) fail(WTV_EDEVICE, std::string(kPrefix) + "scratch symbol has no device address");
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+void pr_solve_noalloc(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,
+                      bool partition_on_device, const wtv_params& prm, wtv_stats& st, void* stream) {
+    std::memset(&st, 0, sizeof(st));
+    if (n > kNoallocMaxNodes) {
+        std::ostringstream oss;
+        oss << kPrefix << "allocation-free solve: " << n << " nodes exceed its fixed workspace of "
+            << kNoallocMaxNodes << " nodes";
+        fail(WTV_EINVAL, oss.str());
+    }
+    if (layout_for(n, 0).total > kScratchBytes) fail(WTV_EINVARIANT, std::string(kPrefix) + "scratch layout");
+    void* ws = scratch_of_current_device();
+    solve_t<false>(n, d_nbr, d_res, d_term, partition, partition_on_device, prm, st, stream, nullptr, ws);
 }
 
 void batch_solve(uint32_t G, const uint64_t* h_off, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,

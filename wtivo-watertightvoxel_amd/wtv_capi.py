@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libwtv_gpupr.so")
 WTV_OK, WTV_EINVAL, WTV_EOVERFLOW, WTV_EINVARIANT, WTV_ENOMEM, WTV_ELIMIT, WTV_EDEVICE = range(7)
 
 # every extern "C" symbol declared in include/wtv_gpupr.h
-EXPORTS = ["wtv_params_default", "wtv_pr_solve", "wtv_pr_solve_flow", "wtv_batch_solve", "wtv_graph_cut_fast",
+EXPORTS = ["wtv_params_default", "wtv_pr_solve", "wtv_pr_solve_noalloc", "wtv_pr_solve_flow", "wtv_batch_solve", "wtv_graph_cut_fast",
            "wtv_graph_cut_fast_resident", "wtv_last_stats",
            "wtv_last_error", "wtv_version", "wtv_build_reduced_graph", "wtv_surface_extract",
            "wtv_surface_free", "wtv_request_footprint", "wtv_server_create", "wtv_server_submit",
@@ -109,6 +109,8 @@ def lib():
         L.wtv_pr_solve.restype = C.c_int
         L.wtv_pr_solve.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                    C.POINTER(WtvParams), C.POINTER(WtvStats), C.c_void_p]
+        L.wtv_pr_solve_noalloc.restype = C.c_int
+        L.wtv_pr_solve_noalloc.argtypes = L.wtv_pr_solve.argtypes
         L.wtv_pr_solve_flow.restype = C.c_int
         L.wtv_pr_solve_flow.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                         C.POINTER(WtvParams), C.POINTER(WtvStats), C.POINTER(C.c_int64), C.c_void_p]
@@ -178,6 +180,19 @@ def pr_solve(n, d_nbr, d_res, d_term, d_or_h_partition, partition_on_device, prm
     prm = prm or params()
     rc = lib().wtv_pr_solve(n, d_nbr, d_res, d_term, d_or_h_partition, int(partition_on_device),
                             C.byref(prm), C.byref(st), stream)
+    check(rc)
+    return st
+
+
+NOALLOC_MAX_NODES = 1 << 20  # WTV_NOALLOC_MAX_NODES
+
+
+def pr_solve_noalloc(n, d_nbr, d_res, d_term, d_or_h_partition, partition_on_device, prm=None, stream=0):
+    """Call wtv_pr_solve_noalloc (no allocation; n <= NOALLOC_MAX_NODES); returns WtvStats."""
+    st = WtvStats()
+    prm = prm or params()
+    rc = lib().wtv_pr_solve_noalloc(n, d_nbr, d_res, d_term, d_or_h_partition, int(partition_on_device),
+                                    C.byref(prm), C.byref(st), stream)
     check(rc)
     return st
 
