@@ -5,6 +5,8 @@
 Outputs (git-ignored, shipped with the tree):
     wtivo-watertightvoxel_amd/libwtv_gpupr.so
     wtivo-watertightvoxel_amd/wtivo_gpupr<EXT_SUFFIX>
+    wtivo-watertightvoxel_amd/libwtv_delaunay.so  (host tetrahedraliser of the
+        benchmark and test meshes, not part of the solver)
 Objects go to wtivo-watertightvoxel_amd/_build/.
 """
 from __future__ import annotations
@@ -25,6 +27,9 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 
 LIB = os.path.join(HERE, "libwtv_gpupr.so")
+# the tetrahedraliser of the benchmark and test meshes: host C++, not part of
+# the solver library
+DELAUNAY_LIB = os.path.join(HERE, "libwtv_delaunay.so")
 EXT = os.path.join(HERE, "wtivo_gpupr" + sysconfig.get_config_var("EXT_SUFFIX"))
 
 HIP_SOURCES = ["wtv_solver.hip", "wtv_flow.hip", "wtv_graph_build.hip", "wtv_topology.hip", "wtv_udf.hip"]
@@ -78,6 +83,11 @@ def build(verbose: bool = False) -> dict:
           "-Wl,-soname,libwtv_gpupr.so"], verbose)
     os.replace(tmp, LIB)
 
+    tmp = DELAUNAY_LIB + ".tmp"
+    _run(["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-fPIC", "-shared", "-pthread", "-Wall", "-Wextra",
+          os.path.join(CSRC, "wtv_delaunay.cpp"), "-o", tmp], verbose)
+    os.replace(tmp, DELAUNAY_LIB)
+
     import pybind11
 
     py_inc = sysconfig.get_paths()["include"]
@@ -87,7 +97,7 @@ def build(verbose: bool = False) -> dict:
           os.path.join(CSRC, "pybind_wtivo_gpupr.cpp"), "-o", tmp,
           "-L", HERE, "-lwtv_gpupr", "-Wl,-rpath,$ORIGIN"], verbose)
     os.replace(tmp, EXT)
-    return {"lib": LIB, "ext": EXT, "seconds": time.time() - t0}
+    return {"lib": LIB, "ext": EXT, "delaunay": DELAUNAY_LIB, "seconds": time.time() - t0}
 
 
 if __name__ == "__main__":

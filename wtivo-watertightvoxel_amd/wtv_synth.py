@@ -590,16 +590,23 @@ def torus_surface_points(t: Torus, m: int, rng: np.random.Generator) -> np.ndarr
     """m points uniform by area on a torus surface.  The angle around the tube,
     phi, has density proportional to R + r cos(phi) (rejection sampling)."""
     a, u, w = _torus_frame(t)
-    phis = np.empty(0)
-    while phis.size < m:
-        cand = rng.uniform(0.0, 2.0 * np.pi, size=2 * (m - phis.size) + 64)
-        keep = rng.uniform(size=cand.size) * (t.R + t.r) < t.R + t.r * np.cos(cand)
-        phis = np.concatenate([phis, cand[keep]])
-    phi = phis[:m]
-    th = rng.uniform(0.0, 2.0 * np.pi, size=m)
-    rad = t.R + t.r * np.cos(phi)
-    return (np.asarray(t.c, np.float64)[None, :] + (rad * np.cos(th))[:, None] * u[None, :]
-            + (rad * np.sin(th))[:, None] * w[None, :] + (t.r * np.sin(phi))[:, None] * a[None, :])
+    # numpy draws the numbers, torch (vectorised on the host) does the trigonometry
+    phis = []
+    have = 0
+    while have < m:
+        cand = torch.from_numpy(rng.uniform(0.0, 2.0 * np.pi, size=2 * (m - have) + 64))
+        acc = torch.from_numpy(rng.uniform(size=cand.numel()))
+        keep = acc * (t.R + t.r) < t.R + t.r * torch.cos(cand)
+        phis.append(cand[keep])
+        have += int(phis[-1].numel())
+    phi = torch.cat(phis)[:m]
+    th = torch.from_numpy(rng.uniform(0.0, 2.0 * np.pi, size=m))
+    rad = t.R + t.r * torch.cos(phi)
+    out = torch.tensor(t.c, dtype=torch.float64)[None, :].repeat(m, 1)
+    out += (rad * torch.cos(th))[:, None] * torch.from_numpy(u)[None, :]
+    out += (rad * torch.sin(th))[:, None] * torch.from_numpy(w)[None, :]
+    out += (t.r * torch.sin(phi))[:, None] * torch.from_numpy(a)[None, :]
+    return out.numpy()
 
 
 def genus3_point_cloud(npts: int, seed: int, res: float | None = None, surface_frac: float = 0.95):
@@ -637,6 +644,73 @@ def genus3_point_cloud(npts: int, seed: int, res: float | None = None, surface_f
     return np.concatenate([S, U]), shape, res
 
 
+GRID_BITS = 21  # coordinates of tetrahedralised points: multiples of 2^-21 in [0, 1)
+
+
+def quantise_points(P: np.ndarray):
+    """Points of the unit box snapped down to the 2^-21 grid: (integer coordinates
+    int32 [n,3], the same points as float64 [n,3]).  Both forms are exact (and
+    exact in fp32 too), so a tetrahedralisation of the integers is one of the
+    float64 points."""
+    scale = float(1 << GRID_BITS)
+    Q = np.clip(np.floor(np.asarray(P, np.float64) * scale), 0, scale - 1).astype(np.int32)
+    return Q, Q.astype(np.float64) / scale
+
+
+_DT_LIB = None
+
+
+def _delaunay_lib():
+    global _DT_LIB
+    if _DT_LIB is None:
+        import ctypes as C
+        import os
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libwtv_delaunay.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"libwtv_delaunay.so is not built ({path}); run __graft_entry__.build()")
+        L = C.CDLL(path)
+        L.wtvd_triangulate.restype = C.c_void_p
+        L.wtvd_triangulate.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.POINTER(C.c_int64),
+                                       C.POINTER(C.c_int64), C.POINTER(C.c_int)]
+        L.wtvd_export.restype = None
+        L.wtvd_export.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.wtvd_check.restype = C.c_int
+        L.wtvd_check.argtypes = [C.c_void_p]
+        L.wtvd_free.restype = None
+        L.wtvd_free.argtypes = [C.c_void_p]
+        _DT_LIB = L
+    return _DT_LIB
+
+
+def native_delaunay(Q: np.ndarray, threads: int | None = None, check: bool = False):
+    """Delaunay tetrahedralisation of integer points (int32 [n,3] in [0, 2^21),
+    as quantise_points returns) by csrc/wtv_delaunay.cpp: (tets int32 [N,4],
+    nbrs int32 [N,4], repeated points skipped).  Tets come in a canonical order
+    (by their smallest vertex in Morton order); slot s holds the neighbour
+    across the face opposite local vertex 3-s, -1 on the hull."""
+    import ctypes as C
+    import os
+    L = _delaunay_lib()
+    Q = np.ascontiguousarray(Q, dtype=np.int32)
+    if threads is None:
+        threads = min(16, os.cpu_count() or 1)
+    nt, rep, st = C.c_int64(0), C.c_int64(0), C.c_int(0)
+    h = L.wtvd_triangulate(Q.ctypes.data, Q.shape[0], int(threads), C.byref(nt), C.byref(rep), C.byref(st))
+    if not h:
+        raise RuntimeError(f"tetrahedralisation failed (status {st.value})")
+    try:
+        if check:
+            rc = L.wtvd_check(h)
+            if rc != 0:
+                raise RuntimeError(f"tetrahedralisation check failed ({rc})")
+        tets = np.empty((nt.value, 4), np.int32)
+        nbrs = np.empty((nt.value, 4), np.int32)
+        L.wtvd_export(h, tets.ctypes.data, nbrs.ctypes.data)
+    finally:
+        L.wtvd_free(h)
+    return tets, nbrs, int(rep.value)
+
+
 def _morton_key(P: np.ndarray, bits: int = 21) -> np.ndarray:
     lo = P.min(0)
     span = max(float((P.max(0) - lo).max()), 1e-300)
@@ -671,15 +745,26 @@ def delaunay_mesh(points: np.ndarray, morton: bool = True) -> TetMesh:
                    nbrs.astype(np.int32))
 
 
+def native_delaunay_mesh(points: np.ndarray, threads: int | None = None) -> TetMesh:
+    """Delaunay mesh of `points` (unit box) snapped to the 2^-21 grid, by the
+    native tetrahedraliser; V holds the snapped points, tets come in canonical
+    order (Morton order of their smallest vertex)."""
+    Q, V = quantise_points(points)
+    tets, nbrs, _ = native_delaunay(Q, threads)
+    return TetMesh(0, V, tets, nbrs)
+
+
 def delaunay_genus3_graph(npts: int, seed: int, lam: float = 10.0, flip: float = 0.02, device=None,
-                          res: float | None = None):
+                          res: float | None = None, method: str = "qhull", threads: int | None = None):
     """Soft-seed graph of the Delaunay mesh of genus3_point_cloud(npts, seed):
     labels from the shape SDF at tet centroids with `flip` seeded flips,
     terminal lam * volume, facet area * exp(-|UDF|/h) with h one voxel (1/res).
+    method "qhull": scipy's Qhull, Morton-ordered tets; "native": the native
+    tetrahedraliser on the points snapped to the 2^-21 grid, canonical order.
     Returns (graph dict as soft_graph_t, mesh, meta)."""
     dev = _dev(device)
     P, shape, res = genus3_point_cloud(npts, seed, res)
-    mesh = delaunay_mesh(P)
+    mesh = native_delaunay_mesh(P, threads) if method == "native" else delaunay_mesh(P)
     V = torch.from_numpy(mesh.V).to(dev)
     T = torch.from_numpy(mesh.tets).to(dev)
     Nb = torch.from_numpy(mesh.nbrs).to(dev)
