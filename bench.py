@@ -1,26 +1,36 @@
 #!/usr/bin/env python3
 """Benchmark: one graph-cut solve of the ~78M-tet production-shape graph on one MI355X.
 
-Workload (the largest single-GPU configuration, "cut solve s @78M tets"):
-  * mesh   : jittered Kuhn 6-tet subdivision of a 235^3 grid = 77,867,250 tets
-             (stand-in for the 12M-point Delaunay mesh, which cannot be
-             generated on the GPU host within a bench run), int32 4-slot layout;
-  * labels : genus-3 solid (three fused tori), inside = 0, 2% seeded flips;
+Workload ("cut solve s @78M tets"; the largest single-GPU configuration):
+  * points : 12,000,000 in the unit box, 95% uniform by area on a seeded
+             genus-3 surface (three fused tori) with N(0, 0.5 voxel) normal
+             noise at resolution 1536, 5% uniform, snapped to the 2^-21 grid;
+  * mesh   : their 3D Delaunay tetrahedralisation (77.5M tets) by the native
+             tetrahedraliser (wtivo-watertightvoxel_amd/csrc/wtv_delaunay.cpp,
+             equal to Qhull's tets), int32 4-slot layout, canonical tet order;
+  * labels : the genus-3 solid's SDF at tet centroids, inside = 0, 2% flips;
   * caps   : soft seeds, lambda_fill = 10: terminal = 10*vol on the labelled
-             side, facet = area * exp(-|UDF(facet centroid)|/h), h = 1 cell, fp32,
+             side, facet = area * exp(-|UDF(facet centroid)|/h), h = 1 voxel, fp32,
              quantised with one power-of-two scale (integer arithmetic, as the
              reference solver's uint32/int32/int64 state).
-A step restores the pristine residuals/terminals on the device (1.56 GB
-device copy) and runs wtv_pr_solve through the C-ABI with every input already
-resident in HBM; the partition stays on the device.
+--workload kuhn235 measures the former stand-in instead (Kuhn 6-tet subdivision
+of a 235^3 grid, 77,867,250 tets, same labels and capacity model).
+A step restores the pristine residuals/terminals on the device and runs
+wtv_pr_solve through the C-ABI with every input already resident in HBM; the
+partition stays on the device.
 
 Checks every run: all steps return the same flow and labels; the flow equals
-the integer capacity of the returned cut (max-flow/min-cut certificate);
-and (unless --no-cpu) the CPU oracle's Boykov-Kolmogorov solve of the same
-graph returns the same flow and labels.
+the integer capacity of the returned cut (max-flow/min-cut certificate).  The
+CPU baseline is Boykov-Kolmogorov (oracle/wtv_oracle.c) on a bounded sample of
+the same workload: the Delaunay mesh of 2M points of the same generator
+(12.9M tets), whose flow and labels must equal the GPU's on that sample; its
+time is scaled to the headline's tet count.  (BK on the full 77.5M-tet graph
+takes ~100 s on the GPU host; the GPU tests check it at full size.)
 
 Further legs, each reported in its own object of the line:
-  * graph_cut_fast: the drop-in call (build + solve + scatter) on the same mesh
+  * kuhn_standin: the Kuhn 235^3 graph of the earlier headline, timed and
+    checked against BK at full size;
+  * graph_cut_fast: the drop-in call (build + solve + scatter) on the Kuhn mesh
     with the reference capacity model, certificate-checked, with the CPU
     restatement of the build timed and compared array for array;
   * flow_mode: the maximum flow with conservation (excess returned to the
@@ -30,9 +40,8 @@ Further legs, each reported in its own object of the line:
     against the oracle;
   * surface: surface extraction of the label-0 region, checked against the
     oracle, beside the reference's chunked topology download;
-  * shards: the headline graph's 8 z-slabs solved one by one, each the block
+  * shards: the Kuhn graph's 8 z- and x-slabs solved one by one, each the block
     one GPU of an 8-way partition would hold (certificate-checked);
-  * delaunay: the Qhull mesh of a 400k-point genus-3 cloud, checked against BK;
   * batched: 256 uneven graphs, looped vs one autotuned batched solve;
   * serving: Poisson-arriving requests into the native server, p50/p99;
   * atomics: probe rates (scattered, coalesced, one address) and rocprof counters;
@@ -65,6 +74,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=("delaunay", "kuhn235"), default="delaunay",
+                    help="headline graph: the 12M-point Delaunay mesh, or the Kuhn 235^3 grid")
+    ap.add_argument("--points", type=int, default=12_000_000, help="Delaunay headline: points")
+    ap.add_argument("--cpu-sample-points", type=int, default=2_000_000,
+                    help="Delaunay headline: points of the bounded sample the CPU baseline solves")
     ap.add_argument("--k", type=int, default=235, help="Kuhn grid size (235 -> 77.87M tets)")
     ap.add_argument("--seed", type=int, default=20260001)
     ap.add_argument("--period", type=int, default=1024)
@@ -77,7 +91,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=96, help="serving leg: number of requests (0 = skip)")
     ap.add_argument("--serve-load", type=float, default=0.8, help="offered load vs looped-solve capacity")
     ap.add_argument("--serve-slots", type=int, default=4, help="serving leg: concurrent batches (worker slots)")
-    ap.add_argument("--delaunay-points", type=int, default=400_000,
+    ap.add_argument("--no-standin-leg", action="store_true", help="skip the Kuhn 235^3 leg")
+    ap.add_argument("--delaunay-points", type=int, default=0,
                     help="Delaunay leg: genus-3 point-cloud size (0 = skip)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuned re-runs of the single-graph legs")
@@ -130,22 +145,18 @@ def main():
 
     # ---------------------------------------------------------- workload
     t = time.time()
-    k = a.k
-    V, T, Nb = S.kuhn_mesh_t(k, seed=a.seed, device=dev)
-    shape = S.genus3(a.seed)
-    sdf = S.tet_sdf_t(V, T, shape)
-    N = int(T.shape[0])
-    flips = torch.from_numpy(S.flip_mask(N, 0.02, a.seed + 1).astype(np.uint8)).to(dev)
-    labels = (sdf >= 0).to(torch.uint8) ^ flips
-    g = S.soft_graph_t(V, T, Nb, labels, shape, 10.0, 1.0 / k, rev_slots=S.KUHN_REV_SLOT)
-    del sdf, flips, V
+    if a.workload == "delaunay":
+        g, meta = delaunay_graph(a.points, a.seed, torch, np, S, dev)
+    else:
+        g, meta = kuhn_graph(a.k, a.seed, torch, np, S, dev)
+    N = int(meta["tets"])
     nbr, res0, term0 = g["nbr"], g["res"], g["term"]
     res = res0.clone()
     term = term0.clone()
     part = torch.empty(N, dtype=torch.uint8, device=dev)
     torch.cuda.synchronize()
     gen_s = time.time() - t
-    log(f"[bench] workload k={k} tets={N} inside={float((labels == 0).float().mean()):.4f} "
+    log(f"[bench] workload {meta['workload']}: tets={N} inside={meta['inside_frac']:.4f} "
         f"scale=2^{int(np.log2(g['scale']))} gen={gen_s:.1f}s")
 
     stream = torch.cuda.current_stream().cuda_stream
@@ -188,11 +199,6 @@ def main():
     flow_mode = None
     if not a.no_flow_leg:
         flow_mode = flow_leg(a, torch, np, W, N, nbr, res, term, res0, term0, part, part_h, prm, stream, raw_flow)
-
-    # -------------------------------------------------- one GPU's share of an 8-way partition
-    shards = None
-    if not a.no_shard_leg:
-        shards = shard_leg(a, torch, W, k, nbr, res0, term0, stream)
 
     # -------------------------------------------------- profiled solve
     prof = None
@@ -253,7 +259,9 @@ def main():
 
     # -------------------------------------------------- CPU baseline
     cpu = None
-    if not a.no_cpu:
+    if not a.no_cpu and a.workload == "delaunay":
+        cpu = cpu_sample_leg(a, torch, np, W, S, dev, stream, N)
+    elif not a.no_cpu:
         import oracle
         oracle.build()
         nbr_h = nbr.cpu().numpy()
@@ -290,6 +298,16 @@ def main():
                    "counters": counters}
 
     ceiling = copy_ceiling(torch)
+    del res, term, part, nbr, res0, term0, g
+    torch.cuda.empty_cache()
+    standin = None
+    shards = None
+    if not a.no_standin_leg and a.workload == "delaunay":
+        standin, shards = standin_leg(a, torch, np, W, S, dev, stream)
+    elif not a.no_shard_leg:
+        gk, _ = kuhn_graph(a.k, a.seed, torch, np, S, dev)
+        shards = shard_leg(a, torch, W, a.k, gk["nbr"], gk["res"], gk["term"], stream)
+        del gk
     if roofline:
         roofline["copy_ceiling"] = ceiling
         roofline["frac_of_copy"] = round(roofline["achieved"] / ceiling, 5)
@@ -316,7 +334,6 @@ def main():
 
     batched = None
     if a.batch_graphs > 0:
-        del res, term, part
         batched = batch_leg(a, torch, np, W, S, dev, stream)
 
     serving = None
@@ -337,12 +354,10 @@ def main():
         "vs_baseline": None,
         "dtype": "int64",
         "data": "synthetic",
-        "config": {"workload": f"kuhn{k}_genus3_softseed_lambda10", "tets": N, "free_nodes": n,
-                   "lambda_fill": 10.0, "h_cells": 1.0, "label_flips": 0.02, "seed": a.seed,
-                   "global_relabel_period": a.period, "local_steps": a.local_steps,
-                   "relabel_alpha": a.alpha if a.alpha != 0.0 else 2.0, "tail_cap": 512, "grid_wgs": 1024,
-                   "wg_size": 256,
-                   "stand_in": "Kuhn grid replaces the 12M-point Delaunay mesh"},
+        "config": dict(meta, **{"free_nodes": n, "lambda_fill": 10.0, "label_flips": 0.02, "seed": a.seed,
+                                "global_relabel_period": a.period, "local_steps": a.local_steps,
+                                "relabel_alpha": a.alpha if a.alpha != 0.0 else 2.0, "tail_cap": 512,
+                                "grid_wgs": 1024, "wg_size": 256}),
         "tets_per_s": round(N / value, 1),
         "autotuned": tuned,
         "t_solve_ms_device": round(s0["t_solve_ms"], 3),
@@ -366,6 +381,7 @@ def main():
         "udf_labels": udf,
         "surface": surface,
         "shards": shards,
+        "kuhn_standin": standin,
         "delaunay": delaunay,
         "batched": batched,
         "serving": serving,
@@ -478,6 +494,146 @@ def copy_ceiling(torch, nbytes=1 << 30, reps=3):
     gbs = 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
     del src, dst
     return round(gbs, 1)
+
+
+def kuhn_graph(k, seed, torch, np, S, dev):
+    """Soft-seed graph of the jittered Kuhn k^3 mesh (6k^3 tets) of the genus-3
+    solid: labels from the SDF at tet centroids with 2% flips, lambda 10,
+    h = 1 cell.  Returns (graph dict of soft_graph_t, meta)."""
+    V, T, Nb = S.kuhn_mesh_t(k, seed=seed, device=dev)
+    shape = S.genus3(seed)
+    sdf = S.tet_sdf_t(V, T, shape)
+    N = int(T.shape[0])
+    flips = torch.from_numpy(S.flip_mask(N, 0.02, seed + 1).astype(np.uint8)).to(dev)
+    labels = (sdf >= 0).to(torch.uint8) ^ flips
+    g = S.soft_graph_t(V, T, Nb, labels, shape, 10.0, 1.0 / k, rev_slots=S.KUHN_REV_SLOT)
+    meta = {"workload": f"kuhn{k}_genus3_softseed_lambda10", "tets": N, "h_cells": 1.0,
+            "inside_frac": round(float((labels == 0).float().mean()), 4)}
+    del V, T, Nb, sdf, flips, labels
+    return g, meta
+
+
+def delaunay_graph(npts, seed, torch, np, S, dev):
+    """Soft-seed graph of the Delaunay mesh of `npts` genus-3 points (95% on the
+    surface with 0.5-voxel normal noise at resolution 1536 * sqrt(npts/12e6),
+    5% uniform), snapped to the 2^-21 grid and tetrahedralised by the native
+    tetrahedraliser on the host's cores; labels from the SDF at tet centroids
+    with 2% flips, lambda 10, h = 1 voxel.  Returns (graph dict, meta)."""
+    t0 = time.time()
+    P, shape, res = S.genus3_point_cloud(npts, seed)
+    Q, V = S.quantise_points(P)
+    del P
+    t1 = time.time()
+    threads = min(16, os.cpu_count() or 1)
+    tets, nbrs, rep = S.native_delaunay(Q, threads)
+    t2 = time.time()
+    del Q
+    N = int(tets.shape[0])
+    log(f"[bench] delaunay mesh: {npts} points ({t1 - t0:.1f}s), {N} tets on {threads} threads ({t2 - t1:.1f}s)")
+    Vd = torch.from_numpy(V).to(dev)
+    Td = torch.from_numpy(tets).to(dev)
+    Nd = torch.from_numpy(nbrs).to(dev)
+    del tets, nbrs, V
+    sdf = S.tet_sdf_t(Vd, Td, shape)
+    flips = torch.from_numpy(S.flip_mask(N, 0.02, seed + 1).astype(np.uint8)).to(dev)
+    labels = (sdf >= 0).to(torch.uint8) ^ flips
+    g = S.soft_graph_t(Vd, Td, Nd, labels, shape, 10.0, 1.0 / res)
+    meta = {"workload": f"delaunay_genus3_{npts // 1_000_000 if npts >= 1_000_000 else npts}"
+                        f"{'m' if npts >= 1_000_000 else ''}pts_softseed_lambda10",
+            "points": npts, "tets": N, "repeated_points": rep, "voxel_res": round(res, 2), "h_voxels": 1.0,
+            "mesher": "native Delaunay (csrc/wtv_delaunay.cpp, canonical tet order)",
+            "mesh_seconds": round(t2 - t1, 2), "mesh_threads": threads, "points_seconds": round(t1 - t0, 2),
+            "inside_frac": round(float((labels == 0).float().mean()), 4)}
+    del Vd, Td, Nd, sdf, flips, labels
+    return g, meta
+
+
+def cpu_sample_leg(a, torch, np, W, S, dev, stream, n_full):
+    """CPU baseline on a bounded sample of the headline workload: the Delaunay
+    graph of a.cpu_sample_points points of the same generator, solved by
+    Boykov-Kolmogorov and by the restated reference push-relabel (one core
+    each), both checked equal to the GPU's flow and labels on that graph.
+    The BK time is reported per solve, scaled by tet count to the headline."""
+    import oracle
+    oracle.build()
+    g, meta = delaunay_graph(a.cpu_sample_points, a.seed + 5, torch, np, S, dev)
+    N = int(meta["tets"])
+    nbr, res0, term0 = g["nbr"], g["res"], g["term"]
+    res, term = res0.clone(), term0.clone()
+    part = torch.empty(N, dtype=torch.uint8, device=dev)
+    prm = W.params(global_relabel_period=a.period, local_steps=a.local_steps, relabel_alpha=a.alpha)
+    ts = []
+    for _ in range(4):
+        res.copy_(res0)
+        term.copy_(term0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = W.pr_solve(N, nbr.data_ptr(), res.data_ptr(), term.data_ptr(), part.data_ptr(), 1, prm, stream)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    gpu_s = sorted(ts[1:])[1]
+    nbr_h = nbr.cpu().numpy()
+    res_h = res0.cpu().numpy().view(np.uint32)
+    term_h = term0.cpu().numpy()
+    part_h = part.cpu().numpy()
+    tc = time.perf_counter()
+    b = oracle.bk_solve(nbr_h, res_h, term_h)
+    bk_s = time.perf_counter() - tc
+    assert b.raw_flow == st.raw_flow and np.array_equal(b.partition, part_h), "CPU sample: BK differs from the GPU"
+    tc = time.perf_counter()
+    pr = oracle.pr_solve(nbr_h, res_h, term_h)
+    pr_s = time.perf_counter() - tc
+    assert pr.raw_flow == st.raw_flow and np.array_equal(pr.partition, part_h), \
+        "CPU sample: restated push-relabel differs from the GPU"
+    scale = n_full / N
+    log(f"[bench] cpu sample: {N} tets, BK {bk_s:.2f}s, restated push-relabel {pr_s:.2f}s, GPU {gpu_s * 1e3:.1f} ms")
+    return {"value": round(bk_s * scale, 3), "unit": "s", "cores": 1, "kind": "port",
+            "sample": f"Boykov-Kolmogorov (oracle/wtv_oracle.c), 1 thread, on the Delaunay graph of "
+                      f"{a.cpu_sample_points} points of the same generator ({N} tets, {meta['repeated_points']} "
+                      f"repeated points): {bk_s:.3f} s, scaled by tet count x{scale:.3f} to the headline's "
+                      f"{n_full} tets; flow and labels equal to the GPU's on the sample",
+            "sample_tets": N, "sample_bk_s": round(bk_s, 3), "sample_restated_push_relabel_s": round(pr_s, 3),
+            "sample_gpu_s": round(gpu_s, 5), "sample_speedup_vs_bk": round(bk_s / gpu_s, 1),
+            "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
+
+
+def standin_leg(a, torch, np, W, S, dev, stream):
+    """The Kuhn 235^3 graph that stood in for the Delaunay mesh earlier: timed
+    solves, the flow checked against BK's (1 core) at full size, and its 8-way
+    slab partition (shard leg)."""
+    g, meta = kuhn_graph(a.k, a.seed, torch, np, S, dev)
+    N = int(meta["tets"])
+    nbr, res0, term0 = g["nbr"], g["res"], g["term"]
+    res, term = res0.clone(), term0.clone()
+    part = torch.empty(N, dtype=torch.uint8, device=dev)
+    prm = W.params(global_relabel_period=a.period, local_steps=a.local_steps, relabel_alpha=a.alpha)
+    ts, st = [], None
+    for _ in range(max(a.steps, 3) + 1):
+        res.copy_(res0)
+        term.copy_(term0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = W.pr_solve(N, nbr.data_ptr(), res.data_ptr(), term.data_ptr(), part.data_ptr(), 1, prm, stream)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    t_s = sum(ts[1:]) / len(ts[1:])
+    cut, _ = cut_capacity(torch, nbr, res0, term0, part)
+    assert cut == st.raw_flow, "Kuhn leg: cut capacity differs from the flow"
+    out = {"workload": meta["workload"], "tets": N, "s_per_solve": round(t_s, 5), "tets_per_s": round(N / t_s, 1),
+           "raw_flow": st.raw_flow, "certificate": "flow == integer cut capacity",
+           "work": {k2: getattr(st, k2) for k2 in ("rounds", "sub_rounds", "global_relabels", "bfs_levels")}}
+    if not a.no_cpu:
+        import oracle
+        tc = time.perf_counter()
+        b = oracle.bk_solve(nbr.cpu().numpy(), res0.cpu().numpy().view(np.uint32), term0.cpu().numpy())
+        out["cpu_bk_s"] = round(time.perf_counter() - tc, 3)
+        assert b.raw_flow == st.raw_flow and np.array_equal(b.partition, part.cpu().numpy()), \
+            "Kuhn leg: BK differs from the GPU"
+        out["identical_to_bk"] = True
+    log(f"[bench] kuhn stand-in: {N} tets, {t_s * 1e3:.2f} ms/solve")
+    shards = None if a.no_shard_leg else shard_leg(a, torch, W, a.k, nbr, res0, term0, stream)
+    del res, term, part, nbr, res0, term0, g
+    return out, shards
 
 
 def cut_leg(a, torch, np, W, S, dev, stream):

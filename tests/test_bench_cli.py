@@ -24,5 +24,6 @@ def test_every_leg_has_a_switch():
     assert r.returncode == 0
     for flag in ["--no-cpu", "--no-profile", "--batch-graphs", "--serve-requests", "--delaunay-points", "--no-tune",
                  "--no-cut-leg", "--no-topology-leg", "--no-udf-leg", "--no-surface-leg", "--no-flow-leg", "--no-shard-leg",
-                 "--copy-probe", "--atomic-probe", "--steps", "--warmup"]:
+                 "--copy-probe", "--atomic-probe", "--steps", "--warmup", "--no-standin-leg",
+                 "--workload", "--points", "--cpu-sample-points"]:
         assert flag in r.stdout, flag
