@@ -6,7 +6,9 @@
 * Poisson-arriving serving requests (100k to 20M tets, lambda 10) through the
   native server, each result equal to BK's and to the same graph solved alone;
 * the 78M-tet soft-seed Kuhn 235^3 genus-3 graph (the single-graph stand-in);
-* the Delaunay mesh of 2M genus-3 points (12.9M tets), scipy's Qhull.
+* the Delaunay mesh of 2M genus-3 points (12.9M tets), scipy's Qhull;
+* config 1 as written: the Delaunay mesh of 12M genus-3 points (~77.5M tets),
+  the native tetrahedraliser.
 """
 import time
 
@@ -163,3 +165,41 @@ def test_delaunay_2m_points_vs_bk(orc, torch_cuda):
     assert np.array_equal(p_h, bk.partition)
     assert orc.certificate(g.nbr, g.res, g.term, dr.cpu().numpy().view(np.uint32), dt.cpu().numpy(), p_h,
                            st.raw_flow) == 0
+
+
+@pytest.mark.timeout(900)
+def test_delaunay_12m_points_config1_vs_bk(orc, torch_cuda):
+    """Config 1 as written: 12M genus-3 points (95% on the fused surface with
+    0.5-voxel noise at resolution 1536, 5% uniform), their Delaunay mesh
+    (~77.5M tets, native tetrahedraliser), soft seeds, lambda 10, 2% flips.
+    Flow and labels equal BK's (~100 s on one core)."""
+    import os
+    torch = torch_cuda
+    t0 = time.time()
+    P, shape, res = S.genus3_point_cloud(12_000_000, 20260001)
+    Q, V = S.quantise_points(P)
+    del P
+    tets, nbrs, _ = S.native_delaunay(Q, min(16, os.cpu_count() or 1))
+    del Q
+    N = tets.shape[0]
+    print(f"config 1 mesh: {N} tets in {time.time() - t0:.1f} s", flush=True)
+    assert 76e6 < N < 79e6
+    Vd, Td, Nd = (torch.from_numpy(x).cuda() for x in (V, tets, nbrs))
+    del tets, nbrs, V
+    sdf = S.tet_sdf_t(Vd, Td, shape)
+    flips = torch.from_numpy(S.flip_mask(N, 0.02, 20260002).astype(np.uint8)).cuda()
+    lab = (sdf >= 0).to(torch.uint8) ^ flips
+    g = S.soft_graph_t(Vd, Td, Nd, lab, shape, 10.0, 1.0 / res)
+    del Vd, Td, Nd, sdf, flips, lab
+    nbr, res0, term0 = g["nbr"], g["res"], g["term"]
+    r, t = res0.clone(), term0.clone()
+    part = torch.empty(N, dtype=torch.uint8, device="cuda")
+    st = W.pr_solve(N, nbr.data_ptr(), r.data_ptr(), t.data_ptr(), part.data_ptr(), 1, W.params(),
+                    torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    print(f"config 1 solve: {st.t_solve_ms:.1f} ms device, flow {st.raw_flow}", flush=True)
+    nb_h, r0_h, t0_h = nbr.cpu().numpy(), res0.cpu().numpy().view(np.uint32), term0.cpu().numpy()
+    del g, nbr, res0, term0, r, t
+    bk = orc.bk_solve(nb_h, r0_h, t0_h)
+    assert st.raw_flow == bk.raw_flow
+    assert np.array_equal(part.cpu().numpy(), bk.partition)
