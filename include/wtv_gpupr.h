@@ -238,17 +238,22 @@ int wtv_pr_solve_flow(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_te
                       wtv_stats* stats, int64_t* returned, void* stream);
 
 /* ---------------------------------------------------------------------- */
-/* Serving: requests are graphs already resident on the device.  Worker     */
-/* slots (one host thread + one stream each) take every waiting request     */
-/* that fits when they free up and solve them as one batch; admission also  */
-/* requires the HBM footprint of the batch to fit the budget.               */
+/* Serving: requests are graphs already resident on the device.  The server  */
+/* runs `slots` sessions, each a host thread with its own stream and its own */
+/* node arena in HBM (hbm_budget_bytes / slots, at most max_batch_nodes     */
+/* nodes).  A session admits waiting requests (oldest first, while arena    */
+/* nodes and max_batch graph slots allow) at each global relabel of the     */
+/* solve it is running -- continuous batching: a request that arrives while */
+/* others are solved joins them on the device -- and each graph leaves at   */
+/* the first global relabel after which it has no active vertex, with the   */
+/* flow and labels of a lone solve.                                         */
 /* ---------------------------------------------------------------------- */
 typedef struct wtv_server_config {
-    int32_t slots;             /* concurrent batches on the device (default 4)   */
-    int32_t max_batch;         /* requests per batch (default 64)                */
-    uint64_t max_batch_nodes;  /* nodes per batch (default 2^30)                 */
-    uint64_t hbm_budget_bytes; /* footprint budget (default 200 GB)              */
-    wtv_params params;         /* solver knobs for every batch                   */
+    int32_t slots;             /* sessions on the device (default 4)             */
+    int32_t max_batch;         /* graphs resident in one session (default 64)    */
+    uint64_t max_batch_nodes;  /* arena nodes per session (default 2^27)         */
+    uint64_t hbm_budget_bytes; /* all arenas together (default 200 GB)           */
+    wtv_params params;         /* solver knobs for every session                 */
 } wtv_server_config;
 
 /* Ordering: the solve reads d_nbr, d_res and d_term only after the work that
@@ -269,22 +274,23 @@ typedef struct wtv_request {
 
 typedef struct wtv_result {
     int32_t status;            /* wtv_status of the solve                        */
-    int32_t batch_size;        /* requests solved together with this one         */
-    int32_t slot;              /* worker slot that ran it                        */
+    int32_t batch_size;        /* graphs resident when it joined, itself incl.   */
+    int32_t slot;              /* session that solved it                         */
     int32_t pad;
     int64_t raw_flow;
-    uint64_t footprint_bytes;  /* admitted HBM footprint of this request          */
+    uint64_t footprint_bytes;  /* arena bytes this request occupied              */
     double submit_s;           /* seconds since server start                     */
     double start_s;
     double end_s;
 } wtv_result;
 
-/* Footprint a request of n nodes is admitted with (graph + workspace bytes). */
+/* Arena bytes a request of n nodes occupies: n rounded up to 1024 nodes, times
+ * the arena's bytes per node (graph copy + solver state). */
 uint64_t wtv_request_footprint(uint32_t n);
 
 /* Create a server on the current device; NULL on failure (see wtv_last_error). */
 void* wtv_server_create(const wtv_server_config* cfg);
-/* Queue a request; returns a ticket.  Fails if the request alone exceeds the budget. */
+/* Queue a request; returns a ticket.  WTV_ENOMEM if it exceeds one session's arena. */
 int wtv_server_submit(void* server, const wtv_request* req, uint64_t* ticket);
 /* Block until the ticket's request finished and fill *out. */
 int wtv_server_wait(void* server, uint64_t ticket, wtv_result* out);

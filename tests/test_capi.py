@@ -99,9 +99,10 @@ def test_pybind_rejects_non_cuda_like_reference():
         G.graph_cut_fast(np.zeros((4, 2)), t, t, np.ones(1, np.uint8), 10.0)
 
 
-def test_request_footprint_is_linear():
+def test_request_footprint_counts_1024_node_tiles():
     f = W.lib().wtv_request_footprint
-    assert f(0) > 0 and f(2000) - f(1000) == f(3000) - f(2000)
+    assert f(0) == 0 and f(1) == f(1024) > 0 and f(1025) == 2 * f(1024)
+    assert f(3000) == 3 * f(1) and f(10 ** 6) == 977 * f(1)
 
 
 def test_topology_entry_points_reject_bad_arguments_without_device():

@@ -655,12 +655,15 @@ def test_server_orders_the_solve_after_the_request_stream(orc, torch_cuda):
 
 
 def test_server_admission_by_footprint(orc, torch_cuda):
+    """One session whose arena (the HBM budget) holds one 3000-node request at a
+    time: the requests are solved one after another, each alone, with the same
+    results; a request larger than the arena is refused at submit."""
     graphs = [S.random_graph(3000, 950 + i, cap_max=40) for i in range(4)]
     fp = W.lib().wtv_request_footprint(3000)
+    assert fp == W.lib().wtv_request_footprint(3072) and fp < W.lib().wtv_request_footprint(3073)
     dev = [_dev_graph(torch_cuda, *g) for g in graphs]
     torch_cuda.cuda.synchronize()
-    # the budget admits one request at a time: no batching, results unchanged
-    with W.Server(slots=2, max_batch=8, hbm_budget_bytes=int(fp * 1.5)) as srv:
+    with W.Server(slots=1, max_batch=8, hbm_budget_bytes=int(fp * 1.5)) as srv:
         tickets = [srv.submit(3000, d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), d[3].data_ptr())
                    for d in dev]
         res = [srv.wait(t) for t in tickets]
@@ -669,8 +672,52 @@ def test_server_admission_by_footprint(orc, torch_cuda):
         assert all(spans[i][1] <= spans[i + 1][0] + 1e-6 for i in range(len(spans) - 1)), "budget exceeded"
         with pytest.raises(W.WtvError, match="HBM budget"):
             srv.submit(10000, dev[0][0].data_ptr(), dev[0][1].data_ptr(), dev[0][2].data_ptr(), dev[0][3].data_ptr())
-    for g, r in zip(graphs, res):
-        assert r.raw_flow == orc.bk_solve(*g).raw_flow
+    for g, d, r in zip(graphs, dev, res):
+        w = orc.bk_solve(*g)
+        assert r.status == 0 and r.raw_flow == w.raw_flow
+        assert np.array_equal(d[3].cpu().numpy()[:3000], w.partition)
+
+
+def test_server_joins_requests_into_a_running_solve(orc, torch_cuda):
+    """Continuous batching: with one session, a 20M-tet graph is being solved
+    when small requests arrive; they join it on the device at its next global
+    relabel and, each graph leaving at the first relabel after its last active
+    vertex, finish before it.  The small results equal BK's, the large one the
+    same graph solved alone."""
+    import time
+    big = S.batch_workload(1, 20260011, 20_000_000, 20_000_001, device="cuda", lam_range=(10.0, 10.0))[0]
+    small = [S.random_graph(int(400 + 60 * i), 980 + i, cap_max=40) for i in range(6)]
+    _, _, k = _kuhn_soft(12, 13)
+    small.append((k.nbr, k.res, k.term))
+    n_big = big["N"]
+    dbig = (big["nbr"], big["res"].clone(), big["term"].clone(), torch_cuda.empty(n_big, dtype=torch_cuda.uint8,
+                                                                                   device="cuda"))
+    alone_part = torch_cuda.empty(n_big, dtype=torch_cuda.uint8, device="cuda")
+    alone = W.pr_solve(n_big, big["nbr"].data_ptr(), big["res"].data_ptr(), big["term"].data_ptr(),
+                       alone_part.data_ptr(), 1, W.params(global_relabel_period=8),
+                       torch_cuda.cuda.current_stream().cuda_stream)
+    dsmall = [_dev_graph(torch_cuda, *g) for g in small]
+    torch_cuda.cuda.synchronize()
+    # a global relabel every macro round: many admission and exit points
+    with W.Server(slots=1, max_batch=16, global_relabel_period=8) as srv:
+        tb = srv.submit(n_big, dbig[0].data_ptr(), dbig[1].data_ptr(), dbig[2].data_ptr(), dbig[3].data_ptr())
+        time.sleep(0.002)  # the large solve is under way (it takes tens of ms)
+        ts = [srv.submit(g[2].shape[0], d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), d[3].data_ptr())
+              for g, d in zip(small, dsmall)]
+        rs = [srv.wait(t) for t in ts]
+        rb = srv.wait(tb)
+    print(f"large {rb.start_s:.4f}-{rb.end_s:.4f} small",
+          [(r.batch_size, round(r.start_s, 4), round(r.end_s, 4)) for r in rs])
+    assert rb.status == 0 and all(r.status == 0 for r in rs)
+    assert rs[0].submit_s < rb.end_s, "the large solve ended before the small requests arrived"
+    assert all(r.batch_size >= 2 for r in rs), "a small request did not join the running solve"
+    assert all(r.end_s < rb.end_s for r in rs), "a small graph did not leave before the large one"
+    for g, d, r in zip(small, dsmall, rs):
+        w = orc.bk_solve(*g)
+        assert r.raw_flow == w.raw_flow
+        assert np.array_equal(d[3].cpu().numpy()[:g[2].shape[0]], w.partition)
+    assert rb.raw_flow == alone.raw_flow
+    assert torch_cuda.equal(dbig[3], alone_part)
 
 
 # ------------------------------------------------------------ device graph build

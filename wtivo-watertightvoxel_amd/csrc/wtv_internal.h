@@ -45,6 +45,45 @@ void batch_solve(uint32_t G, const uint64_t* h_off, const int32_t* d_nbr, uint32
                  uint8_t* partition, bool partition_on_device, int64_t* h_flows, const wtv_params& p,
                  wtv_stats& st, void* stream);
 
+// ------------------------------------------------------------ serving
+// One request as an arena session sees it.
+struct SessionJob {
+    uint64_t ticket = 0;
+    wtv_request req{};
+    void* ready = nullptr;  // hipEvent_t recorded on the request's stream at submit, or null
+    bool alone = false;     // solve with no other graph resident (after a shared solve failed)
+};
+
+// The request queue arena sessions pull from (wtv_server.cpp).
+class SessionQueue {
+  public:
+    virtual ~SessionQueue() = default;
+    // Oldest-first requests whose node counts, each rounded up to a multiple
+    // of 1024, fit one after another into `free_nodes` contiguous nodes and
+    // `free_slots` graph slots; an `alone` request is taken only when nothing
+    // is resident, and by itself.
+    virtual std::vector<SessionJob> take(uint64_t free_nodes, uint32_t free_slots, bool resident_any) = 0;
+    // Whether take() would return a request now.
+    virtual bool waiting(uint64_t free_nodes, uint32_t free_slots, bool resident_any) = 0;
+    // Blocks until a request waits (true) or the queue stops with nothing waiting (false).
+    virtual bool wait_nonempty() = 0;
+    // A request is done.
+    virtual void finish(uint64_t ticket, int status, int64_t flow, int32_t batch_size, int32_t slot, double start_s,
+                        double end_s) = 0;
+    // Requests a failed shared solve held, back to the front of the queue to be solved alone.
+    virtual void requeue_alone(const std::vector<SessionJob>& jobs) = 0;
+    // Seconds since the server started.
+    virtual double now() const = 0;
+};
+
+// Arena bytes per node of a session (rows, workspace, graph ids).
+uint64_t arena_bytes_per_node();
+
+// Runs an arena session on the calling thread until the queue stops: an
+// arena of `capacity` nodes and `slots` graph slots on stream s, fed from q.
+void session_serve(SessionQueue& q, const wtv_params& prm, uint32_t capacity, uint32_t slots, void* s,
+                   int32_t worker);
+
 // Counters of the reduced-graph build (wtivo_gpupr_bindings.cpp:139-318).
 struct BuildInfo {
     uint64_t free_n, fixed_source, fixed_sink, full_refs, free_refs, terminal_edges;

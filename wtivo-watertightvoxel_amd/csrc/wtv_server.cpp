@@ -1,18 +1,22 @@
 // wtv_server.cpp -- serving loop around the solver (no reference counterpart).
 //
-// Requests are reduced graphs already resident in HBM.  A server owns S worker
-// slots; each slot is a host thread with its own HIP stream (default priority).
-// Admission is FIFO: when a slot is free it takes the oldest waiting requests
-// that fit max_batch, max_batch_nodes and the HBM budget (footprint of the
-// graphs plus the solver workspace of the batch), and solves them together:
-// one request goes straight to the plain solver, several are gathered into
-// contiguous buffers, solved as one union (wtv_batch_solve semantics, every
-// graph keeps its own source height) and scattered back.  Requests that arrive
-// while other batches are on the device are admitted as slots and memory free
-// up, and batches in different slots overlap on the device.  Results do not
-// depend on the mix: each graph's flow and labels are those of a lone solve.
+// Requests are reduced graphs already resident in HBM.  A server runs S arena
+// sessions (session_serve, wtv_solver.hip), each a host thread with its own
+// HIP stream (default priority) and its own node arena in HBM.  A session
+// admits waiting requests at its global relabels: continuous batching, so a
+// request that arrives while others are being solved joins them on the
+// device at that session's next relabel, as arena nodes and graph slots
+// allow, and an idle session takes it at once.  Each graph leaves at the
+// first relabel after which it has no active vertex: its partition,
+// residuals and terminals are written to the request's buffers.  Admission
+// is FIFO by arena footprint; the arenas together stay within the HBM
+// budget.  Results do not depend on the mix: each graph's flow and labels are
+// those of a lone solve.  A request whose graph is invalid fails alone at
+// admission; if a shared solve fails, its requests are solved again one by
+// one, so only the request at fault reports the error.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -34,52 +38,42 @@ inline void cks(hipError_t e, const char* where) {
     fail(WTV_EDEVICE, std::string(kPrefix) + "HIP at " + where + ": " + hipGetErrorString(e));
 }
 
-struct Job {
-    uint64_t ticket;
-    wtv_request req;
-    uint64_t footprint;
-    double submit_s;
-    hipEvent_t ready = nullptr;  // recorded on req.stream at submit
+inline uint64_t aligned_nodes(uint64_t n) { return (n + 1023u) / 1024u * 1024u; }
+
+struct Ticket {
+    hipEvent_t ready = nullptr;  // recorded on the request's stream at submit
     bool done = false;
     wtv_result res{};
-    ~Job() { if (ready) (void)hipEventDestroy(ready); }
-};
-
-struct DevMem {
-    void* p = nullptr;
-    ~DevMem() { if (p) (void)hipFree(p); }
-};
-
-// Waits for the stream on scope exit, also when an exception unwinds: the
-// gather buffers declared before it must outlive every copy queued on it.
-struct SyncOnExit {
-    hipStream_t s;
-    ~SyncOnExit() { (void)hipStreamSynchronize(s); }
+    ~Ticket() { if (ready) (void)hipEventDestroy(ready); }
 };
 
 }  // namespace
 
 uint64_t request_footprint(uint32_t n) {
-    // graph 36 B + plain-solver workspace 26 B per node + batched extras
-    // (graph id 4, union ids 16, gathered nbr/res/term/partition 37) + slack
-    return static_cast<uint64_t>(n) * (36ull + 26ull + 20ull + 37ull) + (1ull << 20);
+    // the arena nodes a request occupies (rounded up to the seed sweep's
+    // 1024-node tile) times the arena bytes per node
+    return aligned_nodes(n) * arena_bytes_per_node();
 }
 
-class Server {
+class Server final : public SessionQueue {
   public:
     explicit Server(const wtv_server_config& c) : cfg_(c) {
-        // four slots: concurrent latency-bound solves overlap on the device; more
-        // than the four hardware queues of a process did not lower latency further
         if (cfg_.slots <= 0) cfg_.slots = 4;
         if (cfg_.max_batch <= 0) cfg_.max_batch = 64;
-        if (cfg_.max_batch_nodes == 0) cfg_.max_batch_nodes = 1ull << 30;
+        if (cfg_.max_batch_nodes == 0) cfg_.max_batch_nodes = 1ull << 27;
         if (cfg_.max_batch_nodes > 0x7FFFFFFFull) cfg_.max_batch_nodes = 0x7FFFFFFFull;
         if (cfg_.hbm_budget_bytes == 0) cfg_.hbm_budget_bytes = 200ull * 1000ull * 1000ull * 1000ull;
+        // each session's arena: at most max_batch_nodes, and all of them
+        // together within the HBM budget
+        const uint64_t per = cfg_.hbm_budget_bytes / static_cast<uint64_t>(cfg_.slots) / arena_bytes_per_node();
+        capacity_ = std::min<uint64_t>(cfg_.max_batch_nodes, per) / 1024u * 1024u;
+        if (capacity_ < 1024u) fail(WTV_ENOMEM, std::string(kPrefix) + "HBM budget below one arena tile");
         cks(hipGetDevice(&device_), "hipGetDevice");
         t0_ = std::chrono::steady_clock::now();
+        live_ = cfg_.slots;
         for (int i = 0; i < cfg_.slots; ++i) workers_.emplace_back([this, i]() { worker(i); });
     }
-    ~Server() {
+    ~Server() override {
         {
             std::lock_guard<std::mutex> lk(mu_);
             stop_ = true;
@@ -92,181 +86,152 @@ class Server {
         if (r.n && (!r.d_nbr || !r.d_res || !r.d_term || !r.d_partition))
             fail(WTV_EINVAL, std::string(kPrefix) + "null request pointer");
         if (r.n > 0x7FFFFFFFu) fail(WTV_EINVAL, std::string(kPrefix) + ">INT32_MAX reduced nodes unsupported");
-        const uint64_t fp = request_footprint(r.n);
-        if (fp > cfg_.hbm_budget_bytes) fail(WTV_ENOMEM, std::string(kPrefix) + "request exceeds the HBM budget");
-        std::lock_guard<std::mutex> lk(mu_);
-        if (stop_) fail(WTV_EINVAL, std::string(kPrefix) + "server is stopping");
-        auto job = std::make_shared<Job>();
+        if (aligned_nodes(r.n) > capacity_)
+            fail(WTV_ENOMEM, std::string(kPrefix) + "request exceeds the HBM budget of one session's arena");
+        auto t = std::make_shared<Ticket>();
         if (r.stream) {
             // the solve waits for what the client queued on its stream so far
-            cks(hipEventCreateWithFlags(&job->ready, hipEventDisableTiming), "ready event");
-            cks(hipEventRecord(job->ready, static_cast<hipStream_t>(r.stream)), "ready record");
+            cks(hipEventCreateWithFlags(&t->ready, hipEventDisableTiming), "ready event");
+            cks(hipEventRecord(t->ready, static_cast<hipStream_t>(r.stream)), "ready record");
         }
-        job->ticket = next_ticket_++;
-        job->req = r;
-        job->footprint = fp;
-        job->submit_s = now();
-        jobs_[job->ticket] = job;
-        queue_.push_back(job);
-        cv_work_.notify_one();
-        return job->ticket;
+        std::lock_guard<std::mutex> lk(mu_);
+        if (stop_) fail(WTV_EINVAL, std::string(kPrefix) + "server is stopping");
+        if (live_ == 0) fail(WTV_ENOMEM, std::string(kPrefix) + "no serving session could allocate its arena");
+        SessionJob j;
+        j.ticket = next_ticket_++;
+        j.req = r;
+        j.ready = t->ready;
+        t->res.footprint_bytes = request_footprint(r.n);
+        t->res.submit_s = now();
+        tickets_[j.ticket] = t;
+        queue_.push_back(j);
+        cv_work_.notify_all();
+        return j.ticket;
     }
 
     wtv_result wait(uint64_t ticket) {
         std::unique_lock<std::mutex> lk(mu_);
-        auto it = jobs_.find(ticket);
-        if (it == jobs_.end()) fail(WTV_EINVAL, std::string(kPrefix) + "unknown ticket");
-        auto job = it->second;
-        cv_done_.wait(lk, [&]() { return job->done; });
-        jobs_.erase(ticket);
-        return job->res;
+        auto it = tickets_.find(ticket);
+        if (it == tickets_.end()) fail(WTV_EINVAL, std::string(kPrefix) + "unknown ticket");
+        auto t = it->second;
+        cv_done_.wait(lk, [&]() { return t->done; });
+        tickets_.erase(ticket);
+        return t->res;
     }
 
-  private:
-    double now() const {
+    // ---- SessionQueue
+    std::vector<SessionJob> take(uint64_t free_nodes, uint32_t free_slots, bool resident_any) override {
+        std::lock_guard<std::mutex> lk(mu_);
+        std::vector<SessionJob> out;
+        scan(free_nodes, free_slots, resident_any, &out);
+        return out;
+    }
+    bool waiting(uint64_t free_nodes, uint32_t free_slots, bool resident_any) override {
+        std::lock_guard<std::mutex> lk(mu_);
+        return scan(free_nodes, free_slots, resident_any, nullptr);
+    }
+    bool wait_nonempty() override {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_work_.wait(lk, [&]() { return stop_ || !queue_.empty(); });
+        return !queue_.empty();
+    }
+    void finish(uint64_t ticket, int status, int64_t flow, int32_t batch_size, int32_t slot, double start_s,
+                double end_s) override {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            auto it = tickets_.find(ticket);
+            if (it == tickets_.end()) return;
+            wtv_result& r = it->second->res;
+            r.status = status;
+            r.batch_size = batch_size;
+            r.slot = slot;
+            r.raw_flow = flow;
+            r.start_s = std::max(start_s, r.submit_s);
+            r.end_s = end_s;
+            it->second->done = true;
+        }
+        cv_done_.notify_all();
+    }
+    void requeue_alone(const std::vector<SessionJob>& jobs) override {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (size_t i = jobs.size(); i-- > 0;) {
+                SessionJob j = jobs[i];
+                j.alone = true;
+                queue_.push_front(j);
+            }
+        }
+        cv_work_.notify_all();
+    }
+    double now() const override {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
     }
 
-    // Oldest-first admission of everything that fits; called with mu_ held.
-    std::vector<std::shared_ptr<Job>> take_batch() {
-        std::vector<std::shared_ptr<Job>> b;
-        uint64_t nodes = 0, bytes = 0;
-        while (!queue_.empty() && static_cast<int>(b.size()) < cfg_.max_batch) {
-            auto& j = queue_.front();
-            if (!b.empty() && nodes + j->req.n > cfg_.max_batch_nodes) break;
-            if (used_bytes_ + bytes + j->footprint > cfg_.hbm_budget_bytes) break;
-            nodes += j->req.n;
-            bytes += j->footprint;
-            b.push_back(j);
+  private:
+    // Oldest-first requests that fit one after another into free_nodes and
+    // free_slots (into *out when given); whether there is at least one.
+    bool scan(uint64_t free_nodes, uint32_t free_slots, bool resident_any, std::vector<SessionJob>* out) {
+        uint64_t room = free_nodes;
+        uint32_t slots = std::min<uint32_t>(free_slots, static_cast<uint32_t>(cfg_.max_batch));
+        bool any = false;
+        while (!queue_.empty() && slots > 0) {
+            const SessionJob& j = queue_.front();
+            if (j.alone) {
+                // solved with nothing else resident
+                if (resident_any || any) break;
+                if (aligned_nodes(j.req.n) > room) break;
+                any = true;
+                if (!out) break;
+                out->push_back(j);
+                queue_.pop_front();
+                break;
+            }
+            const uint64_t need = aligned_nodes(j.req.n);
+            if (need > room) break;
+            room -= need;
+            --slots;
+            any = true;
+            if (!out) break;
+            out->push_back(j);
             queue_.pop_front();
         }
-        used_bytes_ += bytes;
-        return b;
+        return any;
     }
 
     void worker(int slot) {
         (void)hipSetDevice(device_);
         hipStream_t s = nullptr;
-        (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
-        for (;;) {
-            std::vector<std::shared_ptr<Job>> batch;
-            {
-                std::unique_lock<std::mutex> lk(mu_);
-                cv_work_.wait(lk, [&]() {
-                    if (stop_ && queue_.empty()) return true;
-                    if (queue_.empty()) return false;
-                    return used_bytes_ + queue_.front()->footprint <= cfg_.hbm_budget_bytes;
-                });
-                if (queue_.empty()) break;
-                batch = take_batch();
-            }
-            const double start = now();
-            std::vector<int64_t> flows(batch.size(), 0);
-            std::vector<int> status(batch.size(), WTV_OK);
-            const int st = attempt([&]() { run(batch, flows, s); });
-            if (st != WTV_OK) {
-                if (batch.size() == 1) {
-                    status[0] = st;
-                } else {
-                    // a failed batch leaves every request untouched (results are
-                    // scattered back only on success): solve each alone, so only
-                    // the request at fault reports the error
-                    for (size_t i = 0; i < batch.size(); ++i) {
-                        std::vector<std::shared_ptr<Job>> one{batch[i]};
-                        std::vector<int64_t> f1(1, 0);
-                        status[i] = attempt([&]() { run(one, f1, s); });
-                        flows[i] = f1[0];
-                    }
-                }
-            }
-            const double end = now();
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+        try {
+            session_serve(*this, cfg_.params, static_cast<uint32_t>(capacity_),
+                          static_cast<uint32_t>(cfg_.max_batch), s, slot);
+        } catch (const Error& e) {
+            // the session could not start (its arena did not fit): the
+            // others serve the queue; with none left, the waiting requests
+            // fail instead of waiting forever
+            std::vector<uint64_t> dropped;
             {
                 std::lock_guard<std::mutex> lk(mu_);
-                for (size_t i = 0; i < batch.size(); ++i) {
-                    auto& j = batch[i];
-                    used_bytes_ -= j->footprint;
-                    j->res.status = status[i];
-                    j->res.batch_size = static_cast<int32_t>(batch.size());
-                    j->res.slot = slot;
-                    j->res.raw_flow = flows[i];
-                    j->res.footprint_bytes = j->footprint;
-                    j->res.submit_s = j->submit_s;
-                    j->res.start_s = start;
-                    j->res.end_s = end;
-                    j->done = true;
+                if (--live_ == 0) {
+                    for (const SessionJob& j : queue_) dropped.push_back(j.ticket);
+                    queue_.clear();
                 }
             }
-            cv_done_.notify_all();
-            cv_work_.notify_all();
+            for (uint64_t t : dropped) finish(t, e.code, 0, 0, slot, now(), now());
         }
         if (s) (void)hipStreamDestroy(s);
     }
 
-    template <class F>
-    static int attempt(F&& f) {
-        try {
-            f();
-            return WTV_OK;
-        } catch (const Error& e) {
-            return e.code;
-        } catch (...) {
-            return WTV_EINVAL;
-        }
-    }
-
-    void run(const std::vector<std::shared_ptr<Job>>& batch, std::vector<int64_t>& flows, hipStream_t s) {
-        wtv_stats st{};
-        for (const auto& j : batch)
-            if (j->ready) cks(hipStreamWaitEvent(s, j->ready, 0), "ready wait");
-        if (batch.size() == 1) {
-            const wtv_request& r = batch[0]->req;
-            pr_solve(r.n, r.d_nbr, r.d_res, r.d_term, r.d_partition, true, cfg_.params, st, s);
-            flows[0] = st.raw_flow;
-            return;
-        }
-        const uint32_t G = static_cast<uint32_t>(batch.size());
-        std::vector<uint64_t> off(G + 1, 0);
-        for (uint32_t g = 0; g < G; ++g) off[g + 1] = off[g] + batch[g]->req.n;
-        const size_t N = off[G];
-        DevMem nbr, res, term, part;
-        SyncOnExit sync{s};
-        cks(hipMalloc(&nbr.p, std::max<size_t>(N, 1) * 16u), "gather nbr");
-        cks(hipMalloc(&res.p, std::max<size_t>(N, 1) * 16u), "gather res");
-        cks(hipMalloc(&term.p, std::max<size_t>(N, 1) * 4u), "gather term");
-        cks(hipMalloc(&part.p, std::max<size_t>(N, 1)), "gather partition");
-        char* pn = static_cast<char*>(nbr.p);
-        char* pr = static_cast<char*>(res.p);
-        char* pt = static_cast<char*>(term.p);
-        char* pp = static_cast<char*>(part.p);
-        for (uint32_t g = 0; g < G; ++g) {
-            const wtv_request& r = batch[g]->req;
-            if (!r.n) continue;
-            cks(hipMemcpyAsync(pn + off[g] * 16u, r.d_nbr, r.n * 16ull, hipMemcpyDeviceToDevice, s), "gather");
-            cks(hipMemcpyAsync(pr + off[g] * 16u, r.d_res, r.n * 16ull, hipMemcpyDeviceToDevice, s), "gather");
-            cks(hipMemcpyAsync(pt + off[g] * 4u, r.d_term, r.n * 4ull, hipMemcpyDeviceToDevice, s), "gather");
-        }
-        batch_solve(G, off.data(), reinterpret_cast<int32_t*>(nbr.p), reinterpret_cast<uint32_t*>(res.p),
-                    reinterpret_cast<int32_t*>(term.p), reinterpret_cast<uint8_t*>(part.p), true, flows.data(),
-                    cfg_.params, st, s);
-        for (uint32_t g = 0; g < G; ++g) {
-            const wtv_request& r = batch[g]->req;
-            if (!r.n) continue;
-            cks(hipMemcpyAsync(r.d_res, pr + off[g] * 16u, r.n * 16ull, hipMemcpyDeviceToDevice, s), "scatter");
-            cks(hipMemcpyAsync(r.d_term, pt + off[g] * 4u, r.n * 4ull, hipMemcpyDeviceToDevice, s), "scatter");
-            cks(hipMemcpyAsync(r.d_partition, pp + off[g], r.n, hipMemcpyDeviceToDevice, s), "scatter");
-        }
-        cks(hipStreamSynchronize(s), "scatter sync");
-    }
-
     wtv_server_config cfg_;
+    uint64_t capacity_ = 0;  // arena nodes per session
     int device_ = 0;
     std::chrono::steady_clock::time_point t0_;
     std::mutex mu_;
     std::condition_variable cv_work_, cv_done_;
-    std::deque<std::shared_ptr<Job>> queue_;
-    std::map<uint64_t, std::shared_ptr<Job>> jobs_;
+    std::deque<SessionJob> queue_;
+    std::map<uint64_t, std::shared_ptr<Ticket>> tickets_;
     uint64_t next_ticket_ = 1;
-    uint64_t used_bytes_ = 0;
+    int live_ = 0;  // sessions serving the queue
     bool stop_ = false;
     std::vector<std::thread> workers_;
 };

@@ -44,6 +44,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <optional>
 #include <sstream>
@@ -262,19 +263,22 @@ __global__ void __launch_bounds__(kTPB) k_globalize(uint32_t n, uint32_t G, cons
     }
 }
 
-// Reverse slots and preflow init (gpu_push_relabel_fast.cu:69-83).
-__global__ void __launch_bounds__(kTPB) k_prepare(uint32_t n, const int4* __restrict__ nbr4,
+// Reverse slots and preflow init (gpu_push_relabel_fast.cu:69-83) of the
+// nodes [lo, hi), whose neighbours lie in [lo, hi) too (a graph of the
+// arena, or the whole solve).  With `wl` the vertices holding source supply
+// are appended to it (counter *cnt); without, the pass only validates.
+__global__ void __launch_bounds__(kTPB) k_prepare(uint32_t lo, uint32_t hi, const int4* __restrict__ nbr4,
                                                   const int32_t* __restrict__ term, uint8_t* __restrict__ rslot,
-                                                  long long* __restrict__ excess, uint32_t* __restrict__ wl0,
-                                                  Ctl* ctl) {
+                                                  long long* __restrict__ excess, uint32_t* __restrict__ wl,
+                                                  uint32_t* cnt, Ctl* ctl) {
     __shared__ WgQueue<kTPB> q;
     q_init(q);
     const uint32_t stride = gridDim.x * kTPB;
     // uniform trip count per workgroup: every thread reaches each queue flush
-    for (uint32_t base = blockIdx.x * kTPB; base < n; base += stride) {
+    for (uint32_t base = lo + blockIdx.x * kTPB; base < hi; base += stride) {
         const uint32_t v = base + threadIdx.x;
         bool cand = false;
-        if (v < n) {
+        if (v < hi) {
             const int4 nb = nbr4[v];
             uint32_t code = 0;
             int err = 0;
@@ -282,7 +286,7 @@ __global__ void __launch_bounds__(kTPB) k_prepare(uint32_t n, const int4* __rest
             for (int k = 0; k < 4; ++k) {
                 const int u = slot_of(nb, k);
                 if (u < 0) continue;
-                if (static_cast<uint32_t>(u) >= n) { err = 3; continue; }
+                if (static_cast<uint32_t>(u) < lo || static_cast<uint32_t>(u) >= hi) { err = 3; continue; }
                 const int4 nu = nbr4[u];
                 const int iv = static_cast<int>(v);
                 const int j = nu.x == iv ? 0 : nu.y == iv ? 1 : nu.z == iv ? 2 : nu.w == iv ? 3 : -1;
@@ -293,12 +297,14 @@ __global__ void __launch_bounds__(kTPB) k_prepare(uint32_t n, const int4* __rest
             rslot[v] = static_cast<uint8_t>(code);
             const int32_t t = term[v];
             excess[v] = t > 0 ? static_cast<long long>(t) : 0LL;
-            cand = t > 0;  // every vertex with excess joins the first candidate list
+            cand = wl && t > 0;  // every vertex with excess joins the candidate list
         }
-        q_push(q, cand, v);
-        q_flush_if(q, wl0, &ctl->wl_cnt[0], kTPB);
+        if (wl) {
+            q_push(q, cand, v);
+            q_flush_if(q, wl, cnt, kTPB);
+        }
     }
-    q_flush(q, wl0, &ctl->wl_cnt[0]);
+    if (wl) q_flush(q, wl, cnt);
 }
 
 // Batched solves: the graphs that still hold an active vertex.  Graphs share
@@ -355,7 +361,8 @@ __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __
                                                    uint32_t* __restrict__ height, uint8_t* __restrict__ nzm,
                                                    uint32_t* __restrict__ fout, uint32_t first,
                                                    const uint32_t* __restrict__ gid,
-                                                   const uint8_t* __restrict__ galive, Ctl* ctl) {
+                                                   const uint8_t* __restrict__ galive,
+                                                   const uint8_t* __restrict__ gfresh, Ctl* ctl) {
     __shared__ WgQueue<kTPB> q;
     q_init(q);
     const uint32_t span = kTPB * kSeedUnroll;
@@ -370,6 +377,9 @@ __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __
             for (uint32_t g = gid[base]; g <= gid[last] && !any; ++g) any = galive[g] != 0u;
             if (!any) continue;
         }
+        // arena solves: a tile lies in one graph; a graph admitted since the
+        // last relabel has no heights yet, so all of them are written
+        const uint32_t tfirst = gfresh ? static_cast<uint32_t>(gfresh[gid[base]]) : first;
         uint32_t v[kSeedUnroll];
         int32_t t[kSeedUnroll];
 #pragma unroll
@@ -404,7 +414,7 @@ __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __
             // valid labels give it h <= 1, and term < 0 never returns once
             // the sink arc is saturated, so later relabels skip those writes
             if (static_cast<uint64_t>(base) + static_cast<uint64_t>(j) * kTPB + threadIdx.x < n &&
-                (t[j] >= 0 || first)) {
+                (t[j] >= 0 || tfirst)) {
                 height[v[j]] = t[j] < 0 ? 1u : (found[j] ? 2u : kSent);
                 if (t[j] < 0) nzm[v[j]] = 0u;  // height 1 for good: never unreached again
             }
@@ -923,6 +933,85 @@ __global__ void __launch_bounds__(kTailTPB) k_discharge_tail(uint32_t n, uint32_
     }
 }
 
+// ---------------------------------------------------- arena (serving) kernels
+// A serving session keeps many graphs in one node arena.  Graph slot g owns
+// nodes [lo, lo + ag), ag = its node count ng rounded up to a multiple of
+// 1024 (the seed sweep's tile), so every tile belongs to one graph; nodes
+// past ng are padding without arcs or terminals.  Freed ranges are inert:
+// no arcs, no terminals, no excess, height 0 (never taken for unreached)
+// and the sentinel graph slot, whose tiles the sweeps skip.
+
+// Admission of graph slot g: union-wide neighbour ids (ids outside [0, ng)
+// are device invariant 3), graph ids, inert padding.
+__global__ void __launch_bounds__(kTPB) k_admit(uint32_t lo, uint32_t ng, uint32_t ag, uint32_t g,
+                                                const int4* __restrict__ nbr_local, int4* __restrict__ nbrg,
+                                                uint4* __restrict__ res, int32_t* __restrict__ term,
+                                                uint32_t* __restrict__ gid, Ctl* ctl) {
+    int err = 0;
+    for (uint32_t i = blockIdx.x * kTPB + threadIdx.x; i < ag; i += gridDim.x * kTPB) {
+        const uint32_t v = lo + i;
+        gid[v] = g;
+        if (i < ng) {
+            const int4 a = nbr_local[i];
+            auto map = [&](int x) -> int {
+                if (x < 0) return -1;
+                if (static_cast<uint32_t>(x) >= ng) { err = 3; return -1; }
+                return static_cast<int>(lo + static_cast<uint32_t>(x));
+            };
+            nbrg[v] = make_int4(map(a.x), map(a.y), map(a.z), map(a.w));
+        } else {
+            nbrg[v] = make_int4(-1, -1, -1, -1);
+            res[v] = make_uint4(0u, 0u, 0u, 0u);
+            term[v] = 0;
+        }
+    }
+    if (err) set_error(ctl, err);
+}
+
+// Per-slot tables of graph slot g: node count (its source height), flow,
+// residency and the fresh flag (no heights yet).
+__global__ void k_set_slot(uint32_t g, uint32_t ng, uint32_t* gn, unsigned long long* gflow, uint8_t* gres,
+                           uint8_t* gfresh, uint8_t resident) {
+    gn[g] = ng;
+    gflow[g] = 0ull;
+    gres[g] = resident;
+    gfresh[g] = resident;
+}
+
+__global__ void k_fill_u8(uint8_t* p, uint32_t n, uint8_t x) {
+    for (uint32_t i = blockIdx.x * kTPB + threadIdx.x; i < n; i += gridDim.x * kTPB) p[i] = x;
+}
+
+// partition of graph slot g (gpu_push_relabel_fast.cu:315-324) into its
+// caller's buffer; vertices still active are invariant violations
+__global__ void __launch_bounds__(kTPB) k_partition_range(uint32_t lo, uint32_t ng, uint32_t sh,
+                                                          const uint32_t* __restrict__ height,
+                                                          const long long* __restrict__ excess,
+                                                          uint8_t* __restrict__ out, Ctl* ctl) {
+    unsigned long long bad = 0;
+    for (uint32_t i = blockIdx.x * kTPB + threadIdx.x; i < ng; i += gridDim.x * kTPB) {
+        const uint32_t h = height[lo + i];
+        out[i] = h < sh + 1u ? 1u : 0u;
+        if (excess[lo + i] > 0 && h < sh) ++bad;
+    }
+    bad = wave_sum(bad);
+    if (__lane_id() == 0 && bad) atom_add(&ctl->violations, static_cast<uint32_t>(bad));
+}
+
+// Freed nodes [lo, hi) become inert.
+__global__ void __launch_bounds__(kTPB) k_inert(uint32_t lo, uint32_t hi, int4* __restrict__ nbrg,
+                                                int32_t* __restrict__ term, long long* __restrict__ excess,
+                                                uint32_t* __restrict__ height, uint32_t* __restrict__ gid,
+                                                uint32_t sentinel) {
+    for (uint32_t v = lo + blockIdx.x * kTPB + threadIdx.x; v < hi; v += gridDim.x * kTPB) {
+        nbrg[v] = make_int4(-1, -1, -1, -1);
+        term[v] = 0;
+        excess[v] = 0;
+        height[v] = 0u;
+        gid[v] = sentinel;
+    }
+}
+
 // partition = reaches the sink (gpu_push_relabel_fast.cu:315-324); counts any
 // vertex still active after the final BFS (an invariant violation).
 template <bool kB>
@@ -1121,13 +1210,16 @@ struct GraphExec {
 };
 
 struct Layout {
-    size_t height, excess, rslot, nzm, wl0, wl1, rl, ctl, gid, nbrg, gn, goff, gflow, galive, total;
+    size_t height, excess, rslot, nzm, wl0, wl1, rl, ctl, gid, nbrg, gn, goff, gflow, galive, gres, gfresh, ares,
+        aterm, total;
 };
 
 // One allocation carved into 256-byte aligned arrays.  Batched mode adds the
 // graph id per node, a union-wide copy of the neighbour ids and three
-// per-graph tables (node count, offsets, sink flow).
-Layout layout_for(uint32_t n, uint32_t G) {
+// per-graph tables (node count, offsets, sink flow).  An arena (serving) also
+// holds the residual rows and terminals of its graphs and two more per-slot
+// byte tables (resident, fresh).
+Layout layout_for(uint32_t n, uint32_t G, bool arena = false) {
     Layout L{};
     size_t o = 0;
     auto take = [&](size_t bytes) { size_t at = o; o = align_up(o + bytes, 256); return at; };
@@ -1147,6 +1239,12 @@ Layout layout_for(uint32_t n, uint32_t G) {
         L.goff = take((static_cast<size_t>(G) + 1u) * 8u);
         L.gflow = take(static_cast<size_t>(G) * 8u);
         L.galive = take(G);
+        if (arena) {
+            L.gres = take(G);
+            L.gfresh = take(G);
+            L.ares = take(N * 16u);
+            L.aterm = take(N * 4u);
+        }
     }
     L.total = o;
     return L;
@@ -1169,22 +1267,78 @@ struct BatchArgs {
 
 }  // namespace
 
-// fixed_ws != nullptr: the solve allocates nothing (no device or pinned
-// memory, streams, events or graphs).  fixed_ws is a workspace of at least
+// The device arrays of an arena solve, as the feeder that fills it sees them.
+struct ArenaView {
+    hipStream_t s;
+    Ctl* ctl;
+    int4* nbrg;
+    uint4* res;
+    int32_t* term;
+    long long* excess;
+    uint32_t* height;
+    uint8_t* rslot;
+    uint32_t* gid;
+    uint32_t* gn;
+    unsigned long long* gflow;
+    uint8_t* galive;  // graphs with an active vertex (after a relabel's filter)
+    uint8_t* gres;    // resident graphs
+    uint8_t* gfresh;  // graphs admitted since the last relabel
+    uint32_t* wl[2];
+    uint32_t capacity;  // nodes
+    uint32_t slots;     // graph slots; slot `slots` is the sentinel of free nodes
+    uint32_t hwm;       // node bound of the sweeps: the end of the highest used range
+};
+
+// A source of graphs for an arena solve (the serving loop).  solve_t calls it
+// at every global relabel, and when the active list drains, on the solve's
+// own host thread and stream.
+struct Feeder {
+    virtual ~Feeder() = default;
+    // Before a global relabel: admit waiting graphs (their vertices holding
+    // source supply are appended to the active list wl[p]) and set hwm.
+    virtual void admit(ArenaView& a, uint32_t p) = 0;
+    // After the relabel and the active-list filter: finish every resident
+    // graph without an active vertex (its heights are final).
+    virtual void finish(ArenaView& a, uint32_t p) = 0;
+    // Requests are waiting that the arena could take now.
+    virtual bool pending() = 0;
+    // The active list is empty and nothing is resident: block until a graph
+    // can be admitted (true) or the session stops (false).
+    virtual bool wait_for_work() = 0;
+    // Graphs are resident (admitted, not yet finished).
+    virtual bool resident() = 0;
+};
+
+// Optional arguments of solve_t.
+struct SolveExtras {
+    void* ws = nullptr;         // workspace provided by the caller (else allocated)
+    bool noalloc = false;       // allocate nothing: caller's stream, no events or graphs
+    Feeder* feeder = nullptr;   // arena solve fed at relabels (ws: layout_for(n, slots + 1, true))
+    uint32_t slots = 0;         // graph slots of the arena
+};
+
+// X.noalloc: the solve allocates nothing (no device or pinned memory,
+// streams, events or graphs).  X.ws is then a workspace of at least
 // layout_for(n, 0).total bytes, the kernels run on the caller's stream, the
 // host reads the control block into thread-local pageable memory, and every
 // wait is a stream synchronisation.
+// X.feeder: an arena solve (kB): n is the arena's node capacity, the graphs
+// arrive through the feeder at global relabels and leave at the relabel that
+// finds them without an active vertex; d_nbr, d_res, d_term and partition are
+// unused (the arena holds the rows, the feeder writes each graph's results).
 template <bool kB>
 void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,
              bool partition_on_device, const wtv_params& prm, wtv_stats& st, void* stream_v, const BatchArgs* B,
-             void* fixed_ws = nullptr) {
+             const SolveExtras& X = SolveExtras{}) {
     using Clock = std::chrono::steady_clock;
     std::memset(&st, 0, sizeof(st));
     if (!n) return;
-    const bool noalloc = fixed_ws != nullptr;
+    const bool noalloc = X.noalloc;
+    Feeder* const feeder = kB ? X.feeder : nullptr;
     if (noalloc && prm.profile != 0)
         fail(WTV_EINVAL, std::string(kPrefix) + "profile mode needs events: not available without allocation");
-    if (!d_nbr || !d_res || !d_term || !partition) fail(WTV_EINVAL, std::string(kPrefix) + "null graph pointer");
+    if (!feeder && (!d_nbr || !d_res || !d_term || !partition))
+        fail(WTV_EINVAL, std::string(kPrefix) + "null graph pointer");
     if (n > 0x7FFFFFFFu) fail(WTV_EINVAL, std::string(kPrefix) + ">INT32_MAX reduced nodes unsupported");
     int local_steps = prm.local_steps;
     if (local_steps < 1) local_steps = 1;
@@ -1198,9 +1352,14 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     const int wg = prm.wg_size == 0 ? 256 : prm.wg_size;
     if (wg != 256 && wg != 512 && wg != 1024)
         fail(WTV_EINVAL, std::string(kPrefix) + "wg_size must be 0 (256), 256, 512 or 1024");
-    const int sweep_grid = static_cast<int>(std::min<uint64_t>((static_cast<uint64_t>(n) + tpb - 1) / tpb, 8192));
+    // node bound of the sweeps: n, or an arena's high-water mark
+    uint32_t nn = feeder ? 0u : n;
+    auto sweep_grid_of = [&](uint32_t m) {
+        return std::max(1, static_cast<int>(std::min<uint64_t>((static_cast<uint64_t>(m) + tpb - 1) / tpb, 8192)));
+    };
+    int sweep_grid = sweep_grid_of(nn);
     // a level whose frontier holds more than n/32 vertices expands bottom-up
-    const uint32_t bu_threshold = std::max<uint32_t>(n / 32u, 4096u);
+    uint32_t bu_threshold = std::max<uint32_t>(nn / 32u, 4096u);
     hipStream_t cs = static_cast<hipStream_t>(stream_v);
     std::optional<ResLease> lease;
     if (!noalloc) lease.emplace();
@@ -1226,15 +1385,15 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     const char* trace_env = std::getenv("WTV_TRACE");
     const bool trace = trace_env && *trace_env && *trace_env != '0';
 
-    const uint32_t G = kB ? B->G : 0u;
-    const Layout L = layout_for(n, G);
+    const uint32_t G = kB ? (feeder ? X.slots + 1u : B->G) : 0u;
+    const Layout L = layout_for(n, G, feeder != nullptr);
     DeviceBuf ws;
-    if (!noalloc) ck(hipMalloc(&ws.p, L.total), "workspace malloc");
+    if (!X.ws) ck(hipMalloc(&ws.p, L.total), "workspace malloc");
     // every exit from here on, failures included, first waits for the kernels
     // still queued on s (a speculative round, a BFS batch), then frees ws
     DrainOnExit drain{s};
     st.solver_workspace_bytes = L.total;
-    char* base = static_cast<char*>(noalloc ? fixed_ws : ws.p);
+    char* base = static_cast<char*>(X.ws ? X.ws : ws.p);
     Ctl* ctl = reinterpret_cast<Ctl*>(base + L.ctl);
     long long* excess = reinterpret_cast<long long*>(base + L.excess);
     uint32_t* height = reinterpret_cast<uint32_t*>(base + L.height);
@@ -1248,6 +1407,16 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     unsigned long long* goff = kB ? reinterpret_cast<unsigned long long*>(base + L.goff) : nullptr;
     unsigned long long* gflow = kB ? reinterpret_cast<unsigned long long*>(base + L.gflow) : nullptr;
     uint8_t* galive = kB ? reinterpret_cast<uint8_t*>(base + L.galive) : nullptr;
+    ArenaView av{};
+    if (feeder) {
+        // the arena's own rows stand for the caller's arrays
+        d_res = reinterpret_cast<uint32_t*>(base + L.ares);
+        d_term = reinterpret_cast<int32_t*>(base + L.aterm);
+        nbr4 = reinterpret_cast<const int4*>(base + L.nbrg);
+        av = ArenaView{s, ctl, reinterpret_cast<int4*>(base + L.nbrg), reinterpret_cast<uint4*>(d_res), d_term,
+                       excess, height, rslot, gid, gn, gflow, galive, reinterpret_cast<uint8_t*>(base + L.gres),
+                       reinterpret_cast<uint8_t*>(base + L.gfresh), {wl[0], wl[1]}, n, X.slots, 0u};
+    }
 
     // pinned snapshots, or thread-local pageable ones without allocation
     thread_local Ctl tl_ctl[kCtlSnapshots];
@@ -1288,7 +1457,13 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
 
     ck(hipMemsetAsync(ctl, 0, sizeof(Ctl), s), "ctl clear");
     std::vector<uint32_t> h_gn;
-    if constexpr (kB) {
+    if (feeder) {
+        // no graph resident: every slot free, the sentinel never resident
+        k_fill_u8<<<1, kTPB, 0, s>>>(av.gres, G, 0u);
+        k_fill_u8<<<1, kTPB, 0, s>>>(av.gfresh, G, 0u);
+        k_fill_u8<<<1, kTPB, 0, s>>>(galive, G, 0u);
+        ck(hipGetLastError(), "arena tables");
+    } else if constexpr (kB) {
         h_gn.resize(G);
         for (uint32_t g = 0; g < G; ++g) h_gn[g] = static_cast<uint32_t>(B->h_off[g + 1] - B->h_off[g]);
         ck(hipMemcpyAsync(gn, h_gn.data(), G * 4u, hipMemcpyHostToDevice, s), "batch sizes");
@@ -1296,14 +1471,17 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         ck(hipMemsetAsync(gflow, 0, G * 8u, s), "batch flows");
     }
     ev_record(ev_t0);
-    if constexpr (kB) {
-        int4* nbrg = reinterpret_cast<int4*>(base + L.nbrg);
-        k_globalize<<<sweep_grid, tpb, 0, s>>>(n, G, goff, reinterpret_cast<const int4*>(d_nbr), nbrg, gid, ctl);
-        ck(hipGetLastError(), "globalize");
-        nbr4 = nbrg;
+    if (!feeder) {
+        if constexpr (kB) {
+            int4* nbrg = reinterpret_cast<int4*>(base + L.nbrg);
+            k_globalize<<<sweep_grid, tpb, 0, s>>>(n, G, goff, reinterpret_cast<const int4*>(d_nbr), nbrg, gid,
+                                                   ctl);
+            ck(hipGetLastError(), "globalize");
+            nbr4 = nbrg;
+        }
+        k_prepare<<<sweep_grid, tpb, 0, s>>>(0u, n, nbr4, d_term, rslot, excess, wl[0], &ctl->wl_cnt[0], ctl);
+        ck(hipGetLastError(), "prepare");
     }
-    k_prepare<<<sweep_grid, tpb, 0, s>>>(n, nbr4, d_term, rslot, excess, wl[0], ctl);
-    ck(hipGetLastError(), "prepare");
     check_err("prepare");
 
     uint64_t sub = 0;  // sub-rounds run (statistics)
@@ -1322,15 +1500,19 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         uint32_t* bf[2] = {rl, wl[p ^ 1u]};
         k_bfs_begin<<<1, 1, 0, s>>>(ctl, p);
         ck(hipGetLastError(), "bfs begin");
-        const bool skip = kB && skip_done && st.global_relabels > 0;
+        // an arena relabels every resident graph: its seed sweep skips the
+        // tiles of free nodes and writes every height of fresh graphs
+        const bool skip = kB && !feeder && skip_done && st.global_relabels > 0;
         if (skip) {
             k_clear_alive<<<std::max(1, static_cast<int>(std::min<uint64_t>((G + tpb - 1) / tpb, 1024))), tpb, 0,
                             s>>>(G, galive);
             k_mark_alive<<<grid, tpb, 0, s>>>(p, wl[p], gid, galive, ctl);
             ck(hipGetLastError(), "mark alive");
         }
-        k_bfs_seed<<<sweep_grid, tpb, 0, s>>>(n, d_term, nbr4, d_res, height, nzm, bf[0],
-                                              st.global_relabels == 0 ? 1u : 0u, gid, skip ? galive : nullptr, ctl);
+        k_bfs_seed<<<sweep_grid, tpb, 0, s>>>(nn, d_term, nbr4, d_res, height, nzm, bf[0],
+                                              st.global_relabels == 0 ? 1u : 0u, gid,
+                                              feeder ? av.gres : (skip ? galive : nullptr),
+                                              feeder ? av.gfresh : nullptr, ctl);
         ck(hipGetLastError(), "bfs seed");
         // Levels run in batches of 8 launches.  Batches are pipelined: batch
         // b+1 is enqueued before the frontier count after batch b is read, and
@@ -1349,7 +1531,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
                 kt.begin(KernelTimer::kLevel);
                 launch_wg(wg, [&](auto t) {
                     constexpr int T = decltype(t)::value;
-                    k_bfs_level<T><<<grid, T, 0, s>>>(n, d, bu_threshold, bf[d & 1u], bf[(d + 1u) & 1u], nbr4,
+                    k_bfs_level<T><<<grid, T, 0, s>>>(nn, d, bu_threshold, bf[d & 1u], bf[(d + 1u) & 1u], nbr4,
                                                       rslot, nzm, height, ctl);
                 });
                 kt.end();
@@ -1412,9 +1594,28 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         return hctl->wl_cnt[p ^ 1u];
     };
 
-    global_relabel(false);
-    check_err("global relabel");
-    uint32_t active = filter_active();
+    // An arena admits graphs before a relabel (which sets their heights) and
+    // finishes, after the relabel and the filter, every graph left without
+    // an active vertex: that relabel's BFS ran on its final residual graph.
+    auto relabel_and_filter = [&](bool skip_done) -> uint32_t {
+        if (feeder) {
+            feeder->admit(av, static_cast<uint32_t>(pos & 1u));
+            nn = av.hwm;
+            sweep_grid = sweep_grid_of(nn);
+            bu_threshold = std::max<uint32_t>(nn / 32u, 4096u);
+        }
+        global_relabel(skip_done);
+        const uint32_t act = filter_active();
+        if (feeder) feeder->finish(av, static_cast<uint32_t>(pos & 1u));
+        return act;
+    };
+
+    uint32_t active = 0;
+    if (!feeder) {
+        global_relabel(false);
+        check_err("global relabel");
+        active = filter_active();
+    }
 
     int macro_period = 0;
     if (period > 0) macro_period = std::max(1, (period + local_steps - 1) / local_steps);
@@ -1425,7 +1626,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     uint64_t active_at_gr = active;
     // share of the vertices the last global relabel's BFS expanded
     uint64_t pops_at_gr = stat_sum(*hctl, kStBfsPops);
-    double last_gr_reach = n ? static_cast<double>(pops_at_gr) / static_cast<double>(n) : 0.0;
+    double last_gr_reach = nn ? static_cast<double>(pops_at_gr) / static_cast<double>(nn) : 0.0;
 
     // One macro round = local_steps sub-rounds (push + relabel).  Rounds are
     // replayed from a captured graph (one per starting parity) and pipelined:
@@ -1490,11 +1691,10 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     const uint32_t tail_cap = prm.tail_cap == 0 ? 512u : (prm.tail_cap < 0 ? 0u : static_cast<uint32_t>(prm.tail_cap));
     uint64_t relabels_now = relabels_at_gr;
     auto do_global_relabel = [&]() {
-        global_relabel(true);
-        active = filter_active();
+        active = relabel_and_filter(true);
         relabels_now = relabels_at_gr = stat_sum(*hctl, kStRelabels);
         const uint64_t pops = stat_sum(*hctl, kStBfsPops);
-        last_gr_reach = n ? static_cast<double>(pops - pops_at_gr) / static_cast<double>(n) : 0.0;
+        last_gr_reach = nn ? static_cast<double>(pops - pops_at_gr) / static_cast<double>(nn) : 0.0;
         pops_at_gr = pops;
         active_at_gr = active;
     };
@@ -1509,9 +1709,11 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         return std::chrono::duration<double, std::milli>(Clock::now() - last_gr_end).count() >= factor * last_gr_ms;
     };
     auto work_trigger = [&]() -> bool {
-        return alpha > 0.0 &&
-               static_cast<double>(relabels_now - relabels_at_gr) > alpha * static_cast<double>(active_at_gr) + 1024.0 &&
-               affordable();
+        return (alpha > 0.0 &&
+                static_cast<double>(relabels_now - relabels_at_gr) > alpha * static_cast<double>(active_at_gr) + 1024.0 &&
+                affordable()) ||
+               // requests waiting for an arena join at the next affordable relabel
+               (feeder && feeder->pending() && affordable());
     };
     auto check_device_error = [&](const Ctl& c) {
         if (c.error) {
@@ -1521,7 +1723,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         }
     };
 
-    const auto solve0 = Clock::now();
+    auto solve0 = Clock::now();
     // the wall-clock safety limit (gpu_push_relabel_fast.cu:538-544), checked
     // before every macro round the host decides on; the kernels still queued
     // finish before the failure unwinds (DrainOnExit)
@@ -1533,8 +1735,21 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             fail(WTV_ELIMIT, oss.str());
         }
     };
-    while (active) {
-        if (st.rounds >= max_rounds) fail(WTV_ELIMIT, std::string(kPrefix) + "max macro rounds exceeded");
+    uint64_t rounds0 = 0;  // an arena counts its limits per busy period
+    for (;;) {
+        if (!active) {
+            if (!feeder) break;
+            // an empty list: finish the resident graphs (all done) and admit
+            // what waits, or wait for requests when nothing is resident
+            if (!feeder->resident() && !feeder->wait_for_work()) break;
+            if (!feeder->resident()) {
+                solve0 = Clock::now();
+                rounds0 = st.rounds;
+            }
+            do_global_relabel();
+            continue;
+        }
+        if (st.rounds - rounds0 >= max_rounds) fail(WTV_ELIMIT, std::string(kPrefix) + "max macro rounds exceeded");
         check_time();
         if (active <= tail_cap) {
             // ---- one workgroup, sub-rounds back to back
@@ -1551,7 +1766,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
                 }
                 budget = static_cast<unsigned long long>(left) + 1ull;
             }
-            const uint64_t rounds_left = max_rounds - st.rounds;
+            const uint64_t rounds_left = max_rounds - (st.rounds - rounds0);
             const uint32_t max_sub = static_cast<uint32_t>(
                 std::min<uint64_t>(4096ull, rounds_left * static_cast<uint64_t>(local_steps)));
             const auto tail0 = Clock::now();
@@ -1589,7 +1804,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         Pending prev = snapshot();
         for (;;) {
             // speculative next round (a no-op on the device if prev drained the list)
-            const bool spec = st.rounds < max_rounds;
+            const bool spec = st.rounds - rounds0 < max_rounds;
             Pending cur{};
             if (spec) {
                 launch_round();
@@ -1631,6 +1846,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     }
     check_err("solve completion");
     st.solve_seconds = std::chrono::duration<double>(Clock::now() - solve0).count();
+    if (feeder) return;  // the session stops: every graph was finished at a relabel
 
     const auto fr0 = Clock::now();
     global_relabel(false);  // every graph: the labels come from this BFS
@@ -1752,8 +1968,10 @@ void pr_solve_noalloc(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_te
         fail(WTV_EINVAL, oss.str());
     }
     if (layout_for(n, 0).total > kScratchBytes) fail(WTV_EINVARIANT, std::string(kPrefix) + "scratch layout");
-    void* ws = scratch_of_current_device();
-    solve_t<false>(n, d_nbr, d_res, d_term, partition, partition_on_device, prm, st, stream, nullptr, ws);
+    SolveExtras X;
+    X.ws = scratch_of_current_device();
+    X.noalloc = true;
+    solve_t<false>(n, d_nbr, d_res, d_term, partition, partition_on_device, prm, st, stream, nullptr, X);
 }
 
 void batch_solve(uint32_t G, const uint64_t* h_off, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
@@ -1774,4 +1992,274 @@ void batch_solve(uint32_t G, const uint64_t* h_off, const int32_t* d_nbr, uint32
                   stream, &B);
 }
 
+
+// ------------------------------------------------------------ serving sessions
+namespace {
+
+constexpr uint32_t kArenaAlign = 1024u;  // k_bfs_seed's tile: kTPB * kSeedUnroll
+static_assert(kArenaAlign == kTPB * kSeedUnroll, "arena ranges align to the seed sweep's tiles");
+
+inline uint32_t align_nodes(uint32_t n) { return (n + kArenaAlign - 1u) / kArenaAlign * kArenaAlign; }
+
+// Continuous batching on one arena: requests join at global relabels, when
+// they fit the free node ranges and graph slots, and each graph leaves at
+// the first relabel after which it has no active vertex (that relabel's BFS
+// ran on its final residual graph, so its partition is final): its
+// partition, residuals and terminals go back to the caller's buffers and its
+// node range becomes inert.  Graphs share no arcs and keep their own source
+// heights, so each result equals a lone solve of the same graph.
+class ArenaSession final : public Feeder {
+  public:
+    ArenaSession(SessionQueue& q, uint32_t capacity, uint32_t slots, int32_t worker)
+        : q_(q), capacity_(capacity), slots_(slots), worker_(worker) {
+        free_[0] = capacity;
+        for (uint32_t g = slots; g > 0; --g) free_slots_.push_back(g - 1u);
+    }
+
+    void admit(ArenaView& a, uint32_t p) override {
+        a.hwm = hwm_;
+        if (alone_) return;  // a graph solved alone admits no other
+        std::vector<SessionJob> jobs = q_.take(largest_free(), static_cast<uint32_t>(free_slots_.size()),
+                                               !res_.empty());
+        for (size_t i = 0; i < jobs.size(); ++i) {
+            try {
+                admit_one(a, p, jobs[i]);
+            } catch (...) {
+                // the requests taken with it are not lost: back to the queue
+                const std::vector<SessionJob> rest(jobs.begin() + static_cast<std::ptrdiff_t>(i) + 1, jobs.end());
+                if (!rest.empty()) q_.requeue_alone(rest);
+                throw;
+            }
+        }
+        a.hwm = hwm_;
+    }
+
+    void finish(ArenaView& a, uint32_t p) override {
+        if (res_.empty()) return;
+        const uint32_t G = slots_ + 1u;
+        const int gg = std::max(1, static_cast<int>(std::min<uint32_t>((G + kTPB - 1u) / kTPB, 1024u)));
+        k_clear_alive<<<gg, kTPB, 0, a.s>>>(G, a.galive);
+        k_mark_alive<<<1024, kTPB, 0, a.s>>>(p, a.wl[p], a.gid, a.galive, a.ctl);
+        k_fill_u8<<<gg, kTPB, 0, a.s>>>(a.gfresh, G, 0u);
+        ck(hipGetLastError(), "arena alive marks");
+        alive_.resize(G);
+        ck(hipMemcpyAsync(alive_.data(), a.galive, G, hipMemcpyDeviceToHost, a.s), "alive marks D2H");
+        ck(hipStreamSynchronize(a.s), "alive marks");
+        std::vector<size_t> done;
+        for (size_t i = 0; i < res_.size(); ++i)
+            if (!alive_[res_[i].slot]) done.push_back(i);
+        if (done.empty()) return;
+        clear_violations(a);
+        for (size_t i : done) {
+            const Res& r = res_[i];
+            const uint32_t ng = r.job.req.n;
+            k_partition_range<<<grid_of(ng), kTPB, 0, a.s>>>(r.lo, ng, ng, a.height, a.excess,
+                                                             r.job.req.d_partition, a.ctl);
+            ck(hipMemcpyAsync(r.job.req.d_res, a.res + r.lo, static_cast<size_t>(ng) * 16u,
+                              hipMemcpyDeviceToDevice, a.s), "results D2D");
+            ck(hipMemcpyAsync(r.job.req.d_term, a.term + r.lo, static_cast<size_t>(ng) * 4u,
+                              hipMemcpyDeviceToDevice, a.s), "results D2D");
+        }
+        ck(hipGetLastError(), "arena partition");
+        flows_.resize(G);
+        ck(hipMemcpyAsync(flows_.data(), a.gflow, static_cast<size_t>(G) * 8u, hipMemcpyDeviceToHost, a.s),
+           "flows D2H");
+        uint32_t viol = 0;
+        ck(hipMemcpyAsync(&viol, &a.ctl->violations, 4u, hipMemcpyDeviceToHost, a.s), "violations D2H");
+        ck(hipStreamSynchronize(a.s), "arena results");
+        const double t = q_.now();
+        for (size_t k = done.size(); k-- > 0;) {
+            const Res r = res_[done[k]];
+            const int status = viol ? WTV_EINVARIANT : WTV_OK;
+            q_.finish(r.job.ticket, status, static_cast<int64_t>(flows_[r.slot]), r.batch, worker_, r.start, t);
+            release(a, r);
+            res_.erase(res_.begin() + static_cast<std::ptrdiff_t>(done[k]));
+        }
+        if (res_.empty()) alone_ = false;
+        a.hwm = hwm_;
+    }
+
+    bool pending() override {
+        return !alone_ && q_.waiting(largest_free(), static_cast<uint32_t>(free_slots_.size()), !res_.empty());
+    }
+    bool wait_for_work() override { return q_.wait_nonempty(); }
+    bool resident() override { return !res_.empty(); }
+
+    // A shared solve failed: every resident request goes back to the queue to
+    // be solved alone (a request already alone fails with the error).
+    void abort(int code) {
+        std::vector<SessionJob> again;
+        const double t = q_.now();
+        for (const Res& r : res_) {
+            if (r.job.alone) q_.finish(r.job.ticket, code, 0, r.batch, worker_, r.start, t);
+            else again.push_back(r.job);
+        }
+        for (SessionJob& j : again) j.alone = true;
+        if (!again.empty()) q_.requeue_alone(again);
+        res_.clear();
+        free_.clear();
+        free_[0] = capacity_;
+        free_slots_.clear();
+        for (uint32_t g = slots_; g > 0; --g) free_slots_.push_back(g - 1u);
+        hwm_ = 0;
+        alone_ = false;
+    }
+
+  private:
+    struct Res {
+        SessionJob job;
+        uint32_t slot, lo, ag;
+        int32_t batch;  // graphs resident once it joined, itself included
+        double start;
+    };
+
+    static int grid_of(uint32_t m) {
+        return std::max(1, static_cast<int>(std::min<uint64_t>((static_cast<uint64_t>(m) + kTPB - 1) / kTPB, 8192)));
+    }
+
+    uint64_t largest_free() const {
+        uint64_t best = 0;
+        for (const auto& f : free_) best = std::max<uint64_t>(best, f.second);
+        return best;
+    }
+
+    // first fit of `ag` nodes; the caller checked it fits
+    uint32_t carve(uint32_t ag) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < ag) continue;
+            const uint32_t lo = it->first, len = it->second;
+            free_.erase(it);
+            if (len > ag) free_[lo + ag] = len - ag;
+            return lo;
+        }
+        fail(WTV_EINVARIANT, std::string(kPrefix) + "arena range lost");
+    }
+
+    void clear_violations(ArenaView& a) { ck(hipMemsetAsync(&a.ctl->violations, 0, 4u, a.s), "violations clear"); }
+
+    int32_t read_error(ArenaView& a, const char* where) {
+        int32_t err = 0;
+        ck(hipMemcpyAsync(&err, &a.ctl->error, 4u, hipMemcpyDeviceToHost, a.s), where);
+        ck(hipStreamSynchronize(a.s), where);
+        return err;
+    }
+
+    void admit_one(ArenaView& a, uint32_t p, const SessionJob& j) {
+        const uint32_t ng = j.req.n;
+        const double t = q_.now();
+        if (ng == 0) {
+            q_.finish(j.ticket, WTV_OK, 0, static_cast<int32_t>(res_.size()) + 1, worker_, t, t);
+            return;
+        }
+        const uint32_t ag = align_nodes(ng);
+        const uint32_t lo = carve(ag);
+        const uint32_t g = free_slots_.back();
+        free_slots_.pop_back();
+        if (j.ready) ck(hipStreamWaitEvent(a.s, static_cast<hipEvent_t>(j.ready), 0), "ready wait");
+        // an error of the discharge so far belongs to the shared solve, not
+        // to this request: the solve fails and its graphs are solved alone
+        if (const int32_t e0 = read_error(a, "admit check")) {
+            free_slots_.push_back(g);
+            free_[lo] = ag;  // back to the free list (merged at the session's reset)
+            q_.requeue_alone({j});
+            std::ostringstream oss;
+            oss << kPrefix << "device invariant failed at discharge code=" << e0;
+            fail(WTV_EINVARIANT, oss.str());
+        }
+        k_admit<<<grid_of(ag), kTPB, 0, a.s>>>(lo, ng, ag, g, reinterpret_cast<const int4*>(j.req.d_nbr), a.nbrg,
+                                               a.res, a.term, a.gid, a.ctl);
+        ck(hipMemcpyAsync(a.res + lo, j.req.d_res, static_cast<size_t>(ng) * 16u, hipMemcpyDeviceToDevice, a.s),
+           "admit res");
+        ck(hipMemcpyAsync(a.term + lo, j.req.d_term, static_cast<size_t>(ng) * 4u, hipMemcpyDeviceToDevice, a.s),
+           "admit term");
+        k_set_slot<<<1, 1, 0, a.s>>>(g, ng, a.gn, a.gflow, a.gres, a.gfresh, 1u);
+        // validate first (ids in range, reverse arcs): a bad request fails
+        // alone, before any of its vertices joins the active list
+        k_prepare<<<grid_of(ag), kTPB, 0, a.s>>>(lo, lo + ag, a.nbrg, a.term, a.rslot, a.excess, nullptr, nullptr,
+                                                 a.ctl);
+        ck(hipGetLastError(), "admit");
+        const int32_t err = read_error(a, "admit check");
+        Res r{j, g, lo, ag, static_cast<int32_t>(res_.size()) + 1, t};
+        hwm_ = std::max(hwm_, lo + ag);
+        if (err) {
+            ck(hipMemsetAsync(&a.ctl->error, 0, 4u, a.s), "error clear");  // it was this request's
+            q_.finish(j.ticket, WTV_EINVARIANT, 0, r.batch, worker_, t, q_.now());
+            release(a, r);
+            return;
+        }
+        k_prepare<<<grid_of(ag), kTPB, 0, a.s>>>(lo, lo + ag, a.nbrg, a.term, a.rslot, a.excess, a.wl[p],
+                                                 &a.ctl->wl_cnt[p], a.ctl);
+        ck(hipGetLastError(), "admit supply");
+        res_.push_back(r);
+        if (j.alone) alone_ = true;
+    }
+
+    // the graph's nodes become inert and its range and slot free
+    void release(ArenaView& a, const Res& r) {
+        k_inert<<<grid_of(r.ag), kTPB, 0, a.s>>>(r.lo, r.lo + r.ag, a.nbrg, a.term, a.excess, a.height, a.gid,
+                                                 slots_);
+        k_set_slot<<<1, 1, 0, a.s>>>(r.slot, 0u, a.gn, a.gflow, a.gres, a.gfresh, 0u);
+        ck(hipGetLastError(), "release");
+        free_slots_.push_back(r.slot);
+        // merge the range with its free neighbours
+        uint32_t lo = r.lo, len = r.ag;
+        auto nx = free_.lower_bound(lo);
+        if (nx != free_.end() && nx->first == lo + len) {
+            len += nx->second;
+            nx = free_.erase(nx);
+        }
+        if (nx != free_.begin()) {
+            auto pv = std::prev(nx);
+            if (pv->first + pv->second == lo) {
+                lo = pv->first;
+                len += pv->second;
+                free_.erase(pv);
+            }
+        }
+        free_[lo] = len;
+        hwm_ = 0;
+        for (const Res& x : res_)
+            if (x.slot != r.slot) hwm_ = std::max(hwm_, x.lo + x.ag);
+    }
+
+    SessionQueue& q_;
+    const uint32_t capacity_, slots_;
+    const int32_t worker_;
+    std::map<uint32_t, uint32_t> free_;  // free node ranges: start -> length
+    std::vector<uint32_t> free_slots_;
+    std::vector<Res> res_;
+    std::vector<uint8_t> alive_;
+    std::vector<unsigned long long> flows_;
+    uint32_t hwm_ = 0;
+    bool alone_ = false;
+};
+
+}  // namespace
+
+uint64_t arena_bytes_per_node() {
+    // rows 36 B, workspace 26 B, graph id 4 B, union ids 16 B ... as layout_for
+    return layout_for(1u << 20, 2u, true).total / (1u << 20);
+}
+
+void session_serve(SessionQueue& q, const wtv_params& prm, uint32_t capacity, uint32_t slots, void* stream,
+                   int32_t worker) {
+    capacity = std::max(kArenaAlign, capacity / kArenaAlign * kArenaAlign);
+    const Layout L = layout_for(capacity, slots + 1u, true);
+    DeviceBuf ws;
+    ck(hipMalloc(&ws.p, L.total), "arena malloc");
+    ArenaSession sess(q, capacity, slots, worker);
+    SolveExtras X;
+    X.ws = ws.p;
+    X.feeder = &sess;
+    X.slots = slots;
+    for (;;) {
+        wtv_stats st{};
+        try {
+            solve_t<true>(capacity, nullptr, nullptr, nullptr, nullptr, true, prm, st, stream, nullptr, X);
+            return;  // the queue stopped with nothing resident
+        } catch (const Error& e) {
+            sess.abort(e.code);
+        }
+    }
+}
 }  // namespace wtv
