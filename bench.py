@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--workload", choices=("delaunay", "kuhn235"), default="delaunay",
                     help="headline graph: the 12M-point Delaunay mesh, or the Kuhn 235^3 grid")
     ap.add_argument("--points", type=int, default=12_000_000, help="Delaunay headline: points")
+    ap.add_argument("--tet-order", choices=("shuffled", "native"), default="shuffled",
+                    help="Delaunay graphs: tets shuffled (as an exporter in container order gives them) or in the "
+                         "tetrahedraliser's spatial order")
     ap.add_argument("--cpu-sample-points", type=int, default=2_000_000,
                     help="Delaunay headline: points of the bounded sample the CPU baseline solves")
     ap.add_argument("--k", type=int, default=235, help="Kuhn grid size (235 -> 77.87M tets)")
@@ -171,9 +174,10 @@ def main():
     # ---------------------------------------------------------- workload
     t = time.time()
     if a.workload == "delaunay":
-        g, meta = delaunay_graph(a.points, a.seed, torch, np, S, dev)
+        g, meta = delaunay_graph(a.points, a.seed, torch, np, S, dev, a.tet_order)
     else:
         g, meta = kuhn_graph(a.k, a.seed, torch, np, S, dev)
+    native_host = meta.pop("native_order_graph", None)
     N = int(meta["tets"])
     nbr, res0, term0 = g["nbr"], g["res"], g["term"]
     res = res0.clone()
@@ -210,6 +214,30 @@ def main():
             f"sub={s['sub_rounds']} relabels(global)={s['global_relabels']} flow={s['raw_flow']}")
     assert len(set(flows)) == 1, f"flows differ between steps: {flows}"
     raw_flow = flows[0]
+    # the same graph in the tetrahedraliser's spatial order, timed the same way
+    native_order = None
+    if native_host is not None:
+        dn = {k: torch.from_numpy(native_host[k]).to(dev) for k in ("nbr", "res", "term")}
+        rn, tn = dn["res"].clone(), dn["term"].clone()
+        pn = torch.empty(N, dtype=torch.uint8, device=dev)
+
+        def step_native():
+            rn.copy_(dn["res"])
+            tn.copy_(dn["term"])
+            return W.pr_solve(N, dn["nbr"].data_ptr(), rn.data_ptr(), tn.data_ptr(), pn.data_ptr(), 1, prm, stream)
+
+        step_native()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(a.steps):
+            stn = step_native()
+            assert stn.raw_flow == raw_flow, "the native-order solve's flow differs"
+        torch.cuda.synchronize()
+        native_order = {"ms_per_step": round((time.perf_counter() - t0) * 1e3 / a.steps, 3),
+                        "renumbered": int(stn.renumbered), "t_solve_ms_device": round(stn.t_solve_ms, 3),
+                        "note": "the same graph in the tetrahedraliser's canonical order (no renumbering needed)"}
+        log(f"[bench] native tet order: {native_order['ms_per_step']:.1f} ms/step")
+        del dn, rn, tn, pn, native_host
     tuned = None
     if not a.no_tune:
         tuned = autotune_single(a, torch, W, N, nbr, res, term, res0, term0, part, stream, a.steps, raw_flow)
@@ -403,6 +431,8 @@ def main():
         "tets_per_s": round(N / value, 1),
         "autotuned": tuned,
         "t_solve_ms_device": round(s0["t_solve_ms"], 3),
+        "renumbered": int(s0["renumbered"]), "t_order_ms": round(s0["t_order_ms"], 3),
+        "native_order": native_order,
         "raw_flow": raw_flow,
         "cut_energy_fp64": energy,
         "certificate": "flow == integer cut capacity",
@@ -561,12 +591,38 @@ def kuhn_graph(k, seed, torch, np, S, dev):
     return g, meta
 
 
-def delaunay_graph(npts, seed, torch, np, S, dev):
+def shuffle_graph(g, seed, torch, np, dev):
+    """The graph with its tets in a random order, as a mesher that exports its
+    cells in container order hands them over (the permutation is built and
+    applied on the host with numpy).  Returns (graph dict on dev, the input
+    arrays on the host)."""
+    keys = ("nbr", "res", "term", "cap_f32", "term_f32")
+    h = {k: g[k].cpu().numpy() for k in keys}
+    n = h["term"].shape[0]
+    perm = np.random.default_rng(seed).permutation(n)
+    inv = np.empty(n, np.int64)
+    inv[perm] = np.arange(n)
+    nb = h["nbr"][perm]
+    nb = np.where(nb >= 0, inv[np.maximum(nb, 0)], -1).astype(np.int32)
+    assert nb.max() < n
+    out = dict(g)
+    out["nbr"] = torch.from_numpy(nb).to(dev)
+    for k in keys[1:]:
+        out[k] = torch.from_numpy(np.ascontiguousarray(h[k][perm])).to(dev)
+    return out, h
+
+
+def delaunay_graph(npts, seed, torch, np, S, dev, order="shuffled"):
     """Soft-seed graph of the Delaunay mesh of `npts` genus-3 points (95% on the
     surface with 0.5-voxel normal noise at resolution 1536 * sqrt(npts/12e6),
     5% uniform), snapped to the 2^-21 grid and tetrahedralised by the native
     tetrahedraliser on the host's cores; labels from the SDF at tet centroids
-    with 2% flips, lambda 10, h = 1 voxel.  Returns (graph dict, meta)."""
+    with 2% flips, lambda 10, h = 1 voxel.  order "shuffled": the tets in a
+    random order (seeded), not the tetrahedraliser's spatial one, so the
+    solver sees what an exporter in container order gives it; "native": the
+    tetrahedraliser's canonical order.  Returns (graph dict, meta); with
+    "shuffled", meta["native_order_graph"] holds the canonical arrays on the
+    host."""
     t0 = time.time()
     P, shape, res = S.genus3_point_cloud(npts, seed)
     Q, V = S.quantise_points(P)
@@ -586,13 +642,23 @@ def delaunay_graph(npts, seed, torch, np, S, dev):
     flips = torch.from_numpy(S.flip_mask(N, 0.02, seed + 1).astype(np.uint8)).to(dev)
     labels = (sdf >= 0).to(torch.uint8) ^ flips
     g = S.soft_graph_t(Vd, Td, Nd, labels, shape, 10.0, 1.0 / res)
+    inside = round(float((labels == 0).float().mean()), 4)
+    del Vd, Td, Nd, sdf, flips, labels
+    native = None
+    if order == "shuffled":
+        g, native = shuffle_graph(g, seed + 7, torch, np, dev)
     meta = {"workload": f"delaunay_genus3_{npts // 1_000_000 if npts >= 1_000_000 else npts}"
-                        f"{'m' if npts >= 1_000_000 else ''}pts_softseed_lambda10",
+                        f"{'m' if npts >= 1_000_000 else ''}pts_softseed_lambda10"
+                        f"{'_shuffled' if order == 'shuffled' else ''}",
+            "tet_order": ("shuffled: a seeded random permutation of the tets (numpy), as an exporter in container "
+                          "order hands them over; the solver renumbers for locality itself"
+                          if order == "shuffled" else "the tetrahedraliser's canonical (spatial) order"),
             "points": npts, "tets": N, "repeated_points": rep, "voxel_res": round(res, 2), "h_voxels": 1.0,
             "mesher": "native Delaunay (csrc/wtv_delaunay.cpp, canonical tet order)",
             "mesh_seconds": round(t2 - t1, 2), "mesh_threads": threads, "points_seconds": round(t1 - t0, 2),
-            "inside_frac": round(float((labels == 0).float().mean()), 4)}
-    del Vd, Td, Nd, sdf, flips, labels
+            "inside_frac": inside}
+    if native is not None:
+        meta["native_order_graph"] = native
     return g, meta
 
 
@@ -604,7 +670,8 @@ def cpu_sample_leg(a, torch, np, W, S, dev, stream, n_full):
     The BK time is reported per solve, scaled by tet count to the headline."""
     import oracle
     oracle.build()
-    g, meta = delaunay_graph(a.cpu_sample_points, a.seed + 5, torch, np, S, dev)
+    g, meta = delaunay_graph(a.cpu_sample_points, a.seed + 5, torch, np, S, dev, a.tet_order)
+    meta.pop("native_order_graph", None)
     N = int(meta["tets"])
     nbr, res0, term0 = g["nbr"], g["res"], g["term"]
     res, term = res0.clone(), term0.clone()

@@ -89,6 +89,10 @@ typedef struct wtv_stats {
                                    measured on the solver's stream with an empty
                                    kernel of the push launch shape; subtract it
                                    from t_push_ms / n_push for the kernel time  */
+    double t_order_ms;          /* the locality test and, when the graph was
+                                   renumbered, the renumbering and the copy
+                                   back (host wall time)                    */
+    uint64_t renumbered;        /* 1: solved as a locality-renumbered copy   */
 } wtv_stats;
 
 /* Solver knobs.  global_relabel_period, max_rounds and local_steps keep the
@@ -115,6 +119,11 @@ typedef struct wtv_params {
                                       + 1024 and the time since it has reached
                                       its duration; 0 = default 2.0, < 0 = only
                                       the reference period                    */
+    int32_t renumber;              /* single-graph solves: 0 = renumber the nodes
+                                      for locality when fewer than half of a
+                                      sample of arcs join ids < 4096 apart
+                                      (graphs of >= 2^20 nodes), 1 = always,
+                                      -1 = never; results are the same      */
 } wtv_params;
 
 /* Fill *p with the reference defaults. */

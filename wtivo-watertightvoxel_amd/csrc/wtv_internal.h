@@ -24,10 +24,15 @@ struct Error : std::runtime_error {
 // callers that match on it keep working.
 constexpr const char* kPrefix = "GPUPr-FAST-v6.30 ";
 
-// Solve a reduced graph already resident on the device (wtv_solver.hip).
+// Solve a reduced graph already resident on the device, renumbering a
+// disordered graph for locality first (wtv_order.hip).
 void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
               uint8_t* partition, bool partition_on_device, const wtv_params& p,
               wtv_stats& st, void* stream);
+
+// The solve of the graph in the order given (wtv_solver.hip).
+void pr_solve_in_order(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,
+                       bool partition_on_device, const wtv_params& p, wtv_stats& st, void* stream);
 
 // pr_solve with its workspace in a file-scope device array: no allocation of
 // any kind, n <= WTV_NOALLOC_MAX_NODES, kernels on the caller's stream;

@@ -1918,8 +1918,8 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     st.t_event_overhead_us = kt.on ? event_pair_overhead_us(grid, wg, s) : 0.0;
 }
 
-void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,
-              bool partition_on_device, const wtv_params& prm, wtv_stats& st, void* stream) {
+void pr_solve_in_order(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint8_t* partition,
+                       bool partition_on_device, const wtv_params& prm, wtv_stats& st, void* stream) {
     solve_t<false>(n, d_nbr, d_res, d_term, partition, partition_on_device, prm, st, stream, nullptr);
 }
 

@@ -32,7 +32,7 @@ LIB = os.path.join(HERE, "libwtv_gpupr.so")
 DELAUNAY_LIB = os.path.join(HERE, "libwtv_delaunay.so")
 EXT = os.path.join(HERE, "wtivo_gpupr" + sysconfig.get_config_var("EXT_SUFFIX"))
 
-HIP_SOURCES = ["wtv_solver.hip", "wtv_flow.hip", "wtv_graph_build.hip", "wtv_topology.hip", "wtv_udf.hip"]
+HIP_SOURCES = ["wtv_solver.hip", "wtv_order.hip", "wtv_flow.hip", "wtv_graph_build.hip", "wtv_topology.hip", "wtv_udf.hip"]
 HOST_SOURCES = ["wtv_server.cpp", "wtv_upload.cpp", "wtv_capi.cpp"]
 
 # -packed-fp32-ops: no v_pk_*_f32 generation (the kernels are integer work;
