@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--serve-sweep-requests", type=int, default=400)
     ap.add_argument("--serve-p99-ms", type=float, default=100.0, help="serving leg: p99 latency target")
     ap.add_argument("--no-standin-leg", action="store_true", help="skip the Kuhn 235^3 leg")
+    ap.add_argument("--no-small-legs", action="store_true",
+                    help="skip the Kuhn 32^3 and Kuhn 150^3 integer-stress legs (GPU and CPU medians of 5)")
     ap.add_argument("--delaunay-points", type=int, default=0,
                     help="Delaunay leg: genus-3 point-cloud size (0 = skip)")
     ap.add_argument("--no-tune", action="store_true",
@@ -410,6 +412,10 @@ def main():
     if a.serve_requests > 0:
         serving = serve_leg(a, torch, np, W, S, dev, stream)
 
+    small = None
+    if not a.no_small_legs and not a.no_cpu:
+        small = small_configs_leg(a, torch, np, W, S, dev, stream)
+
     value = ms_per_step / 1e3
     line = {
         "metric": "cut solve s @78M tets",
@@ -463,6 +469,7 @@ def main():
         "delaunay": delaunay,
         "batched": batched,
         "serving": serving,
+        "small_configs": small,
         "gen_seconds": round(gen_s, 2),
     }
     print(json.dumps(line), flush=True)
@@ -671,7 +678,7 @@ def cpu_sample_leg(a, torch, np, W, S, dev, stream, n_full):
     import oracle
     oracle.build()
     g, meta = delaunay_graph(a.cpu_sample_points, a.seed + 5, torch, np, S, dev, a.tet_order)
-    meta.pop("native_order_graph", None)
+    native = meta.pop("native_order_graph", None)
     N = int(meta["tets"])
     nbr, res0, term0 = g["nbr"], g["res"], g["term"]
     res, term = res0.clone(), term0.clone()
@@ -700,16 +707,92 @@ def cpu_sample_leg(a, torch, np, W, S, dev, stream, n_full):
     pr_s = time.perf_counter() - tc
     assert pr.raw_flow == st.raw_flow and np.array_equal(pr.partition, part_h), \
         "CPU sample: restated push-relabel differs from the GPU"
+    bk_native_s = None
+    if native is not None:
+        # the same sample in the tetrahedraliser's spatial order
+        tc = time.perf_counter()
+        bn = oracle.bk_solve(native["nbr"], native["res"].view(np.uint32), native["term"])
+        bk_native_s = time.perf_counter() - tc
+        assert bn.raw_flow == st.raw_flow, "CPU sample: BK on the native order differs"
     scale = n_full / N
-    log(f"[bench] cpu sample: {N} tets, BK {bk_s:.2f}s, restated push-relabel {pr_s:.2f}s, GPU {gpu_s * 1e3:.1f} ms")
+    log(f"[bench] cpu sample: {N} tets, BK {bk_s:.2f}s (native order {bk_native_s}), restated push-relabel "
+        f"{pr_s:.2f}s, GPU {gpu_s * 1e3:.1f} ms")
     return {"value": round(bk_s * scale, 3), "unit": "s", "cores": 1, "kind": "port",
             "sample": f"Boykov-Kolmogorov (oracle/wtv_oracle.c), 1 thread, on the Delaunay graph of "
                       f"{a.cpu_sample_points} points of the same generator ({N} tets, {meta['repeated_points']} "
                       f"repeated points): {bk_s:.3f} s, scaled by tet count x{scale:.3f} to the headline's "
                       f"{n_full} tets; flow and labels equal to the GPU's on the sample",
             "sample_tets": N, "sample_bk_s": round(bk_s, 3), "sample_restated_push_relabel_s": round(pr_s, 3),
+            "sample_tet_order": meta.get("tet_order"),
+            "sample_bk_native_order_s": round(bk_native_s, 3) if bk_native_s is not None else None,
             "sample_gpu_s": round(gpu_s, 5), "sample_speedup_vs_bk": round(bk_s / gpu_s, 1),
             "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
+
+
+def small_configs_leg(a, torch, np, W, S, dev, stream):
+    """The plain single solve on the 196,608-tet Kuhn 32^3 sphere with 3 holes
+    (soft seeds, lambda 10, 2% flips), and the adversarial integer stress on
+    the 20,250,000-tet Kuhn 150^3 thin shell with 40 tunnels (log-normal(2)
+    capacities quantised to [1, 2^20], lambda 100): GPU solves and the CPU
+    solvers (Boykov-Kolmogorov and the restated push-relabel, one core each)
+    each timed as the median of 5 after 1 warm-up, flows and labels equal."""
+    import oracle
+    oracle.build()
+
+    def graphs():
+        m = S.kuhn_mesh(32, seed=20260000)
+        shape = S.sphere_with_holes(20260000)
+        lab = S.labels_from_sdf(S.tet_centroids_sdf(m, shape, device=dev), 0.02, seed=20260001)
+        g = S.soft_graph(m, lab, shape, 10.0, 1.0 / 32, device=dev)
+        yield "kuhn32_sphere3holes_softseed", g.nbr, g.res, g.term
+        k = 150
+        V, T, Nb = S.kuhn_mesh_t(k, seed=20260004, device=dev)
+        shape = S.thin_shell_with_tunnels(20260004, 1.0 / k, tunnels=40)
+        lab = S.labels_from_sdf(S.tet_sdf_t(V, T, shape).cpu().numpy(), 0.0)
+        m = S.TetMesh(k, V.cpu().numpy(), T.cpu().numpy(), Nb.cpu().numpy())
+        del V, T, Nb
+        g = S.lognormal_int_graph(m, lab, 100.0, seed=20260005, rev_slots=S.KUHN_REV_SLOT)
+        yield "kuhn150_thin_shell_40_tunnels_integer", g.nbr, g.res, g.term
+
+    def median5(fn):
+        fn()
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            out = fn()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[2], out
+
+    out = {}
+    prm = W.params(global_relabel_period=a.period, local_steps=a.local_steps, relabel_alpha=a.alpha)
+    for name, nbr_h, res_h, term_h in graphs():
+        n = int(term_h.shape[0])
+        dn = torch.from_numpy(np.ascontiguousarray(nbr_h)).to(dev)
+        r0 = torch.from_numpy(np.ascontiguousarray(res_h).view(np.int32)).to(dev)
+        t0_ = torch.from_numpy(np.ascontiguousarray(term_h)).to(dev)
+        r, t = r0.clone(), t0_.clone()
+        part = torch.empty(n, dtype=torch.uint8, device=dev)
+
+        def gpu():
+            r.copy_(r0)
+            t.copy_(t0_)
+            st = W.pr_solve(n, dn.data_ptr(), r.data_ptr(), t.data_ptr(), part.data_ptr(), 1, prm, stream)
+            torch.cuda.synchronize()
+            return st
+
+        g_s, st = median5(gpu)
+        p_h = part.cpu().numpy()
+        bk_s, bk = median5(lambda: oracle.bk_solve(nbr_h, res_h, term_h))
+        pr_s, pr = median5(lambda: oracle.pr_solve(nbr_h, res_h, term_h))
+        assert bk.raw_flow == st.raw_flow == pr.raw_flow, f"{name}: flows differ"
+        assert np.array_equal(bk.partition, p_h) and np.array_equal(pr.partition, p_h), f"{name}: labels differ"
+        out[name] = {"tets": n, "gpu_s_median5": round(g_s, 5), "cpu_bk_s_median5": round(bk_s, 4),
+                     "cpu_restated_push_relabel_s_median5": round(pr_s, 4), "cores": 1,
+                     "speedup_vs_bk": round(bk_s / g_s, 1), "raw_flow": st.raw_flow, "identical": True}
+        log(f"[bench] {name}: {n} tets, GPU {g_s * 1e3:.2f} ms, BK {bk_s * 1e3:.1f} ms, "
+            f"restated push-relabel {pr_s * 1e3:.1f} ms (medians of 5)")
+        del dn, r0, t0_, r, t, part
+    return out
 
 
 def standin_leg(a, torch, np, W, S, dev, stream):
@@ -835,14 +918,22 @@ def cut_leg(a, torch, np, W, S, dev, stream):
         Th = T.cpu().numpy()
         Nh = Nb.cpu().numpy()
         tc = time.perf_counter()
-        r = oracle.reduce(Vh, Th, Nh, labh, 10.0)
+        r1 = oracle.reduce(Vh, Th, Nh, labh, 10.0)
+        cpu1_s = time.perf_counter() - tc
+        del r1
+        threads = min(16, os.cpu_count() or 1)
+        tc = time.perf_counter()
+        r = oracle.reduce(Vh, Th, Nh, labh, 10.0, threads=threads)
         cpu_s = time.perf_counter() - tc
         assert len(r.term) == F
         assert np.array_equal(r.nbr, rn[:F].cpu().numpy()), "cut leg: reduced neighbours differ from the CPU build"
         assert np.array_equal(r.res.view(np.int32), rr[:F].cpu().numpy()), "cut leg: capacities differ"
         assert np.array_equal(r.term, rt[:F].cpu().numpy()), "cut leg: terminals differ"
         out["cpu_build_s"] = round(cpu_s, 3)
-        out["cpu_build"] = "oracle/wtv_oracle.c restatement of the reduced-graph build, 1 thread; arrays equal"
+        out["cpu_build_threads"] = threads
+        out["cpu_build_1thread_s"] = round(cpu1_s, 3)
+        out["cpu_build"] = (f"oracle/wtv_oracle.c restatement of the reduced-graph build on {threads} threads "
+                            "(classify and build over tet ranges, ids serial, as the reference); arrays equal")
     log(f"[bench] graph_cut_fast: {N} tets, {F} free, wall {min(walls) * 1e3:.1f} ms "
         f"(solve {res.solve_seconds * 1e3:.1f} ms, build {res.build_upload_seconds * 1e3:.1f} ms); "
         f"resident {min(rwalls[1:]) * 1e3:.1f} ms")

@@ -80,6 +80,8 @@ def lib():
                                   C.POINTER(C.c_int32), C.POINTER(C.c_uint8), C.c_double,
                                   C.POINTER(C.c_int32), C.POINTER(C.c_uint32), C.POINTER(C.c_int32),
                                   C.POINTER(C.c_uint32), C.POINTER(C.c_int32), C.POINTER(ReduceInfo)]
+        L.wtvo_reduce_mt.restype = C.c_int
+        L.wtvo_reduce_mt.argtypes = list(L.wtvo_reduce.argtypes) + [C.c_int]
         L.wtvo_scatter.restype = None
         L.wtvo_scatter.argtypes = [C.c_size_t, C.POINTER(C.c_uint8), C.c_uint64, C.POINTER(C.c_uint32),
                                    C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]
@@ -129,7 +131,9 @@ class Reduced:
     info: dict
 
 
-def reduce(V, tets, nbrs, labels, fill_T) -> Reduced:
+def reduce(V, tets, nbrs, labels, fill_T, threads: int = 1) -> Reduced:
+    """Reduced-graph build (wtivo_gpupr_bindings.cpp:139-318) on `threads`
+    threads; the arrays do not depend on the thread count."""
     V = np.ascontiguousarray(V, dtype=np.float64)
     tets = np.ascontiguousarray(tets, dtype=np.int32)
     nbrs = np.ascontiguousarray(nbrs, dtype=np.int32)
@@ -141,9 +145,10 @@ def reduce(V, tets, nbrs, labels, fill_T) -> Reduced:
     rs = np.empty((max(N, 1), 4), np.uint32)
     tm = np.empty(max(N, 1), np.int32)
     info = ReduceInfo()
-    rc = lib().wtvo_reduce(N, _p(V, C.c_double), V.shape[0], _p(tets, C.c_int32), _p(nbrs, C.c_int32),
-                           _p(labels, C.c_uint8), float(fill_T), _p(f2f, C.c_int32), _p(fo, C.c_uint32),
-                           _p(nb, C.c_int32), _p(rs, C.c_uint32), _p(tm, C.c_int32), C.byref(info))
+    rc = lib().wtvo_reduce_mt(N, _p(V, C.c_double), V.shape[0], _p(tets, C.c_int32), _p(nbrs, C.c_int32),
+                              _p(labels, C.c_uint8), float(fill_T), _p(f2f, C.c_int32), _p(fo, C.c_uint32),
+                              _p(nb, C.c_int32), _p(rs, C.c_uint32), _p(tm, C.c_int32), C.byref(info),
+                              int(threads))
     if rc != OK:
         raise OracleError(rc, "reduce")
     n = int(info.free_n)

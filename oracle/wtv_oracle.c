@@ -59,6 +59,7 @@
  */
 #define _POSIX_C_SOURCE 199309L
 #include <math.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -135,35 +136,152 @@ typedef struct wtvo_reduce_info {
 /*
  * Caller provides full_to_free[N], free_old[N], nbr_r[4N], res[4N], term[N]
  * (only the first free_n / 4*free_n entries are written).
+ *
+ * Like the reference, classify and build run over contiguous ranges of tets
+ * on `threads` threads with per-range counters summed afterwards, and the
+ * reduced ids are assigned in one serial pass in tet order (bindings.cpp:
+ * 147-188 classify and ids, :253-294 build), so every output is the same for
+ * any thread count.  An error is reported for the first range (in tet
+ * order) that hits one.
  */
-int wtvo_reduce(size_t N, const double* V, size_t nv, const int32_t* tets,
-                const int32_t* nbrs, const uint8_t* labels, double fill_T,
-                int32_t* full_to_free, uint32_t* free_old, int32_t* nbr_r,
-                uint32_t* res, int32_t* term, wtvo_reduce_info* info)
+typedef struct reduce_job {
+    size_t lo, hi;
+    const double* V;
+    size_t nv;
+    const int32_t* tets;
+    const int32_t* nbrs;
+    const uint8_t* labels;
+    double fill_T;
+    size_t N;
+    int32_t* full_to_free;
+    int32_t* nbr_r;
+    uint32_t* res;
+    int32_t* term;
+    wtvo_reduce_info cnt; /* this range's counters */
+    int rc;
+} reduce_job;
+
+/* classify (bindings.cpp:153-171) */
+static void* reduce_classify(void* arg)
 {
-    memset(info, 0, sizeof(*info));
-    /* classify (bindings.cpp:153-171) */
-    for (size_t i = 0; i < N; ++i) {
+    reduce_job* J = (reduce_job*)arg;
+    for (size_t i = J->lo; i < J->hi; ++i) {
         int exterior = 0;
         for (int fi = 0; fi < 4; ++fi) {
-            const int32_t j = nbrs[i * 4u + (size_t)fi];
+            const int32_t j = J->nbrs[i * 4u + (size_t)fi];
             if (j < 0)
                 exterior = 1;
             else {
-                ++info->full_refs;
-                if ((size_t)j >= N) return WTVO_EINVAL; /* neighbor out of range */
+                ++J->cnt.full_refs;
+                if ((size_t)j >= J->N) {
+                    J->rc = WTVO_EINVAL; /* neighbor out of range */
+                    return NULL;
+                }
             }
         }
-        if (labels[i] == 0u) {
-            full_to_free[i] = -2;
-            ++info->fixed_source;
+        if (J->labels[i] == 0u) {
+            J->full_to_free[i] = -2;
+            ++J->cnt.fixed_source;
         } else if (exterior) {
-            full_to_free[i] = -3;
-            ++info->fixed_sink;
+            J->full_to_free[i] = -3;
+            ++J->cnt.fixed_sink;
         } else {
-            full_to_free[i] = -1;
-            ++info->free_n;
+            J->full_to_free[i] = -1;
+            ++J->cnt.free_n;
         }
+    }
+    return NULL;
+}
+
+/* build (bindings.cpp:253-294) */
+static void* reduce_build(void* arg)
+{
+    reduce_job* J = (reduce_job*)arg;
+    for (size_t i = J->lo; i < J->hi; ++i) {
+        const int32_t rr = J->full_to_free[i];
+        if (rr < 0) continue;
+        const size_t local = (size_t)rr;
+        int64_t src = 0, snk = 0;
+        const int32_t* tr = J->tets + i * 4u;
+        for (int fi = 0; fi < 4; ++fi) {
+            J->nbr_r[local * 4u + (size_t)fi] = -1;
+            J->res[local * 4u + (size_t)fi] = 0u;
+        }
+        for (int fi = 0; fi < 4; ++fi) {
+            const int32_t jj = J->nbrs[i * 4u + (size_t)fi];
+            if (jj < 0) { J->rc = WTVO_EINVAL; return NULL; } /* free tets have no hull face */
+            const size_t j = (size_t)jj;
+            const int32_t m = J->full_to_free[j];
+            int st = 0;
+            const int32_t cap = wtvo_capacity(J->V, J->nv, tr, fi, J->labels[i], J->labels[j], J->fill_T, &st);
+            if (st == 1) { J->rc = WTVO_EINVAL; return NULL; }
+            if (st == 2) { J->rc = WTVO_EOVERFLOW; return NULL; }
+            if (m >= 0) {
+                J->nbr_r[local * 4u + (size_t)fi] = m;
+                J->res[local * 4u + (size_t)fi] = (uint32_t)cap;
+                ++J->cnt.free_refs;
+            } else if (m == -2) {
+                src += cap;
+                ++J->cnt.terminal_edges;
+            } else if (m == -3) {
+                snk += cap;
+                ++J->cnt.terminal_edges;
+            } else {
+                J->rc = WTVO_EINVAL;
+                return NULL;
+            }
+        }
+        J->cnt.base_flow += (src < snk) ? src : snk;
+        const int64_t diff = src - snk;
+        if (diff < -2147483648LL || diff > 2147483647LL) { J->rc = WTVO_EINVAL; return NULL; }
+        J->term[local] = (int32_t)diff;
+    }
+    return NULL;
+}
+
+/* Runs fn over the jobs, jobs[1..] on threads of their own; the first
+ * nonzero rc in range order. */
+static int run_ranges(reduce_job* jobs, int nj, void* (*fn)(void*))
+{
+    pthread_t th[256];
+    int started = 0;
+    for (int t = 1; t < nj; ++t) {
+        if (pthread_create(&th[t], NULL, fn, &jobs[t]) != 0) break;
+        started = t;
+    }
+    fn(&jobs[0]);
+    for (int t = started + 1; t < nj; ++t) fn(&jobs[t]); /* any thread that did not start */
+    for (int t = 1; t <= started; ++t) pthread_join(th[t], NULL);
+    for (int t = 0; t < nj; ++t)
+        if (jobs[t].rc) return jobs[t].rc;
+    return WTVO_OK;
+}
+
+int wtvo_reduce_mt(size_t N, const double* V, size_t nv, const int32_t* tets,
+                   const int32_t* nbrs, const uint8_t* labels, double fill_T,
+                   int32_t* full_to_free, uint32_t* free_old, int32_t* nbr_r,
+                   uint32_t* res, int32_t* term, wtvo_reduce_info* info, int threads)
+{
+    memset(info, 0, sizeof(*info));
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    if ((size_t)threads > N / 4096u + 1u) threads = (int)(N / 4096u + 1u);
+    reduce_job jobs[256];
+    for (int t = 0; t < threads; ++t) {
+        reduce_job* J = &jobs[t];
+        memset(J, 0, sizeof(*J));
+        J->lo = N * (size_t)t / (size_t)threads;
+        J->hi = N * (size_t)(t + 1) / (size_t)threads;
+        J->V = V; J->nv = nv; J->tets = tets; J->nbrs = nbrs; J->labels = labels; J->fill_T = fill_T;
+        J->N = N; J->full_to_free = full_to_free; J->nbr_r = nbr_r; J->res = res; J->term = term;
+    }
+    int rc = run_ranges(jobs, threads, reduce_classify);
+    if (rc) return rc;
+    for (int t = 0; t < threads; ++t) {
+        info->full_refs += jobs[t].cnt.full_refs;
+        info->fixed_source += jobs[t].cnt.fixed_source;
+        info->fixed_sink += jobs[t].cnt.fixed_sink;
+        info->free_n += jobs[t].cnt.free_n;
     }
     if (info->full_refs & 1u) return WTVO_EINVAL; /* count not even (:177) */
     /* reduced ids in increasing tet order (:179-188) */
@@ -176,46 +294,23 @@ int wtvo_reduce(size_t N, const double* V, size_t nv, const int32_t* tets,
         }
     }
     if (info->free_n > 2147483647ull) return WTVO_EINVAL;
-    /* build (:253-294) */
-    for (size_t i = 0; i < N; ++i) {
-        const int32_t rr = full_to_free[i];
-        if (rr < 0) continue;
-        const size_t local = (size_t)rr;
-        int64_t src = 0, snk = 0;
-        const int32_t* tr = tets + i * 4u;
-        for (int fi = 0; fi < 4; ++fi) {
-            nbr_r[local * 4u + (size_t)fi] = -1;
-            res[local * 4u + (size_t)fi] = 0u;
-        }
-        for (int fi = 0; fi < 4; ++fi) {
-            const int32_t jj = nbrs[i * 4u + (size_t)fi];
-            if (jj < 0) return WTVO_EINVAL; /* free tets have no hull face */
-            const size_t j = (size_t)jj;
-            const int32_t m = full_to_free[j];
-            int st = 0;
-            const int32_t cap = wtvo_capacity(V, nv, tr, fi, labels[i], labels[j], fill_T, &st);
-            if (st == 1) return WTVO_EINVAL;
-            if (st == 2) return WTVO_EOVERFLOW;
-            if (m >= 0) {
-                nbr_r[local * 4u + (size_t)fi] = m;
-                res[local * 4u + (size_t)fi] = (uint32_t)cap;
-                ++info->free_refs;
-            } else if (m == -2) {
-                src += cap;
-                ++info->terminal_edges;
-            } else if (m == -3) {
-                snk += cap;
-                ++info->terminal_edges;
-            } else
-                return WTVO_EINVAL;
-        }
-        info->base_flow += (src < snk) ? src : snk;
-        const int64_t diff = src - snk;
-        if (diff < -2147483648LL || diff > 2147483647LL) return WTVO_EINVAL;
-        term[local] = (int32_t)diff;
+    rc = run_ranges(jobs, threads, reduce_build);
+    if (rc) return rc;
+    for (int t = 0; t < threads; ++t) {
+        info->free_refs += jobs[t].cnt.free_refs;
+        info->terminal_edges += jobs[t].cnt.terminal_edges;
+        info->base_flow += jobs[t].cnt.base_flow;
     }
     if (info->free_refs & 1ull) return WTVO_EINVAL;
     return WTVO_OK;
+}
+
+int wtvo_reduce(size_t N, const double* V, size_t nv, const int32_t* tets,
+                const int32_t* nbrs, const uint8_t* labels, double fill_T,
+                int32_t* full_to_free, uint32_t* free_old, int32_t* nbr_r,
+                uint32_t* res, int32_t* term, wtvo_reduce_info* info)
+{
+    return wtvo_reduce_mt(N, V, nv, tets, nbrs, labels, fill_T, full_to_free, free_old, nbr_r, res, term, info, 1);
 }
 
 /* Label reconstruction (bindings.cpp:357-364): out = labels; out[free_old[q]] = part[q]. */

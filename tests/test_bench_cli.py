@@ -25,5 +25,6 @@ def test_every_leg_has_a_switch():
     for flag in ["--no-cpu", "--no-profile", "--batch-graphs", "--serve-requests", "--delaunay-points", "--no-tune",
                  "--no-cut-leg", "--no-topology-leg", "--no-udf-leg", "--no-surface-leg", "--no-flow-leg", "--no-shard-leg",
                  "--copy-probe", "--atomic-probe", "--steps", "--warmup", "--no-standin-leg",
-                 "--workload", "--points", "--cpu-sample-points"]:
+                 "--workload", "--points", "--cpu-sample-points", "--no-small-legs", "--tet-order",
+                 "--serve-pool", "--serve-p99-ms", "--gather-probe"]:
         assert flag in r.stdout, flag

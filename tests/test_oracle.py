@@ -660,3 +660,21 @@ def test_sample_labels_errors(orc):
         orc.sample_labels(np.concatenate([o, o]), np.concatenate([v, v]), 1.0, 1 / 64, V, t, *UNIT, 0.0, 0.5)
     with pytest.raises(orc.OracleError):
         orc.sample_labels(o + 3, v, 1.0, 1 / 64, V, t, *UNIT, 0.0, 0.5)
+
+
+def test_reduce_threads_give_identical_arrays(orc):
+    """The build on 1 and on 7 threads: the same reduced graph, counters and
+    errors (ranges are summed; ids are assigned serially in tet order)."""
+    m = S.kuhn_mesh(24, seed=5)
+    lab = S.labels_from_sdf(S.tet_centroids_sdf(m, S.sphere_with_holes(5)), 0.05, seed=6)
+    a = orc.reduce(m.V, m.tets, m.nbrs, lab, 10.0, threads=1)
+    b = orc.reduce(m.V, m.tets, m.nbrs, lab, 10.0, threads=7)
+    assert a.n == b.n and a.info == b.info
+    for x, y in ((a.full_to_free, b.full_to_free), (a.free_old, b.free_old), (a.nbr, b.nbr), (a.res, b.res),
+                 (a.term, b.term)):
+        assert np.array_equal(x, y)
+    bad = m.nbrs.copy()
+    bad[m.N - 3, 1] = m.N + 5
+    for t in (1, 7):
+        with pytest.raises(orc.OracleError):
+            orc.reduce(m.V, m.tets, bad, lab, 10.0, threads=t)
