@@ -666,26 +666,34 @@ __device__ __forceinline__ void push_vertex(bool valid, uint32_t v, uint32_t n, 
                                             unsigned long long& pushes, unsigned long long& sinkf,
                                             unsigned long long& sdlt) {
     sdlt = 0ull;
-    long long e = valid ? atom_load(&excess[v]) : 0LL;
-    const uint32_t h = valid ? height[v] : kSent;
-    const bool act = e > 0 && h < (valid ? src_h<kB>(v, n, gid, gn) : 0u);
+    // every load that depends on v alone is issued at once (one round trip),
+    // before the activity test that needs two of them
+    long long e = 0;
+    uint32_t h = kSent, sh = 0u;
     nb = make_int4(-1, -1, -1, -1);
     uint4 r = make_uint4(0u, 0u, 0u, 0u);
     uint32_t rs = 0;
+    int32_t t = 0;
+    if (valid) {
+        e = atom_load(&excess[v]);
+        h = height[v];
+        nb = nbr4[v];
+        r = reinterpret_cast<const uint4*>(res)[v];
+        rs = rslot[v];
+        t = term[v];
+        sh = src_h<kB>(v, n, gid, gn);
+    }
+    const bool act = (e > 0) & (h < sh);
     uint32_t hu[4] = {kSent, kSent, kSent, kSent};
     long long pushed = 0;
     if (act) {
         ++visits;
-        nb = nbr4[v];
-        r = reinterpret_cast<const uint4*>(res)[v];
-        rs = rslot[v];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int u = slot_of(nb, k);
             if (u >= 0) hu[k] = height[u];
         }
         // v -> sink (gpu_push_relabel_fast.cu:220-236)
-        const int32_t t = term[v];
         if (t < 0 && h == 1u) {
             const long long cap = -static_cast<long long>(t);
             const long long dlt = e < cap ? e : cap;
@@ -696,24 +704,45 @@ __device__ __forceinline__ void push_vertex(bool valid, uint32_t v, uint32_t n, 
             sdlt = static_cast<unsigned long long>(dlt);
             ++pushes;
         }
+    } else {
+        nb = make_int4(-1, -1, -1, -1);
     }
+    // the amounts first (registers only), then the stores and every atomic
+    // back to back: one memory round trip for all of them instead of one per
+    // pushed arc.  v's slots are stored one by one: a neighbour pushing into
+    // v adds to one of v's other slots at the same time (never to a slot v
+    // pushes along: that needs the opposite height order).
+    uint32_t dl[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int u = slot_of(nb, k);
         const uint32_t rk = slot_of(r, k);
-        cand[k] = false;
         if (e > 0 && u >= 0 && rk != 0u && h == hu[k] + 1u) {
-            const uint32_t dlt = static_cast<long long>(rk) < e ? rk : static_cast<uint32_t>(e);
-            res[static_cast<size_t>(v) * 4u + k] = rk - dlt;
-            const uint32_t back = (rs >> (2 * k)) & 3u;
-            atom_add(&res[static_cast<size_t>(u) * 4u + back], dlt);
-            cand[k] = atom_add(&excess[u], static_cast<long long>(dlt)) == 0ll;
-            e -= dlt;
-            pushed += dlt;
+            dl[k] = static_cast<long long>(rk) < e ? rk : static_cast<uint32_t>(e);
+            e -= dl[k];
+            pushed += dl[k];
             ++pushes;
         }
     }
-    keep = act && (pushed == 0 || atom_add(&excess[v], -pushed) != pushed);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (dl[k]) {
+            res[static_cast<size_t>(v) * 4u + k] = slot_of(r, k) - dl[k];
+            const uint32_t back = (rs >> (2 * k)) & 3u;
+            atom_add(&res[static_cast<size_t>(slot_of(nb, k)) * 4u + back], dl[k]);
+        }
+    }
+    // old[k] is read only where dl[k] != 0, where the atomic set it; no
+    // default value, so no merge at the branch joins forces a wait there
+    long long old[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (dl[k]) old[k] = atom_add(&excess[slot_of(nb, k)], static_cast<long long>(dl[k]));
+    long long own = 0ll;
+    if (pushed) own = atom_add(&excess[v], -pushed);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cand[k] = dl[k] != 0u && old[k] == 0ll;
+    keep = act && (pushed == 0 || own != pushed);
 }
 
 // Relabel step of one relabel-list vertex: height = 1 + min over residual
@@ -726,26 +755,40 @@ __device__ __forceinline__ bool relabel_vertex(uint32_t v, uint32_t n, const int
                                                const uint32_t* __restrict__ gid,
                                                const uint32_t* __restrict__ gn, Ctl* ctl,
                                                unsigned long long& relabels) {
-    const long long e = excess[v];
-    uint32_t h = height[v];
-    const uint32_t sh = src_h<kB>(v, n, gid, gn);
-    if (!(e > 0 && h < sh)) return false;
-    const int32_t t = term[v];
+    // every load that depends on v alone at once (one round trip), then the
+    // residual neighbours' heights at once (a second one)
+    long long e = 0;
+    uint32_t h = 0u, sh = 0u;
+    int32_t t = 0;
+    int4 nb = make_int4(-1, -1, -1, -1);
+    uint4 r = make_uint4(0u, 0u, 0u, 0u);
+    if (v < n + 1u) {  // list entries are vertex ids: always
+        e = excess[v];
+        h = height[v];
+        sh = src_h<kB>(v, n, gid, gn);
+        t = term[v];
+        nb = nbr4[v];
+        r = reinterpret_cast<const uint4*>(res)[v];
+    }
+    if (!((e > 0) & (h < sh))) return false;
+    uint32_t x[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int u = slot_of(nb, k);
+        x[k] = u >= 0 && slot_of(r, k) != 0u ? height[u] : kSent;
+    }
     uint32_t mh = kSent;
     if (t < 0) mh = 0u;
     else if (t > 0) mh = sh;
-    const int4 nb = nbr4[v];
-    const uint4 r = reinterpret_cast<const uint4*>(res)[v];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int u = slot_of(nb, k);
         if (u >= 0 && slot_of(r, k) != 0u) {
-            uint32_t x = height[u];
             // a vertex the last BFS did not reach keeps the sentinel; it
             // stands for the reference's normalised height n+1
             // (gpu_push_relabel_fast.cu:141-147)
-            x = x == kSent ? sh + 1u : x;
-            mh = x < mh ? x : mh;
+            const uint32_t y = x[k] == kSent ? sh + 1u : x[k];
+            mh = y < mh ? y : mh;
         }
     }
     if (mh == kSent) {
