@@ -124,6 +124,11 @@ typedef struct wtv_params {
                                       sample of arcs join ids < 4096 apart
                                       (graphs of >= 2^20 nodes), 1 = always,
                                       -1 = never; results are the same      */
+    int32_t skip_finished;         /* batched solves: global relabels between
+                                      the first and the final one pass over
+                                      graphs without an active vertex (0, the
+                                      default); < 0 relabels every graph
+                                      (same results, more work)             */
 } wtv_params;
 
 /* Fill *p with the reference defaults. */

@@ -545,6 +545,34 @@ def test_batch_delaunay_and_kuhn_graphs_vs_bk(orc, torch_cuda):
         assert orc.certificate(nbr, res, term, r_out[a:b], t_out[a:b], part[a:b], int(flows[i])) == 0
 
 
+def test_batch_relabels_skip_finished_graphs(orc, torch_cuda):
+    """Small graphs (sizes not multiples of the 1024-vertex sweep tile) finish
+    long before a large one; with a global relabel every 8 sub-rounds, the
+    relabels in between pass over the finished graphs' tiles.  Results equal
+    BK's and the run with the skip turned off, which examines more vertices."""
+    graphs = [S.random_graph(n, 610 + i, cap_max=30) for i, n in enumerate((700, 1500, 2100, 3300, 777))]
+    _, _, g = _kuhn_soft(40, 21)
+    graphs.insert(2, (g.nbr, g.res, g.term))
+    # 196,608 vertices without terminals: no excess, finished from the start,
+    # every vertex examined by a seed sweep that does not skip it
+    _, _, z = _kuhn_soft(32, 22)
+    graphs.insert(4, (z.nbr, z.res, np.zeros_like(z.term)))
+    runs = {}
+    for skip in (0, -1):
+        off, flows, part, r_out, t_out, st = gpu_batch(torch_cuda, graphs, global_relabel_period=8, local_steps=1,
+                                                       relabel_alpha=-1.0, skip_finished=skip)
+        runs[skip] = (flows.copy(), part.copy(), st)
+        for i, (nbr, res, term) in enumerate(graphs):
+            a, b = int(off[i]), int(off[i + 1])
+            w = orc.bk_solve(nbr, res, term)
+            assert flows[i] == w.raw_flow, (skip, i)
+            assert np.array_equal(part[a:b], w.partition), (skip, i)
+    assert runs[0][2].global_relabels > 4 and runs[-1][2].global_relabels > 4
+    # the skip ran: fewer vertices examined by the seed sweeps per relabel
+    per = {k: v[2].bfs_seed_checks / v[2].global_relabels for k, v in runs.items()}
+    assert per[0] < per[-1] - 100_000, per
+
+
 def test_batch_rejects_bad_offsets_and_ids(torch_cuda):
     nbr, res, term = S.random_graph(40, 9)
     dn = torch_cuda.from_numpy(nbr).cuda()

@@ -1734,7 +1734,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     const uint32_t tail_cap = prm.tail_cap == 0 ? 512u : (prm.tail_cap < 0 ? 0u : static_cast<uint32_t>(prm.tail_cap));
     uint64_t relabels_now = relabels_at_gr;
     auto do_global_relabel = [&]() {
-        active = relabel_and_filter(true);
+        active = relabel_and_filter(prm.skip_finished >= 0);
         relabels_now = relabels_at_gr = stat_sum(*hctl, kStRelabels);
         const uint64_t pops = stat_sum(*hctl, kStBfsPops);
         last_gr_reach = nn ? static_cast<double>(pops - pops_at_gr) / static_cast<double>(nn) : 0.0;

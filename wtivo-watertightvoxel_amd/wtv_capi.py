@@ -48,6 +48,7 @@ class WtvParams(C.Structure):
         ("max_rounds", C.c_uint64), ("wg_size", C.c_int32), ("grid_wgs", C.c_int32),
         ("time_limit_s", C.c_double), ("profile", C.c_int32), ("no_graph", C.c_int32),
         ("tail_cap", C.c_int32), ("relabel_alpha", C.c_double), ("renumber", C.c_int32),
+        ("skip_finished", C.c_int32),
     ]
 
 
