@@ -623,9 +623,10 @@ def test_server_results_equal_alone_and_in_the_mix(orc, torch_cuda):
 
 
 def test_server_isolates_a_failed_request(orc, torch_cuda):
-    """A request whose graph is invalid (a missing reverse arc) fails alone: the
-    batch it was admitted with is re-solved request by request, so every other
-    request still returns its lone-solve flow and labels."""
+    """Requests whose graphs are invalid (a missing reverse arc, an id out of
+    range) fail alone at admission, and the requests admitted with them (some
+    placed in the arena after them) still return their lone-solve flows and
+    labels."""
     _, _, big = _kuhn_soft(24, 11)
     good = [S.random_graph(int(200 + 50 * i), 960 + i, cap_max=40) for i in range(5)]
     bad = S.random_graph(300, 3)
@@ -634,7 +635,12 @@ def test_server_isolates_a_failed_request(orc, torch_cuda):
     u = int(nbr[v, int(np.argwhere(nbr[v] >= 0)[0][0])])
     nbr[u, nbr[u] == v] = -1  # drop the reverse arc
     bad = (nbr, bad[1], bad[2])
-    graphs = [(big.nbr, big.res, big.term)] + good[:2] + [bad] + good[2:]
+    # a second one with a neighbour id out of range (in an empty slot)
+    b2 = S.random_graph(250, 4)
+    n2 = b2[0].copy()
+    r2, k2 = (int(x) for x in np.argwhere(n2 < 0)[0])
+    n2[r2, k2] = 250
+    graphs = [(big.nbr, big.res, big.term)] + good[:2] + [bad] + good[2:4] + [(n2, b2[1], b2[2])] + good[4:]
     dev = [_dev_graph(torch_cuda, *g) for g in graphs]
     torch_cuda.cuda.synchronize()
     # one slot: the first (large) request runs while the rest queue into one batch
@@ -642,10 +648,10 @@ def test_server_isolates_a_failed_request(orc, torch_cuda):
         tickets = [srv.submit(g[2].shape[0], d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), d[3].data_ptr())
                    for g, d in zip(graphs, dev)]
         res = [srv.wait(t) for t in tickets]
-    assert res[3].status == W.WTV_EINVARIANT
+    assert res[3].status == W.WTV_EINVARIANT and res[6].status == W.WTV_EINVARIANT
     assert max(r.batch_size for r in res) > 1, "nothing was batched"
     for i, (g, d, r) in enumerate(zip(graphs, dev, res)):
-        if i == 3:
+        if i in (3, 6):
             continue
         w = orc.bk_solve(*g)
         assert r.status == 0 and r.raw_flow == w.raw_flow, i

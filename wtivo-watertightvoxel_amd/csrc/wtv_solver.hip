@@ -240,12 +240,23 @@ __global__ void __launch_bounds__(kTPB) k_globalize(uint32_t n, uint32_t G, cons
                                                     const int4* __restrict__ nbr_local, int4* __restrict__ nbr_global,
                                                     uint32_t* __restrict__ gid, Ctl* ctl) {
     const uint32_t stride = gridDim.x * kTPB;
-    for (uint32_t v = blockIdx.x * kTPB + threadIdx.x; v < n; v += stride) {
-        uint32_t lo = 0, hi = G;  // largest g with off[g] <= v
+    // largest g with off[g] <= x
+    auto graph_of = [&](uint32_t x) {
+        uint32_t lo = 0, hi = G;
         while (hi - lo > 1u) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (off[mid] <= v) lo = mid; else hi = mid;
+            if (off[mid] <= x) lo = mid; else hi = mid;
         }
+        return lo;
+    };
+    for (uint32_t v = blockIdx.x * kTPB + threadIdx.x; v < n; v += stride) {
+        // one search per wave, for its first vertex; a lane past that graph's
+        // end (the wave straddles graphs) searches for its own
+        const uint32_t v0 = __shfl(v, 0);
+        uint32_t lo = 0;
+        if (__lane_id() == 0) lo = graph_of(v0);
+        lo = __shfl(lo, 0);
+        if (off[lo + 1] <= v) lo = graph_of(v);
         const uint32_t base = static_cast<uint32_t>(off[lo]);
         const uint32_t ng = static_cast<uint32_t>(off[lo + 1] - off[lo]);
         const int4 a = nbr_local[v];
@@ -267,10 +278,11 @@ __global__ void __launch_bounds__(kTPB) k_globalize(uint32_t n, uint32_t G, cons
 // nodes [lo, hi), whose neighbours lie in [lo, hi) too (a graph of the
 // arena, or the whole solve).  With `wl` the vertices holding source supply
 // are appended to it (counter *cnt); without, the pass only validates.
+// Errors go to *err_word when given (a graph of the arena), else to ctl.
 __global__ void __launch_bounds__(kTPB) k_prepare(uint32_t lo, uint32_t hi, const int4* __restrict__ nbr4,
                                                   const int32_t* __restrict__ term, uint8_t* __restrict__ rslot,
                                                   long long* __restrict__ excess, uint32_t* __restrict__ wl,
-                                                  uint32_t* cnt, Ctl* ctl) {
+                                                  uint32_t* cnt, Ctl* ctl, int32_t* err_word) {
     __shared__ WgQueue<kTPB> q;
     q_init(q);
     const uint32_t stride = gridDim.x * kTPB;
@@ -293,7 +305,10 @@ __global__ void __launch_bounds__(kTPB) k_prepare(uint32_t lo, uint32_t hi, cons
                 if (j < 0) { err = 2; continue; }
                 code |= static_cast<uint32_t>(j) << (2 * k);
             }
-            if (err) set_error(ctl, err);
+            if (err) {
+                if (err_word) __hip_atomic_exchange(err_word, err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                else set_error(ctl, err);
+            }
             rslot[v] = static_cast<uint8_t>(code);
             const int32_t t = term[v];
             excess[v] = t > 0 ? static_cast<long long>(t) : 0LL;
@@ -985,11 +1000,12 @@ __global__ void __launch_bounds__(kTailTPB) k_discharge_tail(uint32_t n, uint32_
 // and the sentinel graph slot, whose tiles the sweeps skip.
 
 // Admission of graph slot g: union-wide neighbour ids (ids outside [0, ng)
-// are device invariant 3), graph ids, inert padding.
+// are device invariant 3, written to the slot's error word), graph ids,
+// inert padding.
 __global__ void __launch_bounds__(kTPB) k_admit(uint32_t lo, uint32_t ng, uint32_t ag, uint32_t g,
                                                 const int4* __restrict__ nbr_local, int4* __restrict__ nbrg,
                                                 uint4* __restrict__ res, int32_t* __restrict__ term,
-                                                uint32_t* __restrict__ gid, Ctl* ctl) {
+                                                uint32_t* __restrict__ gid, int32_t* gerr) {
     int err = 0;
     for (uint32_t i = blockIdx.x * kTPB + threadIdx.x; i < ag; i += gridDim.x * kTPB) {
         const uint32_t v = lo + i;
@@ -1008,13 +1024,14 @@ __global__ void __launch_bounds__(kTPB) k_admit(uint32_t lo, uint32_t ng, uint32
             term[v] = 0;
         }
     }
-    if (err) set_error(ctl, err);
+    if (err) __hip_atomic_exchange(&gerr[g], err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Per-slot tables of graph slot g: node count (its source height), flow,
-// residency and the fresh flag (no heights yet).
+// residency, the fresh flag (no heights yet) and a clear error word.
 __global__ void k_set_slot(uint32_t g, uint32_t ng, uint32_t* gn, unsigned long long* gflow, uint8_t* gres,
-                           uint8_t* gfresh, uint8_t resident) {
+                           uint8_t* gfresh, int32_t* gerr, uint8_t resident) {
+    gerr[g] = 0;
     gn[g] = ng;
     gflow[g] = 0ull;
     gres[g] = resident;
@@ -1253,15 +1270,15 @@ struct GraphExec {
 };
 
 struct Layout {
-    size_t height, excess, rslot, nzm, wl0, wl1, rl, ctl, gid, nbrg, gn, goff, gflow, galive, gres, gfresh, ares,
-        aterm, total;
+    size_t height, excess, rslot, nzm, wl0, wl1, rl, ctl, gid, nbrg, gn, goff, gflow, galive, gres, gfresh, gerr,
+        ares, aterm, total;
 };
 
 // One allocation carved into 256-byte aligned arrays.  Batched mode adds the
 // graph id per node, a union-wide copy of the neighbour ids and three
 // per-graph tables (node count, offsets, sink flow).  An arena (serving) also
-// holds the residual rows and terminals of its graphs and two more per-slot
-// byte tables (resident, fresh).
+// holds the residual rows and terminals of its graphs, two more per-slot
+// byte tables (resident, fresh) and a per-slot error word (admission checks).
 Layout layout_for(uint32_t n, uint32_t G, bool arena = false) {
     Layout L{};
     size_t o = 0;
@@ -1285,6 +1302,7 @@ Layout layout_for(uint32_t n, uint32_t G, bool arena = false) {
         if (arena) {
             L.gres = take(G);
             L.gfresh = take(G);
+            L.gerr = take(static_cast<size_t>(G) * 4u);
             L.ares = take(N * 16u);
             L.aterm = take(N * 4u);
         }
@@ -1326,6 +1344,7 @@ struct ArenaView {
     uint8_t* galive;  // graphs with an active vertex (after a relabel's filter)
     uint8_t* gres;    // resident graphs
     uint8_t* gfresh;  // graphs admitted since the last relabel
+    int32_t* gerr;    // per slot: the first failed admission check of its graph
     uint32_t* wl[2];
     uint32_t capacity;  // nodes
     uint32_t slots;     // graph slots; slot `slots` is the sentinel of free nodes
@@ -1458,7 +1477,8 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         nbr4 = reinterpret_cast<const int4*>(base + L.nbrg);
         av = ArenaView{s, ctl, reinterpret_cast<int4*>(base + L.nbrg), reinterpret_cast<uint4*>(d_res), d_term,
                        excess, height, rslot, gid, gn, gflow, galive, reinterpret_cast<uint8_t*>(base + L.gres),
-                       reinterpret_cast<uint8_t*>(base + L.gfresh), {wl[0], wl[1]}, n, X.slots, 0u};
+                       reinterpret_cast<uint8_t*>(base + L.gfresh), reinterpret_cast<int32_t*>(base + L.gerr),
+                       {wl[0], wl[1]}, n, X.slots, 0u};
     }
 
     // pinned snapshots, or thread-local pageable ones without allocation
@@ -1522,7 +1542,8 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             ck(hipGetLastError(), "globalize");
             nbr4 = nbrg;
         }
-        k_prepare<<<sweep_grid, tpb, 0, s>>>(0u, n, nbr4, d_term, rslot, excess, wl[0], &ctl->wl_cnt[0], ctl);
+        k_prepare<<<sweep_grid, tpb, 0, s>>>(0u, n, nbr4, d_term, rslot, excess, wl[0], &ctl->wl_cnt[0], ctl,
+                                             nullptr);
         ck(hipGetLastError(), "prepare");
     }
     check_err("prepare");
@@ -2064,15 +2085,17 @@ class ArenaSession final : public Feeder {
         if (alone_) return;  // a graph solved alone admits no other
         std::vector<SessionJob> jobs = q_.take(largest_free(), static_cast<uint32_t>(free_slots_.size()),
                                                !res_.empty());
-        for (size_t i = 0; i < jobs.size(); ++i) {
-            try {
-                admit_one(a, p, jobs[i]);
-            } catch (...) {
-                // the requests taken with it are not lost: back to the queue
-                const std::vector<SessionJob> rest(jobs.begin() + static_cast<std::ptrdiff_t>(i) + 1, jobs.end());
-                if (!rest.empty()) q_.requeue_alone(rest);
-                throw;
-            }
+        if (jobs.empty()) return;
+        try {
+            admit_round(a, p, jobs);
+        } catch (...) {
+            // the requests taken and not yet resident or finished are not
+            // lost: back to the queue, to be solved alone
+            std::vector<SessionJob> rest;
+            for (const SessionJob& j : jobs)
+                if (j.ticket) rest.push_back(j);
+            if (!rest.empty()) q_.requeue_alone(rest);
+            throw;
         }
         a.hwm = hwm_;
     }
@@ -2187,61 +2210,82 @@ class ArenaSession final : public Feeder {
         return err;
     }
 
-    void admit_one(ArenaView& a, uint32_t p, const SessionJob& j) {
-        const uint32_t ng = j.req.n;
-        const double t = q_.now();
-        if (ng == 0) {
-            q_.finish(j.ticket, WTV_OK, 0, static_cast<int32_t>(res_.size()) + 1, worker_, t, t);
-            return;
-        }
-        const uint32_t ag = align_nodes(ng);
-        const uint32_t lo = carve(ag);
-        const uint32_t g = free_slots_.back();
-        free_slots_.pop_back();
-        if (j.ready) ck(hipStreamWaitEvent(a.s, static_cast<hipEvent_t>(j.ready), 0), "ready wait");
+    // One admission round: every taken request is copied in and validated
+    // (ids in range, reverse arcs) into its slot's error word; one read of
+    // the words then fails the bad requests alone, before any of their
+    // vertices joins the active list, and the rest enter the solve.  A job
+    // whose ticket is set to 0 has been finished or made resident.
+    void admit_round(ArenaView& a, uint32_t p, std::vector<SessionJob>& jobs) {
         // an error of the discharge so far belongs to the shared solve, not
-        // to this request: the solve fails and its graphs are solved alone
+        // to these requests: the solve fails and its graphs are solved alone
         if (const int32_t e0 = read_error(a, "admit check")) {
-            free_slots_.push_back(g);
-            free_[lo] = ag;  // back to the free list (merged at the session's reset)
-            q_.requeue_alone({j});
             std::ostringstream oss;
             oss << kPrefix << "device invariant failed at discharge code=" << e0;
             fail(WTV_EINVARIANT, oss.str());
         }
-        k_admit<<<grid_of(ag), kTPB, 0, a.s>>>(lo, ng, ag, g, reinterpret_cast<const int4*>(j.req.d_nbr), a.nbrg,
-                                               a.res, a.term, a.gid, a.ctl);
-        ck(hipMemcpyAsync(a.res + lo, j.req.d_res, static_cast<size_t>(ng) * 16u, hipMemcpyDeviceToDevice, a.s),
-           "admit res");
-        ck(hipMemcpyAsync(a.term + lo, j.req.d_term, static_cast<size_t>(ng) * 4u, hipMemcpyDeviceToDevice, a.s),
-           "admit term");
-        k_set_slot<<<1, 1, 0, a.s>>>(g, ng, a.gn, a.gflow, a.gres, a.gfresh, 1u);
-        // validate first (ids in range, reverse arcs): a bad request fails
-        // alone, before any of its vertices joins the active list
-        k_prepare<<<grid_of(ag), kTPB, 0, a.s>>>(lo, lo + ag, a.nbrg, a.term, a.rslot, a.excess, nullptr, nullptr,
-                                                 a.ctl);
-        ck(hipGetLastError(), "admit");
-        const int32_t err = read_error(a, "admit check");
-        Res r{j, g, lo, ag, static_cast<int32_t>(res_.size()) + 1, t};
-        hwm_ = std::max(hwm_, lo + ag);
-        if (err) {
-            ck(hipMemsetAsync(&a.ctl->error, 0, 4u, a.s), "error clear");  // it was this request's
-            q_.finish(j.ticket, WTV_EINVARIANT, 0, r.batch, worker_, t, q_.now());
-            release(a, r);
-            return;
+        struct Placed {
+            size_t job;
+            Res r;
+        };
+        std::vector<Placed> placed;
+        const double t = q_.now();
+        for (size_t i = 0; i < jobs.size(); ++i) {
+            SessionJob& j = jobs[i];
+            const uint32_t ng = j.req.n;
+            if (ng == 0) {
+                q_.finish(j.ticket, WTV_OK, 0, static_cast<int32_t>(res_.size() + placed.size()) + 1, worker_, t, t);
+                j.ticket = 0;
+                continue;
+            }
+            const uint32_t ag = align_nodes(ng);
+            const uint32_t lo = carve(ag);
+            const uint32_t g = free_slots_.back();
+            free_slots_.pop_back();
+            Res r{j, g, lo, ag, static_cast<int32_t>(res_.size() + placed.size()) + 1, t};
+            placed.push_back(Placed{i, r});
+            hwm_ = std::max(hwm_, lo + ag);
+            if (j.ready) ck(hipStreamWaitEvent(a.s, static_cast<hipEvent_t>(j.ready), 0), "ready wait");
+            k_set_slot<<<1, 1, 0, a.s>>>(g, ng, a.gn, a.gflow, a.gres, a.gfresh, a.gerr, 1u);
+            k_admit<<<grid_of(ag), kTPB, 0, a.s>>>(lo, ng, ag, g, reinterpret_cast<const int4*>(j.req.d_nbr), a.nbrg,
+                                                   a.res, a.term, a.gid, a.gerr);
+            ck(hipMemcpyAsync(a.res + lo, j.req.d_res, static_cast<size_t>(ng) * 16u, hipMemcpyDeviceToDevice, a.s),
+               "admit res");
+            ck(hipMemcpyAsync(a.term + lo, j.req.d_term, static_cast<size_t>(ng) * 4u, hipMemcpyDeviceToDevice, a.s),
+               "admit term");
+            k_prepare<<<grid_of(ag), kTPB, 0, a.s>>>(lo, lo + ag, a.nbrg, a.term, a.rslot, a.excess, nullptr, nullptr,
+                                                     a.ctl, a.gerr + g);
+            ck(hipGetLastError(), "admit");
         }
-        k_prepare<<<grid_of(ag), kTPB, 0, a.s>>>(lo, lo + ag, a.nbrg, a.term, a.rslot, a.excess, a.wl[p],
-                                                 &a.ctl->wl_cnt[p], a.ctl);
-        ck(hipGetLastError(), "admit supply");
-        res_.push_back(r);
-        if (j.alone) alone_ = true;
+        if (placed.empty()) return;
+        const uint32_t G = slots_ + 1u;
+        errs_.resize(G);
+        ck(hipMemcpyAsync(errs_.data(), a.gerr, static_cast<size_t>(G) * 4u, hipMemcpyDeviceToHost, a.s),
+           "admit errors D2H");
+        ck(hipStreamSynchronize(a.s), "admit errors");
+        for (const Placed& pl : placed) {
+            const Res& r = pl.r;
+            jobs[pl.job].ticket = 0;
+            if (errs_[r.slot]) {
+                q_.finish(r.job.ticket, WTV_EINVARIANT, 0, r.batch, worker_, r.start, q_.now());
+                release(a, r);
+                continue;
+            }
+            k_prepare<<<grid_of(r.ag), kTPB, 0, a.s>>>(r.lo, r.lo + r.ag, a.nbrg, a.term, a.rslot, a.excess, a.wl[p],
+                                                       &a.ctl->wl_cnt[p], a.ctl, a.gerr + r.slot);
+            ck(hipGetLastError(), "admit supply");
+            res_.push_back(r);
+            if (r.job.alone) alone_ = true;
+        }
+        // a release above recomputed the bound before the later graphs joined
+        hwm_ = 0;
+        for (const Res& x : res_) hwm_ = std::max(hwm_, x.lo + x.ag);
     }
 
     // the graph's nodes become inert and its range and slot free
     void release(ArenaView& a, const Res& r) {
         k_inert<<<grid_of(r.ag), kTPB, 0, a.s>>>(r.lo, r.lo + r.ag, a.nbrg, a.term, a.excess, a.height, a.gid,
                                                  slots_);
-        k_set_slot<<<1, 1, 0, a.s>>>(r.slot, 0u, a.gn, a.gflow, a.gres, a.gfresh, 0u);
+        k_set_slot<<<1, 1, 0, a.s>>>(r.slot, 0u, a.gn, a.gflow, a.gres, a.gfresh, a.gerr, 0u);
         ck(hipGetLastError(), "release");
         free_slots_.push_back(r.slot);
         // merge the range with its free neighbours
@@ -2272,6 +2316,7 @@ class ArenaSession final : public Feeder {
     std::vector<uint32_t> free_slots_;
     std::vector<Res> res_;
     std::vector<uint8_t> alive_;
+    std::vector<int32_t> errs_;
     std::vector<unsigned long long> flows_;
     uint32_t hwm_ = 0;
     bool alone_ = false;
