@@ -69,6 +69,28 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+class _StdoutToStderr:
+    """Sends what native code prints to stdout (the drop-in surface call prints
+    a line per call, as the reference does) to stderr, so that stdout holds
+    only the one JSON line."""
+
+    def __enter__(self):
+        import ctypes
+        self.libc = ctypes.CDLL(None)
+        sys.stdout.flush()
+        self.libc.fflush(None)
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+        return self
+
+    def __exit__(self, *exc):
+        self.libc.fflush(None)
+        sys.stdout.flush()
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        return False
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +125,8 @@ def main():
     ap.add_argument("--serve-sweep-requests", type=int, default=400)
     ap.add_argument("--serve-p99-ms", type=float, default=100.0, help="serving leg: p99 latency target")
     ap.add_argument("--no-standin-leg", action="store_true", help="skip the Kuhn 235^3 leg")
+    ap.add_argument("--no-native-order", action="store_true",
+                    help="skip timing the headline graph in the tetrahedraliser's own order")
     ap.add_argument("--no-small-legs", action="store_true",
                     help="skip the Kuhn 32^3 and Kuhn 150^3 integer-stress legs (GPU and CPU medians of 5)")
     ap.add_argument("--delaunay-points", type=int, default=0,
@@ -218,7 +242,7 @@ def main():
     raw_flow = flows[0]
     # the same graph in the tetrahedraliser's spatial order, timed the same way
     native_order = None
-    if native_host is not None:
+    if native_host is not None and not a.no_native_order:
         dn = {k: torch.from_numpy(native_host[k]).to(dev) for k in ("nbr", "res", "term")}
         rn, tn = dn["res"].clone(), dn["term"].clone()
         pn = torch.empty(N, dtype=torch.uint8, device=dev)
@@ -717,15 +741,19 @@ def cpu_sample_leg(a, torch, np, W, S, dev, stream, n_full):
     scale = n_full / N
     log(f"[bench] cpu sample: {N} tets, BK {bk_s:.2f}s (native order {bk_native_s}), restated push-relabel "
         f"{pr_s:.2f}s, GPU {gpu_s * 1e3:.1f} ms")
-    return {"value": round(bk_s * scale, 3), "unit": "s", "cores": 1, "kind": "port",
+    # the CPU solver is credited with its faster order: BK has no renumbering
+    # of its own, and the tetrahedraliser's spatial order suits it best
+    bk_best = min(bk_s, bk_native_s) if bk_native_s is not None else bk_s
+    return {"value": round(bk_best * scale, 3), "unit": "s", "cores": 1, "kind": "port",
             "sample": f"Boykov-Kolmogorov (oracle/wtv_oracle.c), 1 thread, on the Delaunay graph of "
                       f"{a.cpu_sample_points} points of the same generator ({N} tets, {meta['repeated_points']} "
-                      f"repeated points): {bk_s:.3f} s, scaled by tet count x{scale:.3f} to the headline's "
-                      f"{n_full} tets; flow and labels equal to the GPU's on the sample",
+                      f"repeated points) in its faster tet order: {bk_best:.3f} s (shuffled order {bk_s:.3f} s), "
+                      f"scaled by tet count x{scale:.3f} to the headline's {n_full} tets; flow and labels equal "
+                      f"to the GPU's on the sample",
             "sample_tets": N, "sample_bk_s": round(bk_s, 3), "sample_restated_push_relabel_s": round(pr_s, 3),
             "sample_tet_order": meta.get("tet_order"),
             "sample_bk_native_order_s": round(bk_native_s, 3) if bk_native_s is not None else None,
-            "sample_gpu_s": round(gpu_s, 5), "sample_speedup_vs_bk": round(bk_s / gpu_s, 1),
+            "sample_gpu_s": round(gpu_s, 5), "sample_speedup_vs_bk": round(bk_best / gpu_s, 1),
             "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
 
 
@@ -1145,8 +1173,9 @@ def surface_leg(a, torch, np, W, S, dev, stream):
         out = G.surface_extraction_topology_cuda(labh, Vh, T, Nb, 16)
         return time.perf_counter() - t0, out
 
-    call()
-    runs = [call() for _ in range(3)]
+    with _StdoutToStderr():
+        call()
+        runs = [call() for _ in range(3)]
     call_s = min(r[0] for r in runs)
     OV, OF = runs[-1][1]
     # the reference's first pass: the neighbour rows, chunk by chunk, to pageable memory
