@@ -93,6 +93,8 @@ typedef struct wtv_stats {
                                    renumbered, the renumbering and the copy
                                    back (host wall time)                    */
     uint64_t renumbered;        /* 1: solved as a locality-renumbered copy   */
+    uint64_t pull_rounds;       /* macro rounds run as pull-mode sub-rounds
+                                   (wtv_params.pull_min)                    */
 } wtv_stats;
 
 /* Solver knobs.  global_relabel_period, max_rounds and local_steps keep the
@@ -129,6 +131,12 @@ typedef struct wtv_params {
                                       graphs without an active vertex (0, the
                                       default); < 0 relabels every graph
                                       (same results, more work)             */
+    int32_t pull_min;              /* active lists of more than this many
+                                      vertices run pull-mode sub-rounds (plain
+                                      stores and gathers instead of atomics);
+                                      0 = default 4,194,304,
+                                      < 0 = off; results are the same; the
+                                      allocation-free solve never uses them */
 } wtv_params;
 
 /* Fill *p with the reference defaults. */

@@ -935,6 +935,50 @@ def test_profile_mode_counts_kernels(torch_cuda):
     assert st3["t_event_overhead_us"] == 0.0 and st3["n_push"] == 0
 
 
+@pytest.mark.parametrize("pull_min", [1, 64, -1])
+def test_pull_mode_same_answer(orc, torch_cuda, pull_min):
+    """Pull-mode sub-rounds (plain stores and gathers instead of atomics; by
+    default only for lists of millions of vertices, here forced down to every
+    list of more than pull_min vertices) give BK's flow and labels and a valid
+    maximum preflow, alone, mixed with atomic sub-rounds and with the
+    one-workgroup tail, in single, batched and served solves."""
+    _, _, g = _kuhn_soft(24, 81)
+    want = orc.bk_solve(g.nbr, g.res, g.term)
+    for tail_cap in (-1, 0):
+        gpu = gpu_solve(torch_cuda, g.nbr, g.res, g.term, pull_min=pull_min, tail_cap=tail_cap)
+        assert_same(orc, g.nbr, g.res, g.term, gpu, want.raw_flow, want.partition)
+        assert (gpu[1]["pull_rounds"] > 0) == (pull_min > 0)
+    # the reference's source-height quirk and random graphs
+    for seed in range(6):
+        nbr, res, term = S.random_graph(int(40 + 60 * seed), 1300 + seed, cap_max=int(3 + 9 * seed))
+        a = orc.pr_solve(nbr, res, term)
+        if orc.certificate(nbr, res, term, a.res, a.term, a.partition, a.raw_flow) != 0:
+            continue
+        gpu = gpu_solve(torch_cuda, nbr, res, term, pull_min=pull_min, tail_cap=-1)
+        assert_same(orc, nbr, res, term, gpu, a.raw_flow, a.partition)
+    graphs = [S.random_graph(int(200 + 50 * i), 1400 + i, cap_max=30) for i in range(5)]
+    graphs.append((g.nbr, g.res, g.term))
+    off, flows, part, _, _, st = gpu_batch(torch_cuda, graphs, pull_min=pull_min, tail_cap=-1)
+    assert (st.pull_rounds > 0) == (pull_min > 0)
+    for i, gr in enumerate(graphs):
+        b = orc.bk_solve(*gr)
+        assert flows[i] == b.raw_flow
+        assert np.array_equal(part[int(off[i]):int(off[i + 1])], b.partition)
+    wantb = [orc.bk_solve(*gr) for gr in graphs]
+    torch_cuda.cuda.synchronize()
+    dev = [_dev_graph(torch_cuda, *gr) for gr in graphs]
+    torch_cuda.cuda.synchronize()
+    with W.Server(slots=1, max_batch=8, pull_min=pull_min, tail_cap=-1) as srv:
+        tickets = [srv.submit(gr[2].shape[0], d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), d[3].data_ptr())
+                   for gr, d in zip(graphs, dev)]
+        res = [srv.wait(t) for t in tickets]
+    for gr, d, r, w in zip(graphs, dev, res, wantb):
+        assert r.status == 0 and r.raw_flow == w.raw_flow
+        assert np.array_equal(d[3].cpu().numpy()[:gr[2].shape[0]], w.partition)
+        assert orc.certificate(gr[0], gr[1], gr[2], d[1].cpu().numpy().view(np.uint32), d[2].cpu().numpy(),
+                               w.partition, r.raw_flow) == 0
+
+
 @pytest.mark.parametrize("tail_cap", [-1, 16, 2048, 1 << 20])
 def test_discharge_tail_mode_same_answer(orc, torch_cuda, tail_cap):
     """The one-workgroup discharge (small active lists) and the multi-workgroup

@@ -234,6 +234,28 @@ __device__ __forceinline__ uint32_t src_h(uint32_t v, uint32_t n, const uint32_t
 
 // ---------------------------------------------------------------- kernels
 
+// A workgroup's contiguous share of m items (list entries or node ids) in
+// steps of `step`: the items [b, e).  The appends of a workgroup, flushed
+// lazily every few steps, then come from one region of its input, so the
+// lists the next kernel reads keep neighbouring vertices in neighbouring
+// entries and a wave's rows share cache lines.  A grid-stride loop would mix
+// into each flush steps gridDim.x * step entries apart, and over many
+// sub-rounds the lists drift towards a random order (on the 77.5M-node
+// Delaunay graph, push visits from a random-order list take 3 to 6 times
+// as long as from an id-ordered one).  With m <= gridDim.x * step the share
+// is the single step at blockIdx.x * step, as in a grid-stride loop.
+struct Span {
+    uint64_t b, e;
+};
+__device__ __forceinline__ Span wg_span(uint64_t m, uint32_t step) {
+    const uint64_t steps = (m + step - 1u) / step;
+    const uint64_t per = (steps + gridDim.x - 1u) / gridDim.x;
+    const uint64_t b = static_cast<uint64_t>(blockIdx.x) * per * step;
+    const uint64_t e = b + per * step;
+    return Span{b < m ? b : m, e < m ? e : m};
+}
+
+
 // Batched mode: graph g occupies nodes [off[g], off[g+1]) of the union and
 // its neighbour ids are local to g.  Writes union-wide ids and the graph id.
 __global__ void __launch_bounds__(kTPB) k_globalize(uint32_t n, uint32_t G, const unsigned long long* __restrict__ off,
@@ -285,12 +307,12 @@ __global__ void __launch_bounds__(kTPB) k_prepare(uint32_t lo, uint32_t hi, cons
                                                   uint32_t* cnt, Ctl* ctl, int32_t* err_word) {
     __shared__ WgQueue<kTPB> q;
     q_init(q);
-    const uint32_t stride = gridDim.x * kTPB;
     // uniform trip count per workgroup: every thread reaches each queue flush
-    for (uint32_t base = lo + blockIdx.x * kTPB; base < hi; base += stride) {
-        const uint32_t v = base + threadIdx.x;
+    const Span sp = wg_span(hi - lo, kTPB);
+    for (uint64_t base = sp.b; base < sp.e; base += kTPB) {
+        const uint32_t v = lo + static_cast<uint32_t>(base) + threadIdx.x;
         bool cand = false;
-        if (v < hi) {
+        if (base + threadIdx.x < sp.e) {
             const int4 nb = nbr4[v];
             uint32_t code = 0;
             int err = 0;
@@ -381,9 +403,9 @@ __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __
     __shared__ WgQueue<kTPB> q;
     q_init(q);
     const uint32_t span = kTPB * kSeedUnroll;
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * span;
+    const Span sp = wg_span(n, span);
     unsigned long long work = 0;
-    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * span; base < n; base += stride) {
+    for (uint64_t base = sp.b; base < sp.e; base += span) {
         if (galive) {
             // a tile whose graphs are all finished is skipped by the whole
             // workgroup (graph ids ascend with the node id)
@@ -499,11 +521,11 @@ __global__ void __launch_bounds__(TPB) k_bfs_level(uint32_t n, uint32_t d, uint3
         // every frontier entry is expanded once: one add per level
         else if (cnt) atom_add(&ctl->stat[kStBfsPops][d % kStatSlots], static_cast<unsigned long long>(cnt));
     }
-    const uint32_t stride = gridDim.x * TPB;
     unsigned long long work = 0;
     if (bottom_up) {
-        for (uint32_t base = blockIdx.x * TPB; base < n; base += stride) {
-            const uint32_t v = base + threadIdx.x;
+        const Span sp = wg_span(n, TPB);
+        for (uint64_t base = sp.b; base < sp.e; base += TPB) {
+            const uint32_t v = static_cast<uint32_t>(base) + threadIdx.x;
             bool found = false;
             if (v < n && height[v] == kSent) {
                 ++work;
@@ -528,9 +550,12 @@ __global__ void __launch_bounds__(TPB) k_bfs_level(uint32_t n, uint32_t d, uint3
         q_flush(q, fout, out_cnt);
         flush_stat(ctl, kStBuChecks, work);
     } else {
-        for (uint32_t base = blockIdx.x * TPB; base < cnt; base += stride) {
-            const uint32_t i = base + threadIdx.x;
-            const bool valid = i < cnt;
+        // the prefetched entry is this thread's first when the share is the
+        // single step at blockIdx.x * TPB (and for workgroup 0 always)
+        const Span sp = wg_span(cnt, TPB);
+        for (uint64_t base = sp.b; base < sp.e; base += TPB) {
+            const uint32_t i = static_cast<uint32_t>(base) + threadIdx.x;
+            const bool valid = i < sp.e;
             const uint32_t u = !valid ? 0u : (base == blockIdx.x * TPB ? u0 : fin[i]);
             int4 nb;
             bool claim[4];
@@ -550,6 +575,12 @@ __global__ void __launch_bounds__(TPB) k_bfs_level(uint32_t n, uint32_t d, uint3
 // counter ring, exactly as a sequence of k_bfs_level launches would.
 constexpr uint32_t kTailMax = 1024;
 constexpr double kRelabelCost = 1.0;
+// Active lists of more than this many vertices run pull-mode sub-rounds by
+// default.  On the 77.5M-node Delaunay graph a pull-mode sub-round costs
+// 4.6 ms against 7.9 ms for the atomic one while the list holds all ~37M
+// source vertices, and the two meet near 2-3M vertices: the apply sweep
+// reads two words per neighbour where the relabel reads one.
+constexpr uint64_t kPullMinList = 4194304u;
 // The one-workgroup kernels run 1024 threads: a list of up to 1024 items
 // takes one pass per level or phase.
 constexpr int kTailTPB = 1024;
@@ -619,12 +650,12 @@ __global__ void __launch_bounds__(TPB) k_filter_active(uint32_t n, uint32_t p, c
     q_init(q);
     const uint32_t cnt = ctl->wl_cnt[p];
     uint32_t* out_cnt = &ctl->wl_cnt[p ^ 1u];
-    const uint32_t stride = gridDim.x * TPB;
-    for (uint32_t base = blockIdx.x * TPB; base < cnt; base += stride) {
-        const uint32_t i = base + threadIdx.x;
+    const Span sp = wg_span(cnt, TPB);
+    for (uint64_t base = sp.b; base < sp.e; base += TPB) {
+        const uint32_t i = static_cast<uint32_t>(base) + threadIdx.x;
         bool a = false;
         uint32_t v = 0;
-        if (i < cnt) {
+        if (i < sp.e) {
             v = src[i];
             a = height[v] < src_h<kB>(v, n, gid, gn) && excess[v] > 0;
         }
@@ -830,11 +861,11 @@ __global__ void __launch_bounds__(TPB) k_push(uint32_t n, uint32_t p, const uint
     const uint32_t cnt = ctl->wl_cnt[p];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->wl_cnt[p ^ 1u] = 0u;
     uint32_t* rl_cnt = &ctl->rl_cnt[p];
-    const uint32_t stride = gridDim.x * TPB;
     unsigned long long visits = 0, pushes = 0, sinkf = 0;
-    for (uint32_t base = blockIdx.x * TPB; base < cnt; base += stride) {
-        const uint32_t i = base + threadIdx.x;
-        const bool valid = i < cnt;
+    const Span sp = wg_span(cnt, TPB);
+    for (uint64_t base = sp.b; base < sp.e; base += TPB) {
+        const uint32_t i = static_cast<uint32_t>(base) + threadIdx.x;
+        const bool valid = i < sp.e;
         const uint32_t v = valid ? wl[i] : 0u;
         bool keep, cand[4];
         int4 nb;
@@ -868,18 +899,264 @@ __global__ void __launch_bounds__(TPB) k_relabel(uint32_t n, uint32_t p, const u
     const uint32_t cnt = ctl->rl_cnt[p];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rl_cnt[p ^ 1u] = 0u;
     uint32_t* wl_cnt = &ctl->wl_cnt[p ^ 1u];
-    const uint32_t stride = gridDim.x * TPB;
     unsigned long long relabels = 0;
-    for (uint32_t base = blockIdx.x * TPB; base < cnt; base += stride) {
-        const uint32_t i = base + threadIdx.x;
+    const Span sp = wg_span(cnt, TPB);
+    for (uint64_t base = sp.b; base < sp.e; base += TPB) {
+        const uint32_t i = static_cast<uint32_t>(base) + threadIdx.x;
         bool act = false;
         uint32_t v = 0;
-        if (i < cnt) {
+        if (i < sp.e) {
             v = rl[i];
             act = relabel_vertex<kB>(v, n, nbr4, res, term, excess, height, gid, gn, ctl, relabels);
         }
         q_push(q, act, v);
         q_flush_if(q, wl_next, wl_cnt, TPB);
+    }
+    q_flush(q, wl_next, wl_cnt);
+    flush_stat(ctl, kStRelabels, relabels);
+}
+
+// ------------------------------------------------------ pull-mode sub-rounds
+// Large active lists run sub-rounds without atomics.  In the push phase a
+// vertex writes what it pushes along each slot into its own row of `out` and
+// marks itself and its receivers in the byte map `touched`; it updates only
+// its own residuals, terminal and excess, with plain stores.  The apply
+// phase then sweeps the byte map in id order: each marked vertex reads what
+// each neighbour pushed into it -- the neighbour's `out` entry for the slot
+// that points back to it -- adds that to its excess and reverse residuals,
+// clears the entry, relabels itself, and joins the next active list when
+// still active.  Every entry has exactly one reader (the vertex on that
+// slot), which is marked whenever the entry is non-zero, so every entry is
+// zero again when the phase ends; entries of neighbours that did not push
+// read as zero.  A sub-round computes what a k_push/k_relabel pair computes:
+// heights are read-only while pushing, and the relabel sees every residual
+// change of the sub-round.  The atomic sub-round pays three memory-side
+// atomics per pushing vertex (receiver excess, reverse residual, own
+// excess), each a read-modify-write of a 64-byte line at the memory, and
+// those bound its launches while the list holds millions of vertices; here
+// every access is a plain load or store, and the map sweep costs n bytes.
+constexpr int kPullTPB = 256;
+constexpr uint32_t kPullStep = 16u * kPullTPB;  // map bytes per sweep step (16 per lane)
+
+template <bool kB>
+__global__ void __launch_bounds__(kPullTPB) k_push_pull(uint32_t n, uint32_t p, const uint32_t* __restrict__ wl,
+                                                        const int4* __restrict__ nbr4, uint32_t* __restrict__ res,
+                                                        int32_t* __restrict__ term, long long* __restrict__ excess,
+                                                        const uint32_t* __restrict__ height,
+                                                        uint32_t* __restrict__ out, uint8_t* __restrict__ touched,
+                                                        const uint32_t* __restrict__ gid,
+                                                        const uint32_t* __restrict__ gn,
+                                                        unsigned long long* __restrict__ gflow, Ctl* ctl) {
+    const uint32_t cnt = ctl->wl_cnt[p];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->wl_cnt[p ^ 1u] = 0u;
+    unsigned long long visits = 0, pushes = 0, sinkf = 0;
+    const Span sp = wg_span(cnt, kPullTPB);
+    for (uint64_t base = sp.b; base < sp.e; base += kPullTPB) {
+        const uint32_t i = static_cast<uint32_t>(base) + threadIdx.x;
+        const bool valid = i < sp.e;
+        const uint32_t v = valid ? wl[i] : 0u;
+        // nothing else writes v's row, terminal or excess in this phase
+        long long e = 0;
+        uint32_t h = kSent, sh = 0u;
+        int4 nb = make_int4(-1, -1, -1, -1);
+        uint4 r = make_uint4(0u, 0u, 0u, 0u);
+        int32_t t = 0;
+        if (valid) {
+            e = excess[v];
+            h = height[v];
+            nb = nbr4[v];
+            r = reinterpret_cast<const uint4*>(res)[v];
+            t = term[v];
+            sh = src_h<kB>(v, n, gid, gn);
+        }
+        const bool act = (e > 0) & (h < sh);
+        uint32_t hu[4] = {kSent, kSent, kSent, kSent};
+        unsigned long long sdlt = 0ull;
+        long long pushed = 0;
+        if (act) {
+            ++visits;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int u = slot_of(nb, k);
+                if (u >= 0) hu[k] = height[u];
+            }
+            // v -> sink (gpu_push_relabel_fast.cu:220-236)
+            if (t < 0 && h == 1u) {
+                const long long cap = -static_cast<long long>(t);
+                const long long dlt = e < cap ? e : cap;
+                term[v] = static_cast<int32_t>(static_cast<long long>(t) + dlt);
+                e -= dlt;
+                pushed += dlt;
+                sinkf += static_cast<unsigned long long>(dlt);
+                sdlt = static_cast<unsigned long long>(dlt);
+                ++pushes;
+            }
+        }
+        uint32_t dl[4] = {0u, 0u, 0u, 0u};
+        bool arcs = false;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int u = slot_of(nb, k);
+            const uint32_t rk = slot_of(r, k);
+            if (act && e > 0 && u >= 0 && rk != 0u && h == hu[k] + 1u) {
+                dl[k] = static_cast<long long>(rk) < e ? rk : static_cast<uint32_t>(e);
+                e -= dl[k];
+                pushed += dl[k];
+                arcs = true;
+                ++pushes;
+            }
+        }
+        if (arcs) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (dl[k]) {
+                    res[static_cast<size_t>(v) * 4u + k] = slot_of(r, k) - dl[k];
+                    touched[slot_of(nb, k)] = 1u;
+                }
+            // every entry of the row is zero before this store
+            reinterpret_cast<uint4*>(out)[v] = make_uint4(dl[0], dl[1], dl[2], dl[3]);
+        }
+        if (pushed) excess[v] = e;
+        // still holding excess: relabel-checked in the apply phase
+        if (act && e > 0) touched[v] = 1u;
+        flush_graph_flow<kB>(valid, v, sdlt, gid, gflow);
+    }
+    flush_stat(ctl, kStVisits, visits);
+    flush_stat(ctl, kStPushes, pushes);
+    flush_stat(ctl, kStSinkFlow, sinkf);
+}
+
+// Apply phase of a pull-mode sub-round with parity p: a sweep of the first m
+// bytes of the touched map (clearing it) in id order; every marked vertex
+// takes in what its neighbours pushed into it, relabels, and joins the next
+// active list when active.  Every vertex that holds excess after the push
+// phase is marked: the active ones that kept some and every receiver.
+template <bool kB>
+__global__ void __launch_bounds__(kPullTPB) k_apply_relabel(uint32_t n, uint32_t m, uint32_t p,
+                                                            uint32_t* __restrict__ wl_next,
+                                                            const int4* __restrict__ nbr4,
+                                                            const uint8_t* __restrict__ rslot,
+                                                            uint32_t* __restrict__ res,
+                                                            const int32_t* __restrict__ term,
+                                                            long long* __restrict__ excess, uint32_t* height,
+                                                            uint32_t* __restrict__ out,
+                                                            uint8_t* __restrict__ touched,
+                                                            const uint32_t* __restrict__ gid,
+                                                            const uint32_t* __restrict__ gn, Ctl* ctl) {
+    __shared__ WgQueue<kPullTPB> q;
+    __shared__ uint32_t s_ids[kPullStep];
+    __shared__ uint32_t s_wsum[kPullTPB / 64];
+    __shared__ uint32_t s_total;
+    q_init(q);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rl_cnt[p ^ 1u] = 0u;  // an atomic sub-round may follow
+    uint32_t* wl_cnt = &ctl->wl_cnt[p ^ 1u];
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    unsigned long long relabels = 0;
+    const Span sp = wg_span(m, kPullStep);
+    for (uint64_t base = sp.b; base < sp.e; base += kPullStep) {
+        // 16 map bytes per lane, each 0 or 1
+        const uint64_t o = base + 16u * threadIdx.x;
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+        if (o + 16u <= m) {
+            const uint4 x = *reinterpret_cast<const uint4*>(touched + o);
+            w[0] = x.x;
+            w[1] = x.y;
+            w[2] = x.z;
+            w[3] = x.w;
+        } else {
+            for (uint32_t b = 0; b < 16u; ++b)
+                if (o + b < m && touched[o + b]) w[b >> 2] |= 1u << (8u * (b & 3u));
+        }
+        const uint32_t c = __popc(w[0]) + __popc(w[1]) + __popc(w[2]) + __popc(w[3]);
+        // exclusive prefix of c over the workgroup: lanes, then waves
+        uint32_t incl = c;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(incl, d);
+            if (static_cast<int>(lane) >= d) incl += y;
+        }
+        if (lane == 63u) s_wsum[wave] = incl;
+        __syncthreads();
+        uint32_t pos = incl - c;
+        for (uint32_t x = 0; x < wave; ++x) pos += s_wsum[x];
+        if (threadIdx.x == kPullTPB - 1) s_total = pos + c;
+        if (c) {
+#pragma unroll
+            for (int b = 0; b < 16; ++b)
+                if ((w[b >> 2] >> (8 * (b & 3))) & 1u) s_ids[pos++] = static_cast<uint32_t>(o) + static_cast<uint32_t>(b);
+            if (o + 16u <= m) *reinterpret_cast<uint4*>(touched + o) = make_uint4(0u, 0u, 0u, 0u);
+            else
+                for (uint32_t b = 0; b < 16u; ++b)
+                    if (o + b < m) touched[o + b] = 0u;
+        }
+        __syncthreads();
+        const uint32_t total = s_total;
+        for (uint32_t j = 0; j < total; j += kPullTPB) {
+            const uint32_t k = j + threadIdx.x;
+            bool act = false;
+            uint32_t x = 0;
+            if (k < total) {
+                x = s_ids[k];
+                long long e = excess[x];
+                uint32_t h = height[x];
+                const uint32_t sh = src_h<kB>(x, n, gid, gn);
+                const int32_t t = term[x];
+                const int4 nb = nbr4[x];
+                const uint4 r = reinterpret_cast<const uint4*>(res)[x];
+                const uint32_t rs = rslot[x];
+                // each neighbour's entry for the slot back to x and its
+                // height, all at once
+                uint32_t in[4], hu[4];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const int u = slot_of(nb, kk);
+                    in[kk] = 0u;
+                    hu[kk] = kSent;
+                    if (u >= 0) {
+                        in[kk] = out[static_cast<size_t>(u) * 4u + ((rs >> (2 * kk)) & 3u)];
+                        hu[kk] = height[u];
+                    }
+                }
+                long long gain = 0;
+                uint32_t rr[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk)
+                    if (in[kk] != 0u) {
+                        rr[kk] += in[kk];
+                        gain += in[kk];
+                        res[static_cast<size_t>(x) * 4u + kk] = rr[kk];
+                        out[static_cast<size_t>(slot_of(nb, kk)) * 4u + ((rs >> (2 * kk)) & 3u)] = 0u;
+                    }
+                if (gain) {
+                    e += gain;
+                    excess[x] = e;
+                }
+                // relabel (as relabel_vertex)
+                if ((e > 0) & (h < sh)) {
+                    uint32_t mh = kSent;
+                    if (t < 0) mh = 0u;
+                    else if (t > 0) mh = sh;
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) {
+                        if (slot_of(nb, kk) >= 0 && rr[kk] != 0u) {
+                            const uint32_t y = hu[kk] == kSent ? sh + 1u : hu[kk];
+                            mh = y < mh ? y : mh;
+                        }
+                    }
+                    if (mh == kSent) {
+                        set_error(ctl, 5);
+                    } else if (mh + 1u > h) {
+                        h = mh + 1u;
+                        height[x] = h;
+                        ++relabels;
+                    }
+                    act = h < sh;
+                }
+            }
+            q_push(q, act, x);
+            q_flush_if(q, wl_next, wl_cnt, kPullTPB);
+        }
+        __syncthreads();  // s_ids and s_wsum are rewritten by the next step
     }
     q_flush(q, wl_next, wl_cnt);
     flush_stat(ctl, kStRelabels, relabels);
@@ -1271,7 +1548,7 @@ struct GraphExec {
 
 struct Layout {
     size_t height, excess, rslot, nzm, wl0, wl1, rl, ctl, gid, nbrg, gn, goff, gflow, galive, gres, gfresh, gerr,
-        ares, aterm, total;
+        ares, aterm, pout, ptouched, total;
 };
 
 // One allocation carved into 256-byte aligned arrays.  Batched mode adds the
@@ -1279,7 +1556,8 @@ struct Layout {
 // per-graph tables (node count, offsets, sink flow).  An arena (serving) also
 // holds the residual rows and terminals of its graphs, two more per-slot
 // byte tables (resident, fresh) and a per-slot error word (admission checks).
-Layout layout_for(uint32_t n, uint32_t G, bool arena = false) {
+// Pull-mode sub-rounds add 17 B per node: the push rows and the touched map.
+Layout layout_for(uint32_t n, uint32_t G, bool arena = false, bool pull = true) {
     Layout L{};
     size_t o = 0;
     auto take = [&](size_t bytes) { size_t at = o; o = align_up(o + bytes, 256); return at; };
@@ -1292,6 +1570,10 @@ Layout layout_for(uint32_t n, uint32_t G, bool arena = false) {
     L.rl = take(N * 4u);
     L.rslot = take(N);
     L.nzm = take(N);
+    if (pull) {
+        L.pout = take(N * 16u);
+        L.ptouched = take(N);
+    }
     if (G) {
         L.gid = take(N * 4u);
         L.nbrg = take(N * 16u);
@@ -1381,7 +1663,8 @@ struct SolveExtras {
 
 // X.noalloc: the solve allocates nothing (no device or pinned memory,
 // streams, events or graphs).  X.ws is then a workspace of at least
-// layout_for(n, 0).total bytes, the kernels run on the caller's stream, the
+// layout_for(n, 0, false, false).total bytes (no pull-mode arrays: its sub-rounds
+// are all atomic ones), the kernels run on the caller's stream, the
 // host reads the control block into thread-local pageable memory, and every
 // wait is a stream synchronisation.
 // X.feeder: an arena solve (kB): n is the arena's node capacity, the graphs
@@ -1448,7 +1731,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     const bool trace = trace_env && *trace_env && *trace_env != '0';
 
     const uint32_t G = kB ? (feeder ? X.slots + 1u : B->G) : 0u;
-    const Layout L = layout_for(n, G, feeder != nullptr);
+    const Layout L = layout_for(n, G, feeder != nullptr, !noalloc);
     DeviceBuf ws;
     if (!X.ws) ck(hipMalloc(&ws.p, L.total), "workspace malloc");
     // every exit from here on, failures included, first waits for the kernels
@@ -1463,6 +1746,12 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     uint32_t* rl = reinterpret_cast<uint32_t*>(base + L.rl);
     uint8_t* rslot = reinterpret_cast<uint8_t*>(base + L.rslot);
     uint8_t* nzm = reinterpret_cast<uint8_t*>(base + L.nzm);
+    // pull-mode sub-rounds (every solve that may allocate): lists of more
+    // than pull_min vertices
+    const bool pull_on = !noalloc && prm.pull_min >= 0;
+    const uint64_t pull_min = prm.pull_min > 0 ? static_cast<uint64_t>(prm.pull_min) : kPullMinList;
+    uint32_t* pout = pull_on ? reinterpret_cast<uint32_t*>(base + L.pout) : nullptr;
+    uint8_t* ptouched = pull_on ? reinterpret_cast<uint8_t*>(base + L.ptouched) : nullptr;
     const int4* nbr4 = reinterpret_cast<const int4*>(d_nbr);
     uint32_t* gid = kB ? reinterpret_cast<uint32_t*>(base + L.gid) : nullptr;
     uint32_t* gn = kB ? reinterpret_cast<uint32_t*>(base + L.gn) : nullptr;
@@ -1519,6 +1808,11 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     ktp = &kt;
 
     ck(hipMemsetAsync(ctl, 0, sizeof(Ctl), s), "ctl clear");
+    if (pull_on) {
+        // every push row entry zero, no node marked
+        ck(hipMemsetAsync(pout, 0, static_cast<size_t>(n) * 16u, s), "pull rows clear");
+        ck(hipMemsetAsync(ptouched, 0, static_cast<size_t>(n), s), "pull map clear");
+    }
     std::vector<uint32_t> h_gn;
     if (feeder) {
         // no graph resident: every slot free, the sentinel never resident
@@ -1696,12 +1990,28 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     // replayed from a captured graph (one per starting parity) and pipelined:
     // round r+1 is enqueued before the counters of round r are read, so the
     // device never idles on the host's decision.
-    GraphExec gexec[2];
+    // [mode][starting parity]; mode 1 = pull-mode sub-rounds
+    GraphExec gexec[2][2];
+    // the latest active-list size the host has read; it picks the mode of
+    // the rounds launched next (either mode gives the same results)
+    uint64_t known_active = active;
     auto launch_round = [&]() {
         const uint32_t p0 = static_cast<uint32_t>(pos & 1u);
+        const int mode = pull_on && known_active > pull_min ? 1 : 0;
         auto enqueue = [&](uint64_t pos0) {
             for (int j = 0; j < local_steps; ++j) {
                 const uint32_t p = static_cast<uint32_t>((pos0 + static_cast<uint64_t>(j)) & 1u);
+                if (mode) {
+                    kt.begin(KernelTimer::kPush);
+                    k_push_pull<kB><<<grid, kPullTPB, 0, s>>>(n, p, wl[p], nbr4, d_res, d_term, excess, height, pout,
+                                                              ptouched, gid, gn, gflow, ctl);
+                    kt.end();
+                    kt.begin(KernelTimer::kRelabel);
+                    k_apply_relabel<kB><<<grid, kPullTPB, 0, s>>>(n, n, p, wl[p ^ 1u], nbr4, rslot, d_res, d_term,
+                                                                  excess, height, pout, ptouched, gid, gn, ctl);
+                    kt.end();
+                    continue;
+                }
                 kt.begin(KernelTimer::kPush);
                 launch_wg(wg, [&](auto t) {
                     constexpr int T = decltype(t)::value;
@@ -1719,19 +2029,21 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             }
         };
         if (use_graph) {
-            if (!gexec[p0].exec) {
+            GraphExec& ge = gexec[mode][p0];
+            if (!ge.exec) {
                 hipGraph_t g = nullptr;
                 ck(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
                 enqueue(pos);
                 ck(hipStreamEndCapture(s, &g), "end capture");
-                const hipError_t e = hipGraphInstantiate(&gexec[p0].exec, g, nullptr, nullptr, 0);
+                const hipError_t e = hipGraphInstantiate(&ge.exec, g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
                 ck(e, "graph instantiate");
             }
-            ck(hipGraphLaunch(gexec[p0].exec, s), "graph launch");
+            ck(hipGraphLaunch(ge.exec, s), "graph launch");
         } else {
             enqueue(pos);
         }
+        if (mode) ++st.pull_rounds;
         ck(hipGetLastError(), "discharge");
         sub += static_cast<uint64_t>(local_steps);
         pos += static_cast<uint64_t>(local_steps);
@@ -1863,6 +2175,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             continue;
         }
         // ---- large active list: pipelined macro rounds
+        known_active = active;
         launch_round();
         ++st.rounds;
         Pending prev = snapshot();
@@ -1879,6 +2192,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             const Ctl& S = snap[prev.slot];
             check_device_error(S);
             const uint32_t act_prev = S.wl_cnt[prev.pos_after & 1u];
+            known_active = act_prev;
             relabels_now = stat_sum(S, kStRelabels);
             if (trace)
                 std::fprintf(stderr, "[wtv] round %llu: active=%u relabels=%llu\n",
@@ -2031,7 +2345,8 @@ void pr_solve_noalloc(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_te
             << kNoallocMaxNodes << " nodes";
         fail(WTV_EINVAL, oss.str());
     }
-    if (layout_for(n, 0).total > kScratchBytes) fail(WTV_EINVARIANT, std::string(kPrefix) + "scratch layout");
+    if (layout_for(n, 0, false, false).total > kScratchBytes)
+        fail(WTV_EINVARIANT, std::string(kPrefix) + "scratch layout");
     SolveExtras X;
     X.ws = scratch_of_current_device();
     X.noalloc = true;
@@ -2325,7 +2640,7 @@ class ArenaSession final : public Feeder {
 }  // namespace
 
 uint64_t arena_bytes_per_node() {
-    // rows 36 B, workspace 26 B, graph id 4 B, union ids 16 B ... as layout_for
+    // rows 36 B, workspace 43 B, graph id 4 B, union ids 16 B ... as layout_for
     return layout_for(1u << 20, 2u, true).total / (1u << 20);
 }
 
