@@ -291,10 +291,11 @@ def main():
     # Byte model of the k_bfs_level launches: 34 B per top-down frontier
     # expansion (entry 4, nbr 16, rslot 1, the 4 neighbours' mask bytes 4, and
     # the claim that reached the vertex: CAS 4, mask write 1, append 4); a
-    # bottom-up level costs a 4 B height sweep over all n vertices plus 33 B
-    # (nbr 16 + mask 1 + 4 neighbour heights 16) per unreached vertex it examines.
+    # bottom-up level costs a 1 B mask-byte sweep over all n vertices plus 21 B
+    # (nbr 16 + the 4 neighbours' mask bytes 4 + mask write 1) per unreached
+    # vertex it examines.
     def bfs_bytes(x):
-        return 34 * x["bfs_pops"] + 4 * n * x["bfs_bu_levels"] + 33 * x["bfs_bu_checks"]
+        return 34 * x["bfs_pops"] + n * x["bfs_bu_levels"] + 21 * x["bfs_bu_checks"]
 
     # Whole solve: discharge + BFS levels + the level-1/2 sweep of every relabel
     # (term read 4 B per vertex; height write 4 B per vertex in the first, then
@@ -471,8 +472,8 @@ def main():
                                         "bfs_bu_checks", "bfs_seed_checks", "tail_sub_rounds")},
         "solve_roofline": {"bytes_alg": bytes_alg, "achieved_GBs": round(bytes_alg / t_solve_s / 1e9, 2),
                            "frac": round(bytes_alg / t_solve_s / 1e9 / HBM_PEAK_GBS, 5),
-                           "model": "this solver's counters: 69V + 32P + 4L + BFS(34 per top-down pop, 4n per "
-                                    "bottom-up level + 33 per vertex examined) + 4nG + 4n + 53S + 2n",
+                           "model": "this solver's counters: 69V + 32P + 4L + BFS(34 per top-down pop, n per "
+                                    "bottom-up level + 21 per vertex examined) + 4nG + 4n + 53S + 2n",
                            "v1_bytes": v1_bytes,
                            "v1_frac": round(v1_bytes / t_solve_s / 1e9 / HBM_PEAK_GBS, 5),
                            "v1_model": "69V + 32P + 4L + 61B + 20nG + 2n (B = top-down BFS pops, G = global "

@@ -59,6 +59,11 @@ namespace {
 constexpr uint32_t kSent = 0xFFFFFFFFu;
 // bit 4 of a vertex's residual-mask byte during a BFS: not reached yet
 constexpr uint32_t kUnreached = 16u;
+// bit 5, once reached at level >= 2: the parity of that level
+constexpr uint32_t kParityShift = 5u;
+__device__ __forceinline__ uint8_t reached_byte(uint32_t mask_byte, uint32_t level) {
+    return static_cast<uint8_t>((mask_byte & (kUnreached - 1u)) | ((level & 1u) << kParityShift));
+}
 constexpr int kTPB = 256;
 
 // Device control block: worklist counters, flow, work counters, error word.
@@ -390,7 +395,9 @@ __global__ void k_bfs_begin(Ctl* ctl, uint32_t p) {
 // vertices' residual rows, and a bottom-up level reads 1 byte per vertex
 // where it read 16.  Bit 4 (kUnreached) marks a vertex the BFS has not
 // reached: a top-down probe reads only the neighbour's byte, and touches its
-// height only for the claim itself.
+// height only for the claim itself.  Bit 5 holds the parity of the level that
+// reached the vertex (0 for level 2, which this sweep writes), which is all a
+// bottom-up level needs to know of a reached neighbour.
 constexpr int kSeedUnroll = 4;
 static_assert(kSeedUnroll * kTPB <= WgQueue<kTPB>::kCap, "queue holds one append per vertex per iteration");
 __global__ void __launch_bounds__(kTPB) k_bfs_seed(uint32_t n, const int32_t* __restrict__ term,
@@ -489,7 +496,7 @@ __device__ __forceinline__ void td_claims(bool valid, uint32_t u, uint32_t d, co
         // unreached with a residual arc into u; a stale bit only costs a failed CAS
         if ((mv[k] & kUnreached) != 0u && ((mv[k] >> ((rs >> (2 * k)) & 3u)) & 1u) != 0u) {
             claim[k] = atom_cas(&height[slot_of(nb, k)], kSent, d + 1u);
-            if (claim[k]) nzm[slot_of(nb, k)] = static_cast<uint8_t>(mv[k] & (kUnreached - 1u));
+            if (claim[k]) nzm[slot_of(nb, k)] = reached_byte(mv[k], d + 1u);
         }
     }
 }
@@ -527,21 +534,30 @@ __global__ void __launch_bounds__(TPB) k_bfs_level(uint32_t n, uint32_t d, uint3
         for (uint64_t base = sp.b; base < sp.e; base += TPB) {
             const uint32_t v = static_cast<uint32_t>(base) + threadIdx.x;
             bool found = false;
-            if (v < n && height[v] == kSent) {
+            // A reached vertex u with a residual arc v -> u from an unreached
+            // v sits at level d or d + 1: at a level below d, v would have
+            // been reached at the level after it (level 1, the sink seeds,
+            // makes its neighbours level 2 in the seed sweep).  So the parity
+            // bit of u's byte tells level d from a vertex reached in this
+            // same pass, and the level reads bytes (64 per line) where it
+            // read heights (16 per line).  Bytes of free arena ranges, which
+            // the seed sweep does not write, have no arcs and find nothing.
+            const uint32_t m = v < n ? static_cast<uint32_t>(nzm[v]) : 0u;
+            if ((m & kUnreached) != 0u) {
                 ++work;
                 const int4 nb = nbr4[v];
-                const uint32_t m = nzm[v];
-                uint32_t hu[4];
+                uint32_t mu[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const int u = slot_of(nb, k);
-                    hu[k] = (u >= 0 && ((m >> k) & 1u)) ? height[u] : kSent;
+                    mu[k] = (u >= 0 && ((m >> k) & 1u)) ? static_cast<uint32_t>(nzm[u]) : kUnreached;
                 }
 #pragma unroll
-                for (int k = 0; k < 4; ++k) found |= hu[k] == d;
+                for (int k = 0; k < 4; ++k)
+                    found |= (mu[k] & kUnreached) == 0u && ((mu[k] >> kParityShift) & 1u) == (d & 1u);
                 if (found) {
                     height[v] = d + 1u;
-                    nzm[v] = static_cast<uint8_t>(m & (kUnreached - 1u));
+                    nzm[v] = reached_byte(m, d + 1u);
                 }
             }
             q_push(q, found, v);
