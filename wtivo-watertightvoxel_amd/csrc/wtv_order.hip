@@ -34,6 +34,7 @@
 #include <string>
 
 #include "wtv_internal.h"
+#include "wtv_wgqueue.h"
 
 namespace wtv {
 namespace {
@@ -44,6 +45,7 @@ constexpr uint32_t kLocalSpan = 4096u;     // an arc is local below this id dist
 constexpr uint32_t kSampleRows = 65536u;   // rows the disorder test reads
 constexpr uint32_t kMinAutoNodes = 1u << 20;  // smaller graphs stay in the caches
 constexpr int kMaxLevels = 254;            // distances fit the key's low byte
+constexpr int kListGrid = 2048;            // workgroups of a frontier-list level
 
 inline void cko(hipError_t e, const char* where) {
     if (e == hipSuccess) return;
@@ -86,42 +88,68 @@ __global__ void __launch_bounds__(kOrdTPB) k_locality_sample(uint32_t n, uint32_
     }
 }
 
-// w[v] = distance << 24 | cluster; seeds (v % M == 0) are at distance 0.
-__global__ void __launch_bounds__(kOrdTPB) k_cluster_seed(uint32_t n, uint32_t M, uint32_t* __restrict__ w) {
-    for (uint32_t v = blockIdx.x * kOrdTPB + threadIdx.x; v < n; v += gridDim.x * kOrdTPB)
-        w[v] = v % M == 0u ? v / M : kUnreached;
+// w[v] = distance << 24 | cluster; seeds (v % M == 0) are at distance 0 and
+// form the first frontier, f0[c] = c * M; cnt[0] = their number.
+__global__ void __launch_bounds__(kOrdTPB) k_cluster_seed(uint32_t n, uint32_t M, uint32_t clusters,
+                                                          uint32_t* __restrict__ w, uint32_t* __restrict__ f0,
+                                                          uint32_t* __restrict__ cnt) {
+    for (uint32_t v = blockIdx.x * kOrdTPB + threadIdx.x; v < n; v += gridDim.x * kOrdTPB) {
+        const bool seed = v % M == 0u;
+        w[v] = seed ? v / M : kUnreached;
+        if (seed) f0[v / M] = v;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cnt[0] = clusters;
+        cnt[1] = 0u;
+        cnt[2] = 0u;
+    }
 }
 
-// One BFS level L, pushed from its frontier (the nodes at distance L): each
-// offers distance L + 1 and its cluster to its neighbours with an atomic
-// minimum on the packed word, so a node reached at this level ends with the
-// smallest cluster among its frontier neighbours, whatever the timing.  A
-// node already reached has a smaller word and keeps it; a node reached in
-// this pass is at distance L + 1 and not part of this pass's frontier.  A
-// plain load first skips the atomic where it cannot lower the word.
-// *grown counts the nodes reached (the atomic that found them unreached).
+// One BFS level L, pushed from its frontier list `fin` (the nodes at
+// distance L, cnt[L % 3] of them): each offers distance L + 1 and its
+// cluster to its neighbours with an atomic minimum on the packed word, so a
+// node reached at this level ends with the smallest cluster among its
+// frontier neighbours, whatever the timing.  A node already reached has a
+// smaller word and keeps it.  The atomic that finds a node unreached is
+// unique, and appends it to `fout` (count in cnt[(L + 1) % 3], zeroed by the
+// level before; this level zeroes cnt[(L + 2) % 3] for the next).  A plain
+// load first skips the atomic where it cannot lower the word.  The lists
+// replace a sweep of all n words per level (28 levels on the 77.5M-node
+// Delaunay graph).
 __global__ void __launch_bounds__(kOrdTPB) k_cluster_level(uint32_t n, uint32_t L, const int4* __restrict__ nbr4,
-                                                           uint32_t* __restrict__ w, unsigned int* grown) {
-    unsigned int mine = 0;
-    for (uint32_t v = blockIdx.x * kOrdTPB + threadIdx.x; v < n; v += gridDim.x * kOrdTPB) {
-        const uint32_t x = w[v];
-        if (x == kUnreached || (x >> 24) != L) continue;
-        const uint32_t offer = ((L + 1u) << 24) | (x & 0xFFFFFFu);
-        const int4 nb = nbr4[v];
-        const int ids[4] = {nb.x, nb.y, nb.z, nb.w};
+                                                           uint32_t* __restrict__ w, const uint32_t* __restrict__ fin,
+                                                           uint32_t* __restrict__ fout, uint32_t* cnt) {
+    __shared__ WgQueue<kOrdTPB> q;
+    q_init(q);
+    const uint32_t c = cnt[L % 3u];
+    uint32_t* out_cnt = &cnt[(L + 1u) % 3u];
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[(L + 2u) % 3u] = 0u;
+    const Span sp = wg_span(c, kOrdTPB);
+    for (uint64_t base = sp.b; base < sp.e; base += kOrdTPB) {
+        const uint64_t i = base + threadIdx.x;
+        bool got[4] = {false, false, false, false};
+        int ids[4] = {-1, -1, -1, -1};
+        if (i < sp.e) {
+            const uint32_t u = fin[i];
+            const uint32_t offer = ((L + 1u) << 24) | (w[u] & 0xFFFFFFu);
+            const int4 nb = nbr4[u];
+            ids[0] = nb.x;
+            ids[1] = nb.y;
+            ids[2] = nb.z;
+            ids[3] = nb.w;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (!valid_id(ids[k], n)) continue;
-            uint32_t* t = &w[ids[k]];
-            if (__hip_atomic_load(t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= offer) continue;
-            if (atomicMin(t, offer) == kUnreached) ++mine;
+            for (int k = 0; k < 4; ++k) {
+                if (!valid_id(ids[k], n)) continue;
+                uint32_t* t = &w[ids[k]];
+                if (__hip_atomic_load(t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= offer) continue;
+                got[k] = atomicMin(t, offer) == kUnreached;
+            }
         }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) q_push(q, got[k], static_cast<uint32_t>(ids[k]));
+        q_flush_if(q, fout, out_cnt, 4 * kOrdTPB);
     }
-    // one atomic per wave
-    const unsigned long long lanes = __ballot(mine != 0u);
-    unsigned int sum = mine;
-    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-    if (lanes && (threadIdx.x & 63u) == 0u) atomicAdd(grown, sum);
+    q_flush(q, fout, out_cnt);
 }
 
 // In place: w[v] becomes the sort key (cluster << 8 | distance; unreached
@@ -259,23 +287,26 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
     auto* res_o = reinterpret_cast<uint4*>(base + o_res);
     auto* term_o = reinterpret_cast<int32_t*>(base + o_term);
     auto* part_o = reinterpret_cast<uint8_t*>(base + o_part);
-    auto* grown = reinterpret_cast<unsigned int*>(base + o_cnt);
+    auto* lcnt = reinterpret_cast<uint32_t*>(base + o_cnt);
     const int4* nbr4 = reinterpret_cast<const int4*>(d_nbr);
     const int g = grid_for(n);
 
-    // 1. clusters: levels until none grows (a check every 4 levels)
-    k_cluster_seed<<<g, kOrdTPB, 0, s>>>(n, M, w);
+    // 1. clusters: levels until the frontier empties (a check every 4 levels);
+    // the frontiers ping-pong between the sort's key and permutation arrays,
+    // free until the sort
+    k_cluster_seed<<<g, kOrdTPB, 0, s>>>(n, M, clusters, w, keys, lcnt);
     cko(hipGetLastError(), "cluster seeds");
+    uint32_t* fr[2] = {keys, perm};
     for (int L = 0; L < kMaxLevels;) {
-        cko(hipMemsetAsync(grown, 0, 4, s), "cluster count clear");
         const int batch = std::min(4, kMaxLevels - L);
         for (int i = 0; i < batch; ++i, ++L)
-            k_cluster_level<<<g, kOrdTPB, 0, s>>>(n, static_cast<uint32_t>(L), nbr4, w, grown);
+            k_cluster_level<<<kListGrid, kOrdTPB, 0, s>>>(n, static_cast<uint32_t>(L), nbr4, w, fr[L & 1],
+                                                          fr[(L + 1) & 1], lcnt);
         cko(hipGetLastError(), "cluster level");
-        unsigned int h = 0;
-        cko(hipMemcpyAsync(&h, grown, 4, hipMemcpyDeviceToHost, s), "cluster count D2H");
+        uint32_t h[3] = {0u, 0u, 0u};
+        cko(hipMemcpyAsync(h, lcnt, sizeof(h), hipMemcpyDeviceToHost, s), "cluster count D2H");
         cko(hipStreamSynchronize(s), "cluster levels");
-        if (h == 0u) break;
+        if (h[L % 3] == 0u) break;  // level L's frontier is empty
     }
     // 2. order
     k_order_keys<<<g, kOrdTPB, 0, s>>>(n, clusters, w, ids);
