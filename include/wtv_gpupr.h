@@ -95,6 +95,8 @@ typedef struct wtv_stats {
     uint64_t renumbered;        /* 1: solved as a locality-renumbered copy   */
     uint64_t pull_rounds;       /* macro rounds run as pull-mode sub-rounds
                                    (wtv_params.pull_min)                    */
+    uint64_t fused_rounds;      /* macro rounds run as fused push-relabel
+                                   sub-rounds (wtv_params.fused)            */
 } wtv_stats;
 
 /* Solver knobs.  global_relabel_period, max_rounds and local_steps keep the
@@ -137,6 +139,11 @@ typedef struct wtv_params {
                                       0 = default 4,194,304,
                                       < 0 = off; results are the same; the
                                       allocation-free solve never uses them */
+    int32_t fused;                 /* other multi-workgroup sub-rounds: 0 = one
+                                      kernel per sub-round that pushes,
+                                      relabels by at most 2 and pushes again
+                                      (default); < 0 = a push kernel then a
+                                      relabel kernel; results are the same  */
 } wtv_params;
 
 /* Fill *p with the reference defaults. */

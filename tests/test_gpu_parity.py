@@ -910,14 +910,18 @@ def test_surface_after_graph_cut_and_errors(orc, torch_cuda):
 
 def test_graph_replay_equals_plain_launches(orc, torch_cuda):
     """Macro rounds replayed from a captured hipGraph (default) give the same
-    answer as kernel-by-kernel launches, over many replays and odd/even local_steps."""
+    answer as kernel-by-kernel launches, over many replays, local_steps that
+    start rounds at every list phase (position % 6), and fused or two-kernel
+    sub-rounds."""
     _, _, g = _kuhn_soft(24, 70)
     want = orc.bk_solve(g.nbr, g.res, g.term)
-    for ls in (1, 3, 8):
-        for no_graph in (0, 1):
-            gpu = gpu_solve(torch_cuda, g.nbr, g.res, g.term, local_steps=ls, no_graph=no_graph)
-            assert_same(orc, g.nbr, g.res, g.term, gpu, want.raw_flow, want.partition)
-            assert gpu[1]["rounds"] >= 1
+    for fused in (0, -1):
+        for ls in (1, 3, 4, 8):
+            for no_graph in (0, 1):
+                gpu = gpu_solve(torch_cuda, g.nbr, g.res, g.term, local_steps=ls, no_graph=no_graph, fused=fused,
+                                tail_cap=-1)
+                assert_same(orc, g.nbr, g.res, g.term, gpu, want.raw_flow, want.partition)
+                assert gpu[1]["rounds"] >= 1
 
 
 def test_profile_mode_counts_kernels(torch_cuda):
@@ -925,7 +929,10 @@ def test_profile_mode_counts_kernels(torch_cuda):
     part, st, _, _ = gpu_solve(torch_cuda, g.nbr, g.res, g.term, profile=1)
     assert st["sub_rounds"] > 0 and st["n_level"] > 0 and st["t_level_ms"] > 0
     assert st["n_push"] + st["tail_sub_rounds"] >= st["sub_rounds"]
-    assert st["n_push"] == st["n_relabel"] and (st["n_tail"] > 0) == (st["tail_sub_rounds"] > 0)
+    # fused sub-rounds are one kernel (counted as a push), pull-mode ones two
+    assert st["fused_rounds"] > 0 and st["n_relabel"] == 0 and (st["n_tail"] > 0) == (st["tail_sub_rounds"] > 0)
+    _, st4, _, _ = gpu_solve(torch_cuda, g.nbr, g.res, g.term, profile=1, fused=-1)
+    assert st4["fused_rounds"] == 0 and st4["n_push"] == st4["n_relabel"] > 0
     _, st2, _, _ = gpu_solve(torch_cuda, g.nbr, g.res, g.term, profile=1, tail_cap=-1)
     assert st2["n_tail"] == 0 and st2["n_push"] >= st2["sub_rounds"] and st2["t_push_ms"] > 0
     # the event-pair cost is measured in profile mode only, and is a part of
@@ -935,30 +942,33 @@ def test_profile_mode_counts_kernels(torch_cuda):
     assert st3["t_event_overhead_us"] == 0.0 and st3["n_push"] == 0
 
 
+@pytest.mark.parametrize("fused", [0, -1])
 @pytest.mark.parametrize("pull_min", [1, 64, -1])
-def test_pull_mode_same_answer(orc, torch_cuda, pull_min):
+def test_pull_mode_same_answer(orc, torch_cuda, pull_min, fused):
     """Pull-mode sub-rounds (plain stores and gathers instead of atomics; by
     default only for lists of millions of vertices, here forced down to every
     list of more than pull_min vertices) give BK's flow and labels and a valid
-    maximum preflow, alone, mixed with atomic sub-rounds and with the
-    one-workgroup tail, in single, batched and served solves."""
+    maximum preflow, alone, mixed with fused or two-kernel atomic sub-rounds
+    and with the one-workgroup tail, in single, batched and served solves."""
     _, _, g = _kuhn_soft(24, 81)
     want = orc.bk_solve(g.nbr, g.res, g.term)
     for tail_cap in (-1, 0):
-        gpu = gpu_solve(torch_cuda, g.nbr, g.res, g.term, pull_min=pull_min, tail_cap=tail_cap)
+        gpu = gpu_solve(torch_cuda, g.nbr, g.res, g.term, pull_min=pull_min, tail_cap=tail_cap, fused=fused)
         assert_same(orc, g.nbr, g.res, g.term, gpu, want.raw_flow, want.partition)
         assert (gpu[1]["pull_rounds"] > 0) == (pull_min > 0)
+        if fused < 0:
+            assert gpu[1]["fused_rounds"] == 0
     # the reference's source-height quirk and random graphs
     for seed in range(6):
         nbr, res, term = S.random_graph(int(40 + 60 * seed), 1300 + seed, cap_max=int(3 + 9 * seed))
         a = orc.pr_solve(nbr, res, term)
         if orc.certificate(nbr, res, term, a.res, a.term, a.partition, a.raw_flow) != 0:
             continue
-        gpu = gpu_solve(torch_cuda, nbr, res, term, pull_min=pull_min, tail_cap=-1)
+        gpu = gpu_solve(torch_cuda, nbr, res, term, pull_min=pull_min, tail_cap=-1, fused=fused)
         assert_same(orc, nbr, res, term, gpu, a.raw_flow, a.partition)
     graphs = [S.random_graph(int(200 + 50 * i), 1400 + i, cap_max=30) for i in range(5)]
     graphs.append((g.nbr, g.res, g.term))
-    off, flows, part, _, _, st = gpu_batch(torch_cuda, graphs, pull_min=pull_min, tail_cap=-1)
+    off, flows, part, _, _, st = gpu_batch(torch_cuda, graphs, pull_min=pull_min, tail_cap=-1, fused=fused)
     assert (st.pull_rounds > 0) == (pull_min > 0)
     for i, gr in enumerate(graphs):
         b = orc.bk_solve(*gr)
@@ -968,7 +978,7 @@ def test_pull_mode_same_answer(orc, torch_cuda, pull_min):
     torch_cuda.cuda.synchronize()
     dev = [_dev_graph(torch_cuda, *gr) for gr in graphs]
     torch_cuda.cuda.synchronize()
-    with W.Server(slots=1, max_batch=8, pull_min=pull_min, tail_cap=-1) as srv:
+    with W.Server(slots=1, max_batch=8, pull_min=pull_min, tail_cap=-1, fused=fused) as srv:
         tickets = [srv.submit(gr[2].shape[0], d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), d[3].data_ptr())
                    for gr, d in zip(graphs, dev)]
         res = [srv.wait(t) for t in tickets]

@@ -182,6 +182,10 @@ py::dict stats_dict(const wtv_stats& s) {
     d["tail_sub_rounds"] = s.tail_sub_rounds;
     d["bfs_seed_checks"] = s.bfs_seed_checks;
     d["t_event_overhead_us"] = s.t_event_overhead_us;
+    d["t_order_ms"] = s.t_order_ms;
+    d["renumbered"] = s.renumbered;
+    d["pull_rounds"] = s.pull_rounds;
+    d["fused_rounds"] = s.fused_rounds;
     return d;
 }
 

@@ -80,8 +80,19 @@ enum : int {
 };
 constexpr int kStatSlots = 64;
 
+// Phase of a list position: position % 6 fixes both the active-list buffer
+// (phase & 1) and the count slot (phase % 3).
+__host__ __device__ inline uint32_t lbuf(uint32_t ph) { return ph & 1u; }
+__host__ __device__ inline uint32_t lslot(uint32_t ph, uint32_t ahead = 0u) { return (ph + ahead) % 3u; }
+
 struct Ctl {
-    uint32_t wl_cnt[2];   // active lists W[0], W[1]
+    // Active-list counts, a ring of three indexed by position % 3 (the
+    // lists themselves ping-pong between two buffers by position % 2).  A
+    // sub-round at position t reads lc[t % 3], appends to lc[(t + 1) % 3]
+    // (zero when it starts) and zeroes lc[(t + 2) % 3] for the next one, so
+    // a single kernel can read one count, append to the next and clear the
+    // one after without a separate reset launch.
+    uint32_t lc[3];
     uint32_t rl_cnt[2];   // relabel list counts (ping-pong)
     uint32_t bfs_cnt[3];  // BFS frontier counts, ring indexed by level % 3
     uint32_t bfs_levels;
@@ -250,21 +261,22 @@ __global__ void __launch_bounds__(kTPB) k_prepare(uint32_t lo, uint32_t hi, cons
 __global__ void __launch_bounds__(kTPB) k_clear_alive(uint32_t G, uint8_t* __restrict__ galive) {
     for (uint32_t g = blockIdx.x * kTPB + threadIdx.x; g < G; g += gridDim.x * kTPB) galive[g] = 0u;
 }
-__global__ void __launch_bounds__(kTPB) k_mark_alive(uint32_t p, const uint32_t* __restrict__ wl,
+__global__ void __launch_bounds__(kTPB) k_mark_alive(uint32_t ph, const uint32_t* __restrict__ wl,
                                                      const uint32_t* __restrict__ gid, uint8_t* __restrict__ galive,
                                                      const Ctl* ctl) {
-    const uint32_t cnt = ctl->wl_cnt[p];
+    const uint32_t cnt = ctl->lc[lslot(ph)];
     for (uint32_t i = blockIdx.x * kTPB + threadIdx.x; i < cnt; i += gridDim.x * kTPB) galive[gid[wl[i]]] = 1u;
 }
 
 // BFS seeds: vertices with remaining sink capacity get height 1
 // (gpu_push_relabel_fast.cu:85-101); every other vertex starts unreached.
 // No seed list is built: level 1 always expands bottom-up.
-__global__ void k_bfs_begin(Ctl* ctl, uint32_t p) {
+__global__ void k_bfs_begin(Ctl* ctl, uint32_t ph) {
     ctl->bfs_cnt[0] = 0u;
     ctl->bfs_cnt[1] = 0u;
     ctl->bfs_cnt[2] = 0u;
-    ctl->wl_cnt[p ^ 1u] = 0u;  // receives the filtered active list
+    ctl->lc[lslot(ph, 1u)] = 0u;  // receives the filtered active list
+    ctl->lc[lslot(ph, 2u)] = 0u;  // receives the first sub-round's list
     ctl->rl_cnt[0] = 0u;       // the next push phase starts an empty relabel list
     ctl->rl_cnt[1] = 0u;
     ctl->bfs_levels += 1u;     // level 1 is expanded by k_bfs_seed
@@ -550,7 +562,7 @@ __global__ void __launch_bounds__(kTailTPB) k_bfs_tail(uint32_t d0, uint32_t max
 // Unreached vertices keep the sentinel height, which every reader treats as
 // the reference's n+1 (gpu_push_relabel_fast.cu:141-147).
 template <bool kB, int TPB>
-__global__ void __launch_bounds__(TPB) k_filter_active(uint32_t n, uint32_t p, const uint32_t* __restrict__ src,
+__global__ void __launch_bounds__(TPB) k_filter_active(uint32_t n, uint32_t ph, const uint32_t* __restrict__ src,
                                                         uint32_t* __restrict__ dst,
                                                         const long long* __restrict__ excess,
                                                         const uint32_t* __restrict__ height,
@@ -558,8 +570,8 @@ __global__ void __launch_bounds__(TPB) k_filter_active(uint32_t n, uint32_t p, c
                                                         const uint32_t* __restrict__ gn, Ctl* ctl) {
     __shared__ WgQueue<TPB> q;
     q_init(q);
-    const uint32_t cnt = ctl->wl_cnt[p];
-    uint32_t* out_cnt = &ctl->wl_cnt[p ^ 1u];
+    const uint32_t cnt = ctl->lc[lslot(ph)];
+    uint32_t* out_cnt = &ctl->lc[lslot(ph, 1u)];
     const Span sp = wg_span(cnt, TPB);
     for (uint64_t base = sp.b; base < sp.e; base += TPB) {
         const uint32_t i = static_cast<uint32_t>(base) + threadIdx.x;
@@ -757,9 +769,9 @@ __device__ __forceinline__ bool relabel_vertex(uint32_t v, uint32_t n, const int
     return h < sh;
 }
 
-// Push phase of sub-round with parity p (heights read-only).
+// Push phase of the sub-round with phase ph (heights read-only).
 template <bool kB, int TPB>
-__global__ void __launch_bounds__(TPB) k_push(uint32_t n, uint32_t p, const uint32_t* __restrict__ wl,
+__global__ void __launch_bounds__(TPB) k_push(uint32_t n, uint32_t ph, const uint32_t* __restrict__ wl,
                                                uint32_t* __restrict__ rl, const int4* __restrict__ nbr4,
                                                const uint8_t* __restrict__ rslot, uint32_t* res,
                                                int32_t* __restrict__ term, long long* excess,
@@ -768,9 +780,9 @@ __global__ void __launch_bounds__(TPB) k_push(uint32_t n, uint32_t p, const uint
                                                unsigned long long* __restrict__ gflow, Ctl* ctl) {
     __shared__ WgQueue<TPB> q;
     q_init(q);
-    const uint32_t cnt = ctl->wl_cnt[p];
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->wl_cnt[p ^ 1u] = 0u;
-    uint32_t* rl_cnt = &ctl->rl_cnt[p];
+    const uint32_t cnt = ctl->lc[lslot(ph)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->lc[lslot(ph, 2u)] = 0u;
+    uint32_t* rl_cnt = &ctl->rl_cnt[lbuf(ph)];
     unsigned long long visits = 0, pushes = 0, sinkf = 0;
     const Span sp = wg_span(cnt, TPB);
     for (uint64_t base = sp.b; base < sp.e; base += TPB) {
@@ -795,10 +807,10 @@ __global__ void __launch_bounds__(TPB) k_push(uint32_t n, uint32_t p, const uint
     flush_stat(ctl, kStSinkFlow, sinkf);
 }
 
-// Relabel phase of sub-round with parity p: recompute heights of every vertex
-// that was active or received flow, and rebuild the active list.
+// Relabel phase of the sub-round with phase ph: recompute heights of every
+// vertex that was active or received flow, and rebuild the active list.
 template <bool kB, int TPB>
-__global__ void __launch_bounds__(TPB) k_relabel(uint32_t n, uint32_t p, const uint32_t* __restrict__ rl,
+__global__ void __launch_bounds__(TPB) k_relabel(uint32_t n, uint32_t ph, const uint32_t* __restrict__ rl,
                                                   uint32_t* __restrict__ wl_next, const int4* __restrict__ nbr4,
                                                   const uint32_t* __restrict__ res, const int32_t* __restrict__ term,
                                                   const long long* __restrict__ excess, uint32_t* height,
@@ -806,9 +818,9 @@ __global__ void __launch_bounds__(TPB) k_relabel(uint32_t n, uint32_t p, const u
                                                   const uint32_t* __restrict__ gn, Ctl* ctl) {
     __shared__ WgQueue<TPB> q;
     q_init(q);
-    const uint32_t cnt = ctl->rl_cnt[p];
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rl_cnt[p ^ 1u] = 0u;
-    uint32_t* wl_cnt = &ctl->wl_cnt[p ^ 1u];
+    const uint32_t cnt = ctl->rl_cnt[lbuf(ph)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rl_cnt[lbuf(ph) ^ 1u] = 0u;
+    uint32_t* wl_cnt = &ctl->lc[lslot(ph, 1u)];
     unsigned long long relabels = 0;
     const Span sp = wg_span(cnt, TPB);
     for (uint64_t base = sp.b; base < sp.e; base += TPB) {
@@ -823,6 +835,157 @@ __global__ void __launch_bounds__(TPB) k_relabel(uint32_t n, uint32_t p, const u
         q_flush_if(q, wl_next, wl_cnt, TPB);
     }
     q_flush(q, wl_next, wl_cnt);
+    flush_stat(ctl, kStRelabels, relabels);
+}
+
+// ------------------------------------------------ fused push-relabel sub-rounds
+// One kernel per sub-round: a visit pushes along admissible arcs, relabels
+// itself when it keeps excess -- by at most 2 levels -- and pushes again
+// along the arcs its new height made admissible.  Heights stay a valid
+// labelling with every visit running concurrently: a neighbour u that
+// pushes into v (creating v -> u) read h(u) = h_old(v) + 1 and so
+// h_new(v) <= h_old(v) + 2 = h(u) + 1; a neighbour u that v pushes into after
+// raising (creating u -> v) was read at h_new(v) - 1 and rises by at most 2
+// after that read, to at most h_new(v) + 1; a vertex visits once per
+// sub-round, so it rises once.  Appends go straight to the next active list
+// (keep and first-excess receivers, as k_push).
+template <bool kB, int TPB>
+__global__ void __launch_bounds__(TPB) k_push_fused(uint32_t n, uint32_t ph, const uint32_t* __restrict__ wl,
+                                                     uint32_t* __restrict__ wl_next, const int4* __restrict__ nbr4,
+                                                     const uint8_t* __restrict__ rslot, uint32_t* res, int32_t* term,
+                                                     long long* excess, uint32_t* height,
+                                                     const uint32_t* __restrict__ gid, const uint32_t* __restrict__ gn,
+                                                     unsigned long long* __restrict__ gflow, Ctl* ctl) {
+    __shared__ WgQueue<TPB> q;
+    q_init(q);
+    const uint32_t cnt = ctl->lc[lslot(ph)];
+    uint32_t* out_cnt = &ctl->lc[lslot(ph, 1u)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->lc[lslot(ph, 2u)] = 0u;
+        ctl->rl_cnt[lbuf(ph) ^ 1u] = 0u;  // a two-phase sub-round may follow
+    }
+    unsigned long long visits = 0, pushes = 0, sinkf = 0, relabels = 0;
+    const Span sp = wg_span(cnt, TPB);
+    for (uint64_t base = sp.b; base < sp.e; base += TPB) {
+        const uint32_t i = static_cast<uint32_t>(base) + threadIdx.x;
+        const bool valid = i < sp.e;
+        const uint32_t v = valid ? wl[i] : 0u;
+        long long e = 0;
+        uint32_t h = kSent, sh = 0u, rs = 0u;
+        int4 nb = make_int4(-1, -1, -1, -1);
+        uint4 r = make_uint4(0u, 0u, 0u, 0u);
+        int32_t t = 0;
+        if (valid) {
+            e = atom_load(&excess[v]);
+            h = height[v];
+            nb = nbr4[v];
+            r = reinterpret_cast<const uint4*>(res)[v];
+            rs = rslot[v];
+            t = term[v];
+            sh = src_h<kB>(v, n, gid, gn);
+        }
+        const bool act = (e > 0) & (h < sh);
+        uint32_t hu[4] = {kSent, kSent, kSent, kSent};
+        long long pushed = 0;
+        unsigned long long sdlt = 0ull;
+        if (act) {
+            ++visits;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int u = slot_of(nb, k);
+                if (u >= 0) hu[k] = height[u];
+            }
+            if (t < 0 && h == 1u) {
+                const long long cap = -static_cast<long long>(t);
+                const long long dlt = e < cap ? e : cap;
+                t = static_cast<int32_t>(static_cast<long long>(t) + dlt);
+                term[v] = t;
+                e -= dlt;
+                pushed += dlt;
+                sinkf += static_cast<unsigned long long>(dlt);
+                sdlt = static_cast<unsigned long long>(dlt);
+                ++pushes;
+            }
+        }
+        uint32_t dl[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int u = slot_of(nb, k);
+            const uint32_t rk = slot_of(r, k);
+            if (act && e > 0 && u >= 0 && rk != 0u && h == hu[k] + 1u) {
+                dl[k] = static_cast<long long>(rk) < e ? rk : static_cast<uint32_t>(e);
+                e -= dl[k];
+                pushed += dl[k];
+                ++pushes;
+            }
+        }
+        uint32_t hf = h;
+        if (act && e > 0) {
+            // relabel on what v read: 1 + the lowest residual neighbour (the
+            // sink when term < 0, the source height when term > 0)
+            uint32_t mh = kSent;
+            if (t < 0) mh = 0u;
+            else if (t > 0) mh = sh;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (slot_of(nb, k) >= 0 && slot_of(r, k) - dl[k] != 0u) {
+                    const uint32_t y = hu[k] == kSent ? sh + 1u : hu[k];
+                    mh = y < mh ? y : mh;
+                }
+            if (mh == kSent) {
+                set_error(ctl, 5);
+            } else {
+                hf = mh + 1u < h + 2u ? mh + 1u : h + 2u;
+                if (hf > h) {
+                    height[v] = hf;
+                    ++relabels;
+                }
+                // the arcs the new height made admissible
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int u = slot_of(nb, k);
+                    const uint32_t rk = slot_of(r, k) - dl[k];
+                    if (e > 0 && u >= 0 && rk != 0u && hf == hu[k] + 1u) {
+                        const uint32_t d2 = static_cast<long long>(rk) < e ? rk : static_cast<uint32_t>(e);
+                        dl[k] += d2;
+                        e -= d2;
+                        pushed += d2;
+                        ++pushes;
+                    }
+                }
+            }
+        }
+        // a neighbour may push into the very slot v pushes along in this
+        // sub-round (one of the two raised), so v's own slots take atomic
+        // subtractions too; v's loaded residual is never above the slot's
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (dl[k]) {
+                atom_add(&res[static_cast<size_t>(v) * 4u + k], 0u - dl[k]);
+                const uint32_t back = (rs >> (2 * k)) & 3u;
+                atom_add(&res[static_cast<size_t>(slot_of(nb, k)) * 4u + back], dl[k]);
+            }
+        }
+        long long old[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (dl[k]) old[k] = atom_add(&excess[slot_of(nb, k)], static_cast<long long>(dl[k]));
+        long long own = 0ll;
+        if (pushed) own = atom_add(&excess[v], -pushed);
+        bool cand[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cand[k] = dl[k] != 0u && old[k] == 0ll;
+        const bool keep = act && (pushed == 0 || own != pushed) && hf < sh;
+        flush_graph_flow<kB>(valid, v, sdlt, gid, gflow);
+        q_push(q, keep, v);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) q_push(q, cand[k], static_cast<uint32_t>(slot_of(nb, k)));
+        q_flush_if(q, wl_next, out_cnt, 5 * TPB);
+    }
+    q_flush(q, wl_next, out_cnt);
+    flush_stat(ctl, kStVisits, visits);
+    flush_stat(ctl, kStPushes, pushes);
+    flush_stat(ctl, kStSinkFlow, sinkf);
     flush_stat(ctl, kStRelabels, relabels);
 }
 
@@ -849,7 +1012,7 @@ constexpr int kPullTPB = 256;
 constexpr uint32_t kPullStep = 16u * kPullTPB;  // map bytes per sweep step (16 per lane)
 
 template <bool kB>
-__global__ void __launch_bounds__(kPullTPB) k_push_pull(uint32_t n, uint32_t p, const uint32_t* __restrict__ wl,
+__global__ void __launch_bounds__(kPullTPB) k_push_pull(uint32_t n, uint32_t ph, const uint32_t* __restrict__ wl,
                                                         const int4* __restrict__ nbr4, uint32_t* __restrict__ res,
                                                         int32_t* __restrict__ term, long long* __restrict__ excess,
                                                         const uint32_t* __restrict__ height,
@@ -857,8 +1020,8 @@ __global__ void __launch_bounds__(kPullTPB) k_push_pull(uint32_t n, uint32_t p, 
                                                         const uint32_t* __restrict__ gid,
                                                         const uint32_t* __restrict__ gn,
                                                         unsigned long long* __restrict__ gflow, Ctl* ctl) {
-    const uint32_t cnt = ctl->wl_cnt[p];
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->wl_cnt[p ^ 1u] = 0u;
+    const uint32_t cnt = ctl->lc[lslot(ph)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->lc[lslot(ph, 2u)] = 0u;
     unsigned long long visits = 0, pushes = 0, sinkf = 0;
     const Span sp = wg_span(cnt, kPullTPB);
     for (uint64_t base = sp.b; base < sp.e; base += kPullTPB) {
@@ -942,7 +1105,7 @@ __global__ void __launch_bounds__(kPullTPB) k_push_pull(uint32_t n, uint32_t p, 
 // active list when active.  Every vertex that holds excess after the push
 // phase is marked: the active ones that kept some and every receiver.
 template <bool kB>
-__global__ void __launch_bounds__(kPullTPB) k_apply_relabel(uint32_t n, uint32_t m, uint32_t p,
+__global__ void __launch_bounds__(kPullTPB) k_apply_relabel(uint32_t n, uint32_t m, uint32_t ph,
                                                             uint32_t* __restrict__ wl_next,
                                                             const int4* __restrict__ nbr4,
                                                             const uint8_t* __restrict__ rslot,
@@ -958,8 +1121,8 @@ __global__ void __launch_bounds__(kPullTPB) k_apply_relabel(uint32_t n, uint32_t
     __shared__ uint32_t s_wsum[kPullTPB / 64];
     __shared__ uint32_t s_total;
     q_init(q);
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rl_cnt[p ^ 1u] = 0u;  // an atomic sub-round may follow
-    uint32_t* wl_cnt = &ctl->wl_cnt[p ^ 1u];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rl_cnt[lbuf(ph) ^ 1u] = 0u;  // a two-phase sub-round may follow
+    uint32_t* wl_cnt = &ctl->lc[lslot(ph, 1u)];
     const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
     unsigned long long relabels = 0;
     const Span sp = wg_span(m, kPullStep);
@@ -1092,7 +1255,7 @@ __device__ __forceinline__ void lds_append(bool pred, uint32_t val, uint32_t* li
 // height increases were made (the host then runs a global relabel).  Leaves
 // the counters exactly as the same number of k_push/k_relabel pairs would.
 template <bool kB>
-__global__ void __launch_bounds__(kTailTPB) k_discharge_tail(uint32_t n, uint32_t p0, uint32_t cap, uint32_t max_sub,
+__global__ void __launch_bounds__(kTailTPB) k_discharge_tail(uint32_t n, uint32_t ph0, uint32_t cap, uint32_t max_sub,
                                                          unsigned long long rel_budget, uint32_t* wl0, uint32_t* wl1,
                                                          uint32_t* rl, const int4* __restrict__ nbr4,
                                                          const uint8_t* __restrict__ rslot, uint32_t* res,
@@ -1107,7 +1270,7 @@ __global__ void __launch_bounds__(kTailTPB) k_discharge_tail(uint32_t n, uint32_
     __shared__ unsigned long long s_wrel[kTailTPB / 64];
     uint32_t* wl[2] = {wl0, wl1};
     if (threadIdx.x == 0) {
-        s_cur = ctl->wl_cnt[p0];
+        s_cur = ctl->lc[lslot(ph0)];
         s_rel = 0ull;
     }
     __syncthreads();
@@ -1116,7 +1279,7 @@ __global__ void __launch_bounds__(kTailTPB) k_discharge_tail(uint32_t n, uint32_
     for (;;) {
         const uint32_t cur = s_cur;
         if (cur == 0u || cur > cap || sub >= max_sub || s_rel >= rel_budget) break;
-        const uint32_t p = (p0 + sub) & 1u;
+        const uint32_t p = lbuf(ph0 + sub);
         // push phase
         if (threadIdx.x == 0) s_cnt = 0u;
         __syncthreads();
@@ -1169,9 +1332,8 @@ __global__ void __launch_bounds__(kTailTPB) k_discharge_tail(uint32_t n, uint32_
     flush_stat(ctl, kStSinkFlow, sinkf);
     flush_stat(ctl, kStRelabels, relabels_all);
     if (threadIdx.x == 0) {
-        const uint32_t pf = (p0 + sub) & 1u;
-        ctl->wl_cnt[pf] = s_cur;
-        ctl->wl_cnt[pf ^ 1u] = 0u;
+        ctl->lc[lslot(ph0 + sub)] = s_cur;
+        ctl->lc[lslot(ph0 + sub, 1u)] = 0u;
         ctl->rl_cnt[0] = 0u;
         ctl->rl_cnt[1] = 0u;
         ctl->tail_sub = sub;
@@ -1550,10 +1712,10 @@ struct Feeder {
     virtual ~Feeder() = default;
     // Before a global relabel: admit waiting graphs (their vertices holding
     // source supply are appended to the active list wl[p]) and set hwm.
-    virtual void admit(ArenaView& a, uint32_t p) = 0;
+    virtual void admit(ArenaView& a, uint32_t ph) = 0;
     // After the relabel and the active-list filter: finish every resident
     // graph without an active vertex (its heights are final).
-    virtual void finish(ArenaView& a, uint32_t p) = 0;
+    virtual void finish(ArenaView& a, uint32_t ph) = 0;
     // Requests are waiting that the arena could take now.
     virtual bool pending() = 0;
     // The active list is empty and nothing is resident: block until a graph
@@ -1746,16 +1908,18 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             ck(hipGetLastError(), "globalize");
             nbr4 = nbrg;
         }
-        k_prepare<<<sweep_grid, tpb, 0, s>>>(0u, n, nbr4, d_term, rslot, excess, wl[0], &ctl->wl_cnt[0], ctl,
+        k_prepare<<<sweep_grid, tpb, 0, s>>>(0u, n, nbr4, d_term, rslot, excess, wl[0], &ctl->lc[0], ctl,
                                              nullptr);
         ck(hipGetLastError(), "prepare");
     }
     check_err("prepare");
 
     uint64_t sub = 0;  // sub-rounds run (statistics)
-    // Parity position: the active list is wl[pos & 1].  It advances with every
-    // sub-round and once per active-list filter after a global relabel.
+    // List position: the active list is wl[pos & 1], its count lc[pos % 3].
+    // It advances with every sub-round and once per active-list filter after
+    // a global relabel; kernels take its phase pos % 6.
     uint64_t pos = 0;
+    auto phase = [&](uint64_t at) { return static_cast<uint32_t>(at % 6u); };
     // BFS from the sink.  Its frontiers ping-pong between the relabel list
     // (dead between sub-rounds) and the idle worklist, so the active list
     // wl[pos & 1] survives for the filter that follows.
@@ -1766,7 +1930,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         const auto host_b0 = Clock::now();
         const uint32_t p = static_cast<uint32_t>(pos & 1u);
         uint32_t* bf[2] = {rl, wl[p ^ 1u]};
-        k_bfs_begin<<<1, 1, 0, s>>>(ctl, p);
+        k_bfs_begin<<<1, 1, 0, s>>>(ctl, phase(pos));
         ck(hipGetLastError(), "bfs begin");
         // an arena relabels every resident graph: its seed sweep skips the
         // tiles of free nodes and writes every height of fresh graphs
@@ -1774,7 +1938,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         if (skip) {
             k_clear_alive<<<std::max(1, static_cast<int>(std::min<uint64_t>((G + tpb - 1) / tpb, 1024))), tpb, 0,
                             s>>>(G, galive);
-            k_mark_alive<<<grid, tpb, 0, s>>>(p, wl[p], gid, galive, ctl);
+            k_mark_alive<<<grid, tpb, 0, s>>>(phase(pos), wl[p], gid, galive, ctl);
             ck(hipGetLastError(), "mark alive");
         }
         k_bfs_seed<<<sweep_grid, tpb, 0, s>>>(nn, d_term, nbr4, d_res, height, nzm, bf[0],
@@ -1854,12 +2018,13 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         const uint32_t p = static_cast<uint32_t>(pos & 1u);
         launch_wg(wg, [&](auto t) {
             constexpr int T = decltype(t)::value;
-            k_filter_active<kB, T><<<grid, T, 0, s>>>(n, p, wl[p], wl[p ^ 1u], excess, height, gid, gn, ctl);
+            k_filter_active<kB, T><<<grid, T, 0, s>>>(n, phase(pos), wl[p], wl[p ^ 1u], excess, height, gid, gn,
+                                                      ctl);
         });
         ck(hipGetLastError(), "filter active");
         ++pos;
         read_ctl("active count");
-        return hctl->wl_cnt[p ^ 1u];
+        return hctl->lc[lslot(phase(pos))];
     };
 
     // An arena admits graphs before a relabel (which sets their heights) and
@@ -1867,14 +2032,14 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     // an active vertex: that relabel's BFS ran on its final residual graph.
     auto relabel_and_filter = [&](bool skip_done) -> uint32_t {
         if (feeder) {
-            feeder->admit(av, static_cast<uint32_t>(pos & 1u));
+            feeder->admit(av, phase(pos));
             nn = av.hwm;
             sweep_grid = sweep_grid_of(nn);
             bu_threshold = std::max<uint32_t>(nn / 32u, 4096u);
         }
         global_relabel(skip_done);
         const uint32_t act = filter_active();
-        if (feeder) feeder->finish(av, static_cast<uint32_t>(pos & 1u));
+        if (feeder) feeder->finish(av, phase(pos));
         return act;
     };
 
@@ -1896,28 +2061,40 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     uint64_t pops_at_gr = stat_sum(*hctl, kStBfsPops);
     double last_gr_reach = nn ? static_cast<double>(pops_at_gr) / static_cast<double>(nn) : 0.0;
 
-    // One macro round = local_steps sub-rounds (push + relabel).  Rounds are
-    // replayed from a captured graph (one per starting parity) and pipelined:
-    // round r+1 is enqueued before the counters of round r are read, so the
-    // device never idles on the host's decision.
-    // [mode][starting parity]; mode 1 = pull-mode sub-rounds
-    GraphExec gexec[2][2];
+    // One macro round = local_steps sub-rounds.  Rounds are replayed from a
+    // captured graph (one per mode and starting phase) and pipelined: round
+    // r+1 is enqueued before the counters of round r are read, so the device
+    // never idles on the host's decision.
+    // [mode][starting phase]; mode 0 = fused push-relabel, 1 = pull-mode
+    // sub-rounds, 2 = push and relabel kernels
+    GraphExec gexec[3][6];
+    const int small_mode = prm.fused < 0 ? 2 : 0;
     // the latest active-list size the host has read; it picks the mode of
     // the rounds launched next (either mode gives the same results)
     uint64_t known_active = active;
     auto launch_round = [&]() {
-        const uint32_t p0 = static_cast<uint32_t>(pos & 1u);
-        const int mode = pull_on && known_active > pull_min ? 1 : 0;
+        const int mode = pull_on && known_active > pull_min ? 1 : small_mode;
         auto enqueue = [&](uint64_t pos0) {
             for (int j = 0; j < local_steps; ++j) {
-                const uint32_t p = static_cast<uint32_t>((pos0 + static_cast<uint64_t>(j)) & 1u);
-                if (mode) {
+                const uint32_t ph = phase(pos0 + static_cast<uint64_t>(j));
+                const uint32_t p = lbuf(ph);
+                if (mode == 0) {
                     kt.begin(KernelTimer::kPush);
-                    k_push_pull<kB><<<grid, kPullTPB, 0, s>>>(n, p, wl[p], nbr4, d_res, d_term, excess, height, pout,
+                    launch_wg(wg, [&](auto t) {
+                        constexpr int T = decltype(t)::value;
+                        k_push_fused<kB, T><<<grid, T, 0, s>>>(n, ph, wl[p], wl[p ^ 1u], nbr4, rslot, d_res, d_term,
+                                                                excess, height, gid, gn, gflow, ctl);
+                    });
+                    kt.end();
+                    continue;
+                }
+                if (mode == 1) {
+                    kt.begin(KernelTimer::kPush);
+                    k_push_pull<kB><<<grid, kPullTPB, 0, s>>>(n, ph, wl[p], nbr4, d_res, d_term, excess, height, pout,
                                                               ptouched, gid, gn, gflow, ctl);
                     kt.end();
                     kt.begin(KernelTimer::kRelabel);
-                    k_apply_relabel<kB><<<grid, kPullTPB, 0, s>>>(n, n, p, wl[p ^ 1u], nbr4, rslot, d_res, d_term,
+                    k_apply_relabel<kB><<<grid, kPullTPB, 0, s>>>(n, n, ph, wl[p ^ 1u], nbr4, rslot, d_res, d_term,
                                                                   excess, height, pout, ptouched, gid, gn, ctl);
                     kt.end();
                     continue;
@@ -1925,21 +2102,21 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
                 kt.begin(KernelTimer::kPush);
                 launch_wg(wg, [&](auto t) {
                     constexpr int T = decltype(t)::value;
-                    k_push<kB, T><<<grid, T, 0, s>>>(n, p, wl[p], rl, nbr4, rslot, d_res, d_term, excess, height,
+                    k_push<kB, T><<<grid, T, 0, s>>>(n, ph, wl[p], rl, nbr4, rslot, d_res, d_term, excess, height,
                                                      gid, gn, gflow, ctl);
                 });
                 kt.end();
                 kt.begin(KernelTimer::kRelabel);
                 launch_wg(wg, [&](auto t) {
                     constexpr int T = decltype(t)::value;
-                    k_relabel<kB, T><<<grid, T, 0, s>>>(n, p, rl, wl[p ^ 1u], nbr4, d_res, d_term, excess, height,
+                    k_relabel<kB, T><<<grid, T, 0, s>>>(n, ph, rl, wl[p ^ 1u], nbr4, d_res, d_term, excess, height,
                                                         gid, gn, ctl);
                 });
                 kt.end();
             }
         };
         if (use_graph) {
-            GraphExec& ge = gexec[mode][p0];
+            GraphExec& ge = gexec[mode][phase(pos)];
             if (!ge.exec) {
                 hipGraph_t g = nullptr;
                 ck(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
@@ -1953,7 +2130,8 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
         } else {
             enqueue(pos);
         }
-        if (mode) ++st.pull_rounds;
+        if (mode == 1) ++st.pull_rounds;
+        if (mode == 0) ++st.fused_rounds;
         ck(hipGetLastError(), "discharge");
         sub += static_cast<uint64_t>(local_steps);
         pos += static_cast<uint64_t>(local_steps);
@@ -2059,9 +2237,9 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             const uint32_t tail_in = active;
             const uint64_t tail_rel0 = relabels_now;
             kt.begin(KernelTimer::kTail);
-            k_discharge_tail<kB><<<1, kTailTPB, 0, s>>>(n, static_cast<uint32_t>(pos & 1u), tail_cap, max_sub, budget, wl[0],
-                                                   wl[1], rl, nbr4, rslot, d_res, d_term, excess, height, gid, gn,
-                                                   gflow, ctl);
+            k_discharge_tail<kB><<<1, kTailTPB, 0, s>>>(n, phase(pos), tail_cap, max_sub, budget, wl[0], wl[1], rl,
+                                                        nbr4, rslot, d_res, d_term, excess, height, gid, gn, gflow,
+                                                        ctl);
             kt.end();
             ck(hipGetLastError(), "discharge tail");
             read_ctl("discharge tail");
@@ -2074,7 +2252,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             st.rounds += std::max<uint64_t>(1ull, (ts + static_cast<uint64_t>(local_steps) - 1u) /
                                                       static_cast<uint64_t>(local_steps));
             relabels_now = stat_sum(*hctl, kStRelabels);
-            active = hctl->wl_cnt[pos & 1u];
+            active = hctl->lc[lslot(phase(pos))];
             if (trace)
                 std::fprintf(stderr, "[wtv] tail: active %u -> %u, %u sub-rounds, %llu height increases, %.3f ms\n",
                              tail_in, active, ts, static_cast<unsigned long long>(relabels_now - tail_rel0),
@@ -2101,7 +2279,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             if (ktp) ktp->harvest();
             const Ctl& S = snap[prev.slot];
             check_device_error(S);
-            const uint32_t act_prev = S.wl_cnt[prev.pos_after & 1u];
+            const uint32_t act_prev = S.lc[lslot(phase(prev.pos_after))];
             known_active = act_prev;
             relabels_now = stat_sum(S, kStRelabels);
             if (trace)
@@ -2125,7 +2303,7 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
                 ev_wait(*cur.ev, "switch");
                 const Ctl& C2 = snap[cur.slot];
                 check_device_error(C2);
-                active = C2.wl_cnt[cur.pos_after & 1u];
+                active = C2.lc[lslot(phase(cur.pos_after))];
                 relabels_now = stat_sum(C2, kStRelabels);
                 break;
             }
@@ -2305,14 +2483,14 @@ class ArenaSession final : public Feeder {
         for (uint32_t g = slots; g > 0; --g) free_slots_.push_back(g - 1u);
     }
 
-    void admit(ArenaView& a, uint32_t p) override {
+    void admit(ArenaView& a, uint32_t ph) override {
         a.hwm = hwm_;
         if (alone_) return;  // a graph solved alone admits no other
         std::vector<SessionJob> jobs = q_.take(largest_free(), static_cast<uint32_t>(free_slots_.size()),
                                                !res_.empty());
         if (jobs.empty()) return;
         try {
-            admit_round(a, p, jobs);
+            admit_round(a, ph, jobs);
         } catch (...) {
             // the requests taken and not yet resident or finished are not
             // lost: back to the queue, to be solved alone
@@ -2325,12 +2503,12 @@ class ArenaSession final : public Feeder {
         a.hwm = hwm_;
     }
 
-    void finish(ArenaView& a, uint32_t p) override {
+    void finish(ArenaView& a, uint32_t ph) override {
         if (res_.empty()) return;
         const uint32_t G = slots_ + 1u;
         const int gg = std::max(1, static_cast<int>(std::min<uint32_t>((G + kTPB - 1u) / kTPB, 1024u)));
         k_clear_alive<<<gg, kTPB, 0, a.s>>>(G, a.galive);
-        k_mark_alive<<<1024, kTPB, 0, a.s>>>(p, a.wl[p], a.gid, a.galive, a.ctl);
+        k_mark_alive<<<1024, kTPB, 0, a.s>>>(ph, a.wl[lbuf(ph)], a.gid, a.galive, a.ctl);
         k_fill_u8<<<gg, kTPB, 0, a.s>>>(a.gfresh, G, 0u);
         ck(hipGetLastError(), "arena alive marks");
         alive_.resize(G);
@@ -2440,7 +2618,7 @@ class ArenaSession final : public Feeder {
     // the words then fails the bad requests alone, before any of their
     // vertices joins the active list, and the rest enter the solve.  A job
     // whose ticket is set to 0 has been finished or made resident.
-    void admit_round(ArenaView& a, uint32_t p, std::vector<SessionJob>& jobs) {
+    void admit_round(ArenaView& a, uint32_t ph, std::vector<SessionJob>& jobs) {
         // an error of the discharge so far belongs to the shared solve, not
         // to these requests: the solve fails and its graphs are solved alone
         if (const int32_t e0 = read_error(a, "admit check")) {
@@ -2495,8 +2673,8 @@ class ArenaSession final : public Feeder {
                 release(a, r);
                 continue;
             }
-            k_prepare<<<grid_of(r.ag), kTPB, 0, a.s>>>(r.lo, r.lo + r.ag, a.nbrg, a.term, a.rslot, a.excess, a.wl[p],
-                                                       &a.ctl->wl_cnt[p], a.ctl, a.gerr + r.slot);
+            k_prepare<<<grid_of(r.ag), kTPB, 0, a.s>>>(r.lo, r.lo + r.ag, a.nbrg, a.term, a.rslot, a.excess,
+                                                       a.wl[lbuf(ph)], &a.ctl->lc[lslot(ph)], a.ctl, a.gerr + r.slot);
             ck(hipGetLastError(), "admit supply");
             res_.push_back(r);
             if (r.job.alone) alone_ = true;

@@ -36,6 +36,7 @@ class WtvStats(C.Structure):
         ("t_tail_ms", C.c_double), ("n_tail", C.c_uint64), ("tail_sub_rounds", C.c_uint64),
         ("bfs_seed_checks", C.c_uint64), ("t_event_overhead_us", C.c_double),
         ("t_order_ms", C.c_double), ("renumbered", C.c_uint64), ("pull_rounds", C.c_uint64),
+        ("fused_rounds", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -48,7 +49,7 @@ class WtvParams(C.Structure):
         ("max_rounds", C.c_uint64), ("wg_size", C.c_int32), ("grid_wgs", C.c_int32),
         ("time_limit_s", C.c_double), ("profile", C.c_int32), ("no_graph", C.c_int32),
         ("tail_cap", C.c_int32), ("relabel_alpha", C.c_double), ("renumber", C.c_int32),
-        ("skip_finished", C.c_int32), ("pull_min", C.c_int32),
+        ("skip_finished", C.c_int32), ("pull_min", C.c_int32), ("fused", C.c_int32),
     ]
 
 
