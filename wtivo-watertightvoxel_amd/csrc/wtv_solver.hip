@@ -849,6 +849,125 @@ __global__ void __launch_bounds__(TPB) k_relabel(uint32_t n, uint32_t ph, const 
 // after that read, to at most h_new(v) + 1; a vertex visits once per
 // sub-round, so it rises once.  Appends go straight to the next active list
 // (keep and first-excess receivers, as k_push).
+
+// One fused visit of v (all lanes call it; `valid` false for padding lanes).
+// keep: v stays active and joins the next list; cand[k]: the receiver on
+// slot k had no excess before this visit's push (it joins the next list).
+template <bool kB>
+__device__ __forceinline__ void fused_vertex(bool valid, uint32_t v, uint32_t n, const int4* __restrict__ nbr4,
+                                             const uint8_t* __restrict__ rslot, uint32_t* res, int32_t* term,
+                                             long long* excess, uint32_t* height, const uint32_t* __restrict__ gid,
+                                             const uint32_t* __restrict__ gn, Ctl* ctl, bool& keep, bool cand[4],
+                                             int4& nb, unsigned long long& visits, unsigned long long& pushes,
+                                             unsigned long long& sinkf, unsigned long long& relabels,
+                                             unsigned long long& sdlt) {
+    long long e = 0;
+    uint32_t h = kSent, sh = 0u, rs = 0u;
+    nb = make_int4(-1, -1, -1, -1);
+    uint4 r = make_uint4(0u, 0u, 0u, 0u);
+    int32_t t = 0;
+    if (valid) {
+        e = atom_load(&excess[v]);
+        h = height[v];
+        nb = nbr4[v];
+        r = reinterpret_cast<const uint4*>(res)[v];
+        rs = rslot[v];
+        t = term[v];
+        sh = src_h<kB>(v, n, gid, gn);
+    }
+    const bool act = (e > 0) & (h < sh);
+    uint32_t hu[4] = {kSent, kSent, kSent, kSent};
+    long long pushed = 0;
+    sdlt = 0ull;
+    if (act) {
+        ++visits;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int u = slot_of(nb, k);
+            if (u >= 0) hu[k] = height[u];
+        }
+        if (t < 0 && h == 1u) {
+            const long long cap = -static_cast<long long>(t);
+            const long long dlt = e < cap ? e : cap;
+            t = static_cast<int32_t>(static_cast<long long>(t) + dlt);
+            term[v] = t;
+            e -= dlt;
+            pushed += dlt;
+            sinkf += static_cast<unsigned long long>(dlt);
+            sdlt = static_cast<unsigned long long>(dlt);
+            ++pushes;
+        }
+    }
+    uint32_t dl[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int u = slot_of(nb, k);
+        const uint32_t rk = slot_of(r, k);
+        if (act && e > 0 && u >= 0 && rk != 0u && h == hu[k] + 1u) {
+            dl[k] = static_cast<long long>(rk) < e ? rk : static_cast<uint32_t>(e);
+            e -= dl[k];
+            pushed += dl[k];
+            ++pushes;
+        }
+    }
+    uint32_t hf = h;
+    if (act && e > 0) {
+        // relabel on what v read: 1 + the lowest residual neighbour (the
+        // sink when term < 0, the source height when term > 0)
+        uint32_t mh = kSent;
+        if (t < 0) mh = 0u;
+        else if (t > 0) mh = sh;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (slot_of(nb, k) >= 0 && slot_of(r, k) - dl[k] != 0u) {
+                const uint32_t y = hu[k] == kSent ? sh + 1u : hu[k];
+                mh = y < mh ? y : mh;
+            }
+        if (mh == kSent) {
+            set_error(ctl, 5);
+        } else {
+            hf = mh + 1u < h + 2u ? mh + 1u : h + 2u;
+            if (hf > h) {
+                height[v] = hf;
+                ++relabels;
+            }
+            // the arcs the new height made admissible
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int u = slot_of(nb, k);
+                const uint32_t rk = slot_of(r, k) - dl[k];
+                if (e > 0 && u >= 0 && rk != 0u && hf == hu[k] + 1u) {
+                    const uint32_t d2 = static_cast<long long>(rk) < e ? rk : static_cast<uint32_t>(e);
+                    dl[k] += d2;
+                    e -= d2;
+                    pushed += d2;
+                    ++pushes;
+                }
+            }
+        }
+    }
+    // a neighbour may push into the very slot v pushes along in this
+    // sub-round (one of the two raised), so v's own slots take atomic
+    // subtractions too; v's loaded residual is never above the slot's
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (dl[k]) {
+            atom_add(&res[static_cast<size_t>(v) * 4u + k], 0u - dl[k]);
+            const uint32_t back = (rs >> (2 * k)) & 3u;
+            atom_add(&res[static_cast<size_t>(slot_of(nb, k)) * 4u + back], dl[k]);
+        }
+    }
+    long long old[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (dl[k]) old[k] = atom_add(&excess[slot_of(nb, k)], static_cast<long long>(dl[k]));
+    long long own = 0ll;
+    if (pushed) own = atom_add(&excess[v], -pushed);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cand[k] = dl[k] != 0u && old[k] == 0ll;
+    keep = act && (pushed == 0 || own != pushed) && hf < sh;
+}
+
 template <bool kB, int TPB>
 __global__ void __launch_bounds__(TPB) k_push_fused(uint32_t n, uint32_t ph, const uint32_t* __restrict__ wl,
                                                      uint32_t* __restrict__ wl_next, const int4* __restrict__ nbr4,
@@ -870,112 +989,11 @@ __global__ void __launch_bounds__(TPB) k_push_fused(uint32_t n, uint32_t ph, con
         const uint32_t i = static_cast<uint32_t>(base) + threadIdx.x;
         const bool valid = i < sp.e;
         const uint32_t v = valid ? wl[i] : 0u;
-        long long e = 0;
-        uint32_t h = kSent, sh = 0u, rs = 0u;
-        int4 nb = make_int4(-1, -1, -1, -1);
-        uint4 r = make_uint4(0u, 0u, 0u, 0u);
-        int32_t t = 0;
-        if (valid) {
-            e = atom_load(&excess[v]);
-            h = height[v];
-            nb = nbr4[v];
-            r = reinterpret_cast<const uint4*>(res)[v];
-            rs = rslot[v];
-            t = term[v];
-            sh = src_h<kB>(v, n, gid, gn);
-        }
-        const bool act = (e > 0) & (h < sh);
-        uint32_t hu[4] = {kSent, kSent, kSent, kSent};
-        long long pushed = 0;
-        unsigned long long sdlt = 0ull;
-        if (act) {
-            ++visits;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int u = slot_of(nb, k);
-                if (u >= 0) hu[k] = height[u];
-            }
-            if (t < 0 && h == 1u) {
-                const long long cap = -static_cast<long long>(t);
-                const long long dlt = e < cap ? e : cap;
-                t = static_cast<int32_t>(static_cast<long long>(t) + dlt);
-                term[v] = t;
-                e -= dlt;
-                pushed += dlt;
-                sinkf += static_cast<unsigned long long>(dlt);
-                sdlt = static_cast<unsigned long long>(dlt);
-                ++pushes;
-            }
-        }
-        uint32_t dl[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int u = slot_of(nb, k);
-            const uint32_t rk = slot_of(r, k);
-            if (act && e > 0 && u >= 0 && rk != 0u && h == hu[k] + 1u) {
-                dl[k] = static_cast<long long>(rk) < e ? rk : static_cast<uint32_t>(e);
-                e -= dl[k];
-                pushed += dl[k];
-                ++pushes;
-            }
-        }
-        uint32_t hf = h;
-        if (act && e > 0) {
-            // relabel on what v read: 1 + the lowest residual neighbour (the
-            // sink when term < 0, the source height when term > 0)
-            uint32_t mh = kSent;
-            if (t < 0) mh = 0u;
-            else if (t > 0) mh = sh;
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (slot_of(nb, k) >= 0 && slot_of(r, k) - dl[k] != 0u) {
-                    const uint32_t y = hu[k] == kSent ? sh + 1u : hu[k];
-                    mh = y < mh ? y : mh;
-                }
-            if (mh == kSent) {
-                set_error(ctl, 5);
-            } else {
-                hf = mh + 1u < h + 2u ? mh + 1u : h + 2u;
-                if (hf > h) {
-                    height[v] = hf;
-                    ++relabels;
-                }
-                // the arcs the new height made admissible
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int u = slot_of(nb, k);
-                    const uint32_t rk = slot_of(r, k) - dl[k];
-                    if (e > 0 && u >= 0 && rk != 0u && hf == hu[k] + 1u) {
-                        const uint32_t d2 = static_cast<long long>(rk) < e ? rk : static_cast<uint32_t>(e);
-                        dl[k] += d2;
-                        e -= d2;
-                        pushed += d2;
-                        ++pushes;
-                    }
-                }
-            }
-        }
-        // a neighbour may push into the very slot v pushes along in this
-        // sub-round (one of the two raised), so v's own slots take atomic
-        // subtractions too; v's loaded residual is never above the slot's
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (dl[k]) {
-                atom_add(&res[static_cast<size_t>(v) * 4u + k], 0u - dl[k]);
-                const uint32_t back = (rs >> (2 * k)) & 3u;
-                atom_add(&res[static_cast<size_t>(slot_of(nb, k)) * 4u + back], dl[k]);
-            }
-        }
-        long long old[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (dl[k]) old[k] = atom_add(&excess[slot_of(nb, k)], static_cast<long long>(dl[k]));
-        long long own = 0ll;
-        if (pushed) own = atom_add(&excess[v], -pushed);
-        bool cand[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) cand[k] = dl[k] != 0u && old[k] == 0ll;
-        const bool keep = act && (pushed == 0 || own != pushed) && hf < sh;
+        bool keep, cand[4];
+        int4 nb;
+        unsigned long long sdlt;
+        fused_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, gid, gn, ctl, keep, cand, nb, visits,
+                         pushes, sinkf, relabels, sdlt);
         flush_graph_flow<kB>(valid, v, sdlt, gid, gflow);
         q_push(q, keep, v);
 #pragma unroll
@@ -1249,12 +1267,12 @@ __device__ __forceinline__ void lds_append(bool pred, uint32_t val, uint32_t* li
 }
 
 // Discharge tail: when the active list is small, one workgroup runs the same
-// synchronous sub-rounds back to back, with __syncthreads between the push
-// and relabel phases instead of kernel boundaries.  Stops when the list
+// sub-rounds back to back -- fused visits (kF) or a push and a relabel phase
+// -- with __syncthreads instead of kernel boundaries.  Stops when the list
 // empties, grows beyond `cap`, after max_sub sub-rounds, or once rel_budget
 // height increases were made (the host then runs a global relabel).  Leaves
-// the counters exactly as the same number of k_push/k_relabel pairs would.
-template <bool kB>
+// the counters exactly as the same number of multi-workgroup sub-rounds would.
+template <bool kB, bool kF>
 __global__ void __launch_bounds__(kTailTPB) k_discharge_tail(uint32_t n, uint32_t ph0, uint32_t cap, uint32_t max_sub,
                                                          unsigned long long rel_budget, uint32_t* wl0, uint32_t* wl1,
                                                          uint32_t* rl, const int4* __restrict__ nbr4,
@@ -1280,39 +1298,59 @@ __global__ void __launch_bounds__(kTailTPB) k_discharge_tail(uint32_t n, uint32_
         const uint32_t cur = s_cur;
         if (cur == 0u || cur > cap || sub >= max_sub || s_rel >= rel_budget) break;
         const uint32_t p = lbuf(ph0 + sub);
-        // push phase
-        if (threadIdx.x == 0) s_cnt = 0u;
-        __syncthreads();
-        for (uint32_t base = 0; base < cur; base += kTailTPB) {
-            const uint32_t i = base + threadIdx.x;
-            const bool valid = i < cur;
-            const uint32_t v = valid ? wl[p][i] : 0u;
-            bool keep, cand[4];
-            int4 nb;
-            unsigned long long sdlt;
-            push_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, gid, gn, keep, cand, nb,
-                            visits, pushes, sinkf, sdlt);
-            flush_graph_flow<kB>(valid, v, sdlt, gid, gflow);
-            lds_append(keep, v, rl, &s_cnt);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) lds_append(cand[k], static_cast<uint32_t>(slot_of(nb, k)), rl, &s_cnt);
-        }
-        __syncthreads();
-        const uint32_t rcnt = s_cnt;
-        __syncthreads();
-        // relabel phase
-        if (threadIdx.x == 0) s_cnt = 0u;
-        __syncthreads();
         unsigned long long rel = 0;
-        for (uint32_t base = 0; base < rcnt; base += kTailTPB) {
-            const uint32_t i = base + threadIdx.x;
-            bool act = false;
-            uint32_t v = 0;
-            if (i < rcnt) {
-                v = rl[i];
-                act = relabel_vertex<kB>(v, n, nbr4, res, term, excess, height, gid, gn, ctl, rel);
+        if constexpr (kF) {
+            if (threadIdx.x == 0) s_cnt = 0u;
+            __syncthreads();
+            for (uint32_t base = 0; base < cur; base += kTailTPB) {
+                const uint32_t i = base + threadIdx.x;
+                const bool valid = i < cur;
+                const uint32_t v = valid ? wl[p][i] : 0u;
+                bool keep, cand[4];
+                int4 nb;
+                unsigned long long sdlt;
+                fused_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, gid, gn, ctl, keep, cand, nb,
+                                 visits, pushes, sinkf, rel, sdlt);
+                flush_graph_flow<kB>(valid, v, sdlt, gid, gflow);
+                lds_append(keep, v, wl[p ^ 1u], &s_cnt);
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    lds_append(cand[k], static_cast<uint32_t>(slot_of(nb, k)), wl[p ^ 1u], &s_cnt);
             }
-            lds_append(act, v, wl[p ^ 1u], &s_cnt);
+        } else {
+            // push phase
+            if (threadIdx.x == 0) s_cnt = 0u;
+            __syncthreads();
+            for (uint32_t base = 0; base < cur; base += kTailTPB) {
+                const uint32_t i = base + threadIdx.x;
+                const bool valid = i < cur;
+                const uint32_t v = valid ? wl[p][i] : 0u;
+                bool keep, cand[4];
+                int4 nb;
+                unsigned long long sdlt;
+                push_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, gid, gn, keep, cand, nb,
+                                visits, pushes, sinkf, sdlt);
+                flush_graph_flow<kB>(valid, v, sdlt, gid, gflow);
+                lds_append(keep, v, rl, &s_cnt);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) lds_append(cand[k], static_cast<uint32_t>(slot_of(nb, k)), rl, &s_cnt);
+            }
+            __syncthreads();
+            const uint32_t rcnt = s_cnt;
+            __syncthreads();
+            // relabel phase
+            if (threadIdx.x == 0) s_cnt = 0u;
+            __syncthreads();
+            for (uint32_t base = 0; base < rcnt; base += kTailTPB) {
+                const uint32_t i = base + threadIdx.x;
+                bool act = false;
+                uint32_t v = 0;
+                if (i < rcnt) {
+                    v = rl[i];
+                    act = relabel_vertex<kB>(v, n, nbr4, res, term, excess, height, gid, gn, ctl, rel);
+                }
+                lds_append(act, v, wl[p ^ 1u], &s_cnt);
+            }
         }
         relabels_all += rel;
         rel = wave_sum(rel);
@@ -2237,9 +2275,14 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
             const uint32_t tail_in = active;
             const uint64_t tail_rel0 = relabels_now;
             kt.begin(KernelTimer::kTail);
-            k_discharge_tail<kB><<<1, kTailTPB, 0, s>>>(n, phase(pos), tail_cap, max_sub, budget, wl[0], wl[1], rl,
-                                                        nbr4, rslot, d_res, d_term, excess, height, gid, gn, gflow,
-                                                        ctl);
+            if (small_mode == 0)
+                k_discharge_tail<kB, true><<<1, kTailTPB, 0, s>>>(n, phase(pos), tail_cap, max_sub, budget, wl[0],
+                                                                  wl[1], rl, nbr4, rslot, d_res, d_term, excess,
+                                                                  height, gid, gn, gflow, ctl);
+            else
+                k_discharge_tail<kB, false><<<1, kTailTPB, 0, s>>>(n, phase(pos), tail_cap, max_sub, budget, wl[0],
+                                                                   wl[1], rl, nbr4, rslot, d_res, d_term, excess,
+                                                                   height, gid, gn, gflow, ctl);
             kt.end();
             ck(hipGetLastError(), "discharge tail");
             read_ctl("discharge tail");
