@@ -311,6 +311,10 @@ def main():
     roofline = None
     kernels = {}
     if prof:
+        # "push": the discharge's push launches -- k_push_fused by default
+        # (pushes, a raise of at most 2 and the pushes it allows, in one
+        # kernel), k_push with fused sub-rounds off, and the pull-mode
+        # k_push_pull.  The model charges the push-side bytes only.
         kernels = {
             "push": (prof["t_push_ms"], prof["n_push"], 69 * prof["visits"] + 32 * prof["pushes"]),
             "bfs_level": (prof["t_level_ms"], prof["n_level"], bfs_bytes(prof)),
@@ -338,14 +342,15 @@ def main():
                 traffic = json.load(open(tp)).get(name, {}).get("hbm_bytes_per_launch")
             except Exception:
                 traffic = None
+        kname = "k_push_fused" if name == "push" and prof.get("fused_rounds", 0) > 0 else f"k_{name}"
         kp = os.path.join(ROOT, "profiles", f"kernels_{meta['workload']}.json")
         if os.path.exists(kp):
             try:
-                traced_ms = json.load(open(kp)).get(f"k_{name}", {}).get("avg_us")
+                traced_ms = json.load(open(kp)).get(kname, {}).get("avg_us")
                 traced_ms = traced_ms / 1e3 if traced_ms else None
             except Exception:
                 traced_ms = None
-        roofline = {"bound": "hbm", "kernel": f"k_{name}", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+        roofline = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                     "algorithmic_bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg_ms, 5),
                     "avg_launch_ms_with_event_pair": round(avg_events_ms, 5),
@@ -377,9 +382,11 @@ def main():
     atomics = None
     if prof:
         probe = atomic_probe(torch)
-        # per push: reverse-residual add + receiver excess add; per visit: its
-        # own excess subtraction (skipped by the few visits that push nothing)
-        am = (2 * prof["pushes"] + prof["visits"]) / max(prof["n_push"], 1)
+        # per push: reverse-residual add + receiver excess add (+ the own
+        # residual subtraction, atomic in fused sub-rounds); per visit: its own
+        # excess subtraction (skipped by the few visits that push nothing)
+        fused = prof.get("fused_rounds", 0) > 0
+        am = ((3 if fused else 2) * prof["pushes"] + prof["visits"]) / max(prof["n_push"], 1)
         push_rate = am / (max(prof["t_push_ms"] / max(prof["n_push"], 1)
                               - prof.get("t_event_overhead_us", 0.0) / 1e3, 1e-6) * 1e-3)
         counters = None
@@ -390,7 +397,8 @@ def main():
             except Exception:
                 counters = None
         atomics = {"probe": probe,
-                   "k_push": {"atomic_ops_per_launch_model": int(am), "atomic_ops_per_s": round(push_rate, 1),
+                   "push": {"kernel": "k_push_fused" if fused else "k_push",
+                            "atomic_ops_per_launch_model": int(am), "atomic_ops_per_s": round(push_rate, 1),
                               "frac_of_probe_u32_random": round(push_rate / probe["u32_random_atomics_per_s"], 4)},
                    "counters": counters}
 
