@@ -135,9 +135,11 @@ typedef struct wtv_params {
                                       (same results, more work)             */
     int32_t pull_min;              /* active lists of more than this many
                                       vertices run pull-mode sub-rounds (plain
-                                      stores and gathers instead of atomics);
-                                      0 = default 4,194,304,
-                                      < 0 = off; results are the same; the
+                                      stores and gathers instead of atomics,
+                                      17 B/node more workspace); 0 = default:
+                                      off with fused sub-rounds, 4,194,304
+                                      with them off (fused < 0); < 0 = off;
+                                      results are the same; the
                                       allocation-free solve never uses them */
     int32_t fused;                 /* other multi-workgroup sub-rounds: 0 = one
                                       kernel per sub-round that pushes,
@@ -313,8 +315,9 @@ typedef struct wtv_result {
     double end_s;
 } wtv_result;
 
-/* Arena bytes a request of n nodes occupies: n rounded up to 1024 nodes, times
- * the arena's bytes per node (graph copy + solver state). */
+/* Arena bytes a request of n nodes occupies under the default knobs: n rounded
+ * up to 1024 nodes, times the arena's bytes per node (graph copy + solver
+ * state; a server with pull-mode sub-rounds on adds 17 B per node). */
 uint64_t wtv_request_footprint(uint32_t n);
 
 /* Create a server on the current device; NULL on failure (see wtv_last_error). */

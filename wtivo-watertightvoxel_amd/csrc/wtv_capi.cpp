@@ -13,7 +13,7 @@
 
 namespace wtv {
 class Server;
-uint64_t request_footprint(uint32_t n);
+uint64_t request_footprint(uint32_t n, bool pull);
 Server* server_new(const wtv_server_config& c);
 void server_delete(Server* s);
 uint64_t server_submit(Server* s, const wtv_request& r);
@@ -231,7 +231,11 @@ int wtv_udf_sample_tet_labels(const void* udf, const double* vertices, size_t nv
     });
 }
 
-uint64_t wtv_request_footprint(uint32_t n) { return wtv::request_footprint(n); }
+uint64_t wtv_request_footprint(uint32_t n) {
+    wtv_params p;
+    wtv_params_default(&p);
+    return wtv::request_footprint(n, wtv::pull_enabled(p));
+}
 
 void* wtv_server_create(const wtv_server_config* cfg) {
     wtv::Server* srv = nullptr;

@@ -659,12 +659,13 @@ void graph_cut_fast(const double* V, size_t nv, const int32_t* dT, const int32_t
     }
 
     // VRAM precheck after classification, as the reference (:200-212): the
-    // reduced graph (36 B) plus this solver's workspace (43 B), the reduced id
+    // reduced graph (36 B) plus this solver's workspace (26 B; its default
+    // knobs run no pull-mode sub-rounds), the reduced id
     // map (4 B) and the partition (1 B) per free node, the vertices and output
     // labels still to be placed, and 256 MiB of headroom
     size_t gpu_free = 0, gpu_total = 0;
     ckg(hipMemGetInfo(&gpu_free, &gpu_total), "hipMemGetInfo");
-    const uint64_t need = free_N * (36ull + 43ull + 4ull + 1ull) +
+    const uint64_t need = free_N * (36ull + 26ull + 4ull + 1ull) +
                           (resident ? 0ull : static_cast<uint64_t>(nv) * 24u + static_cast<uint64_t>(N)) +
                           (256ull << 20);
     if (need > static_cast<uint64_t>(gpu_free)) fail(WTV_ENOMEM, "insufficient VRAM for reduced graph + solver");

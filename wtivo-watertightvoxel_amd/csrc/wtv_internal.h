@@ -81,8 +81,12 @@ class SessionQueue {
     virtual double now() const = 0;
 };
 
+// Pull-mode sub-rounds run (and their 17 B/node of arrays are allocated) when
+// pull_min > 0, or, with the default pull_min 0, when fused sub-rounds are off.
+bool pull_enabled(const wtv_params& p);
+
 // Arena bytes per node of a session (rows, workspace, graph ids).
-uint64_t arena_bytes_per_node();
+uint64_t arena_bytes_per_node(bool pull);
 
 // Runs an arena session on the calling thread until the queue stops: an
 // arena of `capacity` nodes and `slots` graph slots on stream s, fed from q.

@@ -49,10 +49,10 @@ struct Ticket {
 
 }  // namespace
 
-uint64_t request_footprint(uint32_t n) {
+uint64_t request_footprint(uint32_t n, bool pull) {
     // the arena nodes a request occupies (rounded up to the seed sweep's
     // 1024-node tile) times the arena bytes per node
-    return aligned_nodes(n) * arena_bytes_per_node();
+    return aligned_nodes(n) * arena_bytes_per_node(pull);
 }
 
 class Server final : public SessionQueue {
@@ -65,7 +65,8 @@ class Server final : public SessionQueue {
         if (cfg_.hbm_budget_bytes == 0) cfg_.hbm_budget_bytes = 200ull * 1000ull * 1000ull * 1000ull;
         // each session's arena: at most max_batch_nodes, and all of them
         // together within the HBM budget
-        const uint64_t per = cfg_.hbm_budget_bytes / static_cast<uint64_t>(cfg_.slots) / arena_bytes_per_node();
+        const uint64_t per =
+            cfg_.hbm_budget_bytes / static_cast<uint64_t>(cfg_.slots) / arena_bytes_per_node(pull_enabled(cfg_.params));
         capacity_ = std::min<uint64_t>(cfg_.max_batch_nodes, per) / 1024u * 1024u;
         if (capacity_ < 1024u) fail(WTV_ENOMEM, std::string(kPrefix) + "HBM budget below one arena tile");
         cks(hipGetDevice(&device_), "hipGetDevice");
@@ -101,7 +102,7 @@ class Server final : public SessionQueue {
         j.ticket = next_ticket_++;
         j.req = r;
         j.ready = t->ready;
-        t->res.footprint_bytes = request_footprint(r.n);
+        t->res.footprint_bytes = request_footprint(r.n, pull_enabled(cfg_.params));
         t->res.submit_s = now();
         tickets_[j.ticket] = t;
         queue_.push_back(j);

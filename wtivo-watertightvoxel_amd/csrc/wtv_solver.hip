@@ -1667,7 +1667,7 @@ struct Layout {
 // holds the residual rows and terminals of its graphs, two more per-slot
 // byte tables (resident, fresh) and a per-slot error word (admission checks).
 // Pull-mode sub-rounds add 17 B per node: the push rows and the touched map.
-Layout layout_for(uint32_t n, uint32_t G, bool arena = false, bool pull = true) {
+Layout layout_for(uint32_t n, uint32_t G, bool arena = false, bool pull = false) {
     Layout L{};
     size_t o = 0;
     auto take = [&](size_t bytes) { size_t at = o; o = align_up(o + bytes, 256); return at; };
@@ -1767,7 +1767,7 @@ struct Feeder {
 struct SolveExtras {
     void* ws = nullptr;         // workspace provided by the caller (else allocated)
     bool noalloc = false;       // allocate nothing: caller's stream, no events or graphs
-    Feeder* feeder = nullptr;   // arena solve fed at relabels (ws: layout_for(n, slots + 1, true))
+    Feeder* feeder = nullptr;   // arena solve fed at relabels (ws: layout_for(n, slots + 1, true, pull))
     uint32_t slots = 0;         // graph slots of the arena
 };
 
@@ -1841,7 +1841,10 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     const bool trace = trace_env && *trace_env && *trace_env != '0';
 
     const uint32_t G = kB ? (feeder ? X.slots + 1u : B->G) : 0u;
-    const Layout L = layout_for(n, G, feeder != nullptr, !noalloc);
+    // pull-mode sub-rounds (never in the allocation-free solve): lists of more
+    // than pull_min vertices
+    const bool pull_on = !noalloc && pull_enabled(prm);
+    const Layout L = layout_for(n, G, feeder != nullptr, pull_on);
     DeviceBuf ws;
     if (!X.ws) ck(hipMalloc(&ws.p, L.total), "workspace malloc");
     // every exit from here on, failures included, first waits for the kernels
@@ -1856,9 +1859,6 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     uint32_t* rl = reinterpret_cast<uint32_t*>(base + L.rl);
     uint8_t* rslot = reinterpret_cast<uint8_t*>(base + L.rslot);
     uint8_t* nzm = reinterpret_cast<uint8_t*>(base + L.nzm);
-    // pull-mode sub-rounds (every solve that may allocate): lists of more
-    // than pull_min vertices
-    const bool pull_on = !noalloc && prm.pull_min >= 0;
     const uint64_t pull_min = prm.pull_min > 0 ? static_cast<uint64_t>(prm.pull_min) : kPullMinList;
     uint32_t* pout = pull_on ? reinterpret_cast<uint32_t*>(base + L.pout) : nullptr;
     uint8_t* ptouched = pull_on ? reinterpret_cast<uint8_t*>(base + L.ptouched) : nullptr;
@@ -2770,15 +2770,18 @@ class ArenaSession final : public Feeder {
 
 }  // namespace
 
-uint64_t arena_bytes_per_node() {
-    // rows 36 B, workspace 43 B, graph id 4 B, union ids 16 B ... as layout_for
-    return layout_for(1u << 20, 2u, true).total / (1u << 20);
+bool pull_enabled(const wtv_params& p) { return p.pull_min > 0 || (p.pull_min == 0 && p.fused < 0); }
+
+uint64_t arena_bytes_per_node(bool pull) {
+    // rows 36 B, workspace 26 B (43 B with the pull-mode arrays), graph id 4 B,
+    // union ids 16 B ... as layout_for
+    return layout_for(1u << 20, 2u, true, pull).total / (1u << 20);
 }
 
 void session_serve(SessionQueue& q, const wtv_params& prm, uint32_t capacity, uint32_t slots, void* stream,
                    int32_t worker) {
     capacity = std::max(kArenaAlign, capacity / kArenaAlign * kArenaAlign);
-    const Layout L = layout_for(capacity, slots + 1u, true);
+    const Layout L = layout_for(capacity, slots + 1u, true, pull_enabled(prm));
     DeviceBuf ws;
     ck(hipMalloc(&ws.p, L.total), "arena malloc");
     ArenaSession sess(q, capacity, slots, worker);
