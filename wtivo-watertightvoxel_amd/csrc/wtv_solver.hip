@@ -857,7 +857,7 @@ template <bool kB>
 __device__ __forceinline__ void fused_vertex(bool valid, uint32_t v, uint32_t n, const int4* __restrict__ nbr4,
                                              const uint8_t* __restrict__ rslot, uint32_t* res, int32_t* term,
                                              long long* excess, uint32_t* height, const uint32_t* __restrict__ gid,
-                                             const uint32_t* __restrict__ gn, Ctl* ctl, bool& keep, bool cand[4],
+                                             const uint32_t* __restrict__ gn, bool& keep, bool cand[4],
                                              int4& nb, unsigned long long& visits, unsigned long long& pushes,
                                              unsigned long long& sinkf, unsigned long long& relabels,
                                              unsigned long long& sdlt) {
@@ -913,7 +913,11 @@ __device__ __forceinline__ void fused_vertex(bool valid, uint32_t v, uint32_t n,
     uint32_t hf = h;
     if (act && e > 0) {
         // relabel on what v read: 1 + the lowest residual neighbour (the
-        // sink when term < 0, the source height when term > 0)
+        // sink when term < 0, the source height when term > 0).  A vertex
+        // that sees no residual arc keeps its height and stays listed: a
+        // concurrent push into v adds the reverse residual and the excess
+        // with two atomics, and v's loads may see the excess add before the
+        // residual one; the next visit sees both.
         uint32_t mh = kSent;
         if (t < 0) mh = 0u;
         else if (t > 0) mh = sh;
@@ -923,9 +927,7 @@ __device__ __forceinline__ void fused_vertex(bool valid, uint32_t v, uint32_t n,
                 const uint32_t y = hu[k] == kSent ? sh + 1u : hu[k];
                 mh = y < mh ? y : mh;
             }
-        if (mh == kSent) {
-            set_error(ctl, 5);
-        } else {
+        if (mh != kSent) {
             hf = mh + 1u < h + 2u ? mh + 1u : h + 2u;
             if (hf > h) {
                 height[v] = hf;
@@ -992,7 +994,7 @@ __global__ void __launch_bounds__(TPB) k_push_fused(uint32_t n, uint32_t ph, con
         bool keep, cand[4];
         int4 nb;
         unsigned long long sdlt;
-        fused_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, gid, gn, ctl, keep, cand, nb, visits,
+        fused_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, gid, gn, keep, cand, nb, visits,
                          pushes, sinkf, relabels, sdlt);
         flush_graph_flow<kB>(valid, v, sdlt, gid, gflow);
         q_push(q, keep, v);
@@ -1309,7 +1311,7 @@ __global__ void __launch_bounds__(kTailTPB) k_discharge_tail(uint32_t n, uint32_
                 bool keep, cand[4];
                 int4 nb;
                 unsigned long long sdlt;
-                fused_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, gid, gn, ctl, keep, cand, nb,
+                fused_vertex<kB>(valid, v, n, nbr4, rslot, res, term, excess, height, gid, gn, keep, cand, nb,
                                  visits, pushes, sinkf, rel, sdlt);
                 flush_graph_flow<kB>(valid, v, sdlt, gid, gflow);
                 lds_append(keep, v, wl[p ^ 1u], &s_cnt);
