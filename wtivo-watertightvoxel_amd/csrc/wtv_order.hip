@@ -189,13 +189,17 @@ __global__ void __launch_bounds__(kOrdTPB) k_gather_rows(uint32_t n, const uint3
     }
 }
 
-__global__ void __launch_bounds__(kOrdTPB) k_scatter_rows(uint32_t n, const uint32_t* __restrict__ perm,
+// The results back in the caller's order: node o takes row inv[o] of the
+// copy.  Reads are random and writes coalesced, the cheaper way round: a
+// scatter's random 16-, 4- and 1-byte writes each cost a read-modify-write
+// of a whole line at the memory.
+__global__ void __launch_bounds__(kOrdTPB) k_scatter_rows(uint32_t n, const uint32_t* __restrict__ inv,
                                                           const uint4* __restrict__ res_o,
                                                           const int32_t* __restrict__ term_o,
                                                           const uint8_t* __restrict__ part_o, uint4* __restrict__ res4,
                                                           int32_t* __restrict__ term, uint8_t* __restrict__ part) {
-    for (uint32_t j = blockIdx.x * kOrdTPB + threadIdx.x; j < n; j += gridDim.x * kOrdTPB) {
-        const uint32_t o = perm[j];
+    for (uint32_t o = blockIdx.x * kOrdTPB + threadIdx.x; o < n; o += gridDim.x * kOrdTPB) {
+        const uint32_t j = inv[o];
         res4[o] = res_o[j];
         term[o] = term_o[j];
         part[o] = part_o[j];
@@ -325,7 +329,7 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
                       prm, st, stream);
     const auto t1 = Clock::now();
     uint8_t* part_dst = partition_on_device ? partition : reinterpret_cast<uint8_t*>(keys);
-    k_scatter_rows<<<g, kOrdTPB, 0, s>>>(n, perm, res_o, term_o, part_o, reinterpret_cast<uint4*>(d_res), d_term,
+    k_scatter_rows<<<g, kOrdTPB, 0, s>>>(n, w, res_o, term_o, part_o, reinterpret_cast<uint4*>(d_res), d_term,
                                          part_dst);
     cko(hipGetLastError(), "order scatter");
     if (!partition_on_device) cko(hipMemcpyAsync(partition, part_dst, nz, hipMemcpyDeviceToHost, s), "partition D2H");
