@@ -467,7 +467,7 @@ def main():
         "config": dict(meta, **{"free_nodes": n, "lambda_fill": 10.0, "label_flips": 0.02, "seed": a.seed,
                                 "global_relabel_period": a.period, "local_steps": a.local_steps,
                                 "relabel_alpha": a.alpha if a.alpha != 0.0 else 2.0, "tail_cap": 512,
-                                "grid_wgs": 1024, "wg_size": 256}),
+                                "grid_wgs": 2048, "wg_size": 256}),
         "tets_per_s": round(N / value, 1),
         "autotuned": tuned,
         "t_solve_ms_device": round(s0["t_solve_ms"], 3),
@@ -1452,7 +1452,7 @@ def batch_leg(a, torch, np, W, S, dev, stream):
                       "graphs admitted at its first relabel and each leaving after its last active vertex; "
                       "identical results",
             "autotuned": best, "autotune_trials": len(trials), "autotune_seconds": round(t_tune, 2),
-            "looped_params": "defaults (alpha 2, local_steps 8, tail_cap 512, grid 1024, wg 256)",
+            "looped_params": "defaults (alpha 2, local_steps 8, tail_cap 512, grid 2048, wg 256)",
             "batched_work": {"sub_rounds": st.sub_rounds, "global_relabels": st.global_relabels,
                              "bfs_levels": st.bfs_levels}}
 

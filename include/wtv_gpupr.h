@@ -110,7 +110,7 @@ typedef struct wtv_params {
     int32_t wg_size;               /* workgroup size of the list kernels (push, relabel,
                                       BFS level, active filter): 0 (= 256), 256, 512 or
                                       1024; other values fail with WTV_EINVAL */
-    int32_t grid_wgs;              /* 0 = default            */
+    int32_t grid_wgs;              /* 0 = default (2048)     */
     double time_limit_s;           /* 0 = 240 s (reference)  */
     int32_t profile;               /* 1 = time every kernel with HIP events */
     int32_t no_graph;              /* 1 = launch kernels one by one instead of

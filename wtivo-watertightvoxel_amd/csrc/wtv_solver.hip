@@ -1804,7 +1804,10 @@ void solve_t(uint32_t n, const int32_t* d_nbr, uint32_t* d_res, int32_t* d_term,
     uint64_t max_rounds = prm.max_rounds ? prm.max_rounds : 2000000ull;
     const double time_limit = prm.time_limit_s > 0 ? prm.time_limit_s : 240.0;
     const int tpb = kTPB;  // sweeps over all vertices
-    int grid = prm.grid_wgs > 0 ? prm.grid_wgs : 1024;
+    // 2048 workgroups per list launch: the BFS levels take ~6% less per
+    // relabel than with 1024 on the 77.5M-tet Delaunay graph, the discharge
+    // launches no longer (4096: the same)
+    int grid = prm.grid_wgs > 0 ? prm.grid_wgs : 2048;
     // workgroup size of the list kernels (push, relabel, BFS level, filter)
     const int wg = prm.wg_size == 0 ? 256 : prm.wg_size;
     if (wg != 256 && wg != 512 && wg != 1024)

@@ -18,12 +18,12 @@ import json
 import os
 import time
 
-DEFAULT = {"relabel_alpha": 2.0, "local_steps": 8, "tail_cap": 512, "grid_wgs": 1024, "wg_size": 256}
+DEFAULT = {"relabel_alpha": 2.0, "local_steps": 8, "tail_cap": 512, "grid_wgs": 2048, "wg_size": 256}
 SPACE = {
     "relabel_alpha": [2.0, 4.0, 8.0, 16.0, 32.0],
     "local_steps": [4, 8, 16, 32],
     "tail_cap": [-1, 512, 2048, 8192],
-    "grid_wgs": [512, 1024, 2048],
+    "grid_wgs": [1024, 2048, 4096],
     "wg_size": [256, 512, 1024],
 }
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_params.json")
