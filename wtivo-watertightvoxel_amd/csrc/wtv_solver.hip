@@ -497,11 +497,14 @@ __global__ void __launch_bounds__(TPB) k_bfs_level(uint32_t n, uint32_t d, uint3
 // counter ring, exactly as a sequence of k_bfs_level launches would.
 constexpr uint32_t kTailMax = 1024;
 constexpr double kRelabelCost = 1.0;
-// Active lists of more than this many vertices run pull-mode sub-rounds by
-// default.  On the 77.5M-node Delaunay graph a pull-mode sub-round costs
-// 4.6 ms against 7.9 ms for the atomic one while the list holds all ~37M
-// source vertices, and the two meet near 2-3M vertices: the apply sweep
-// reads two words per neighbour where the relabel reads one.
+// With fused sub-rounds off, active lists of more than this many vertices
+// run pull-mode sub-rounds by default.  On the 77.5M-node Delaunay graph a
+// pull-mode sub-round costs 4.6 ms against 7.9 ms for a push/relabel pair
+// while the list holds all ~37M source vertices, and the two meet near 2-3M
+// vertices: the apply sweep reads two words per neighbour where the relabel
+// reads one.  A fused sub-round (4.5 ms there, and less after it as the list
+// shrinks faster) beats both, so with fused sub-rounds pull mode runs only on
+// request (wtv_params.pull_min > 0).
 constexpr uint64_t kPullMinList = 4194304u;
 // The one-workgroup kernels run 1024 threads: a list of up to 1024 items
 // takes one pass per level or phase.
@@ -1119,7 +1122,7 @@ __global__ void __launch_bounds__(kPullTPB) k_push_pull(uint32_t n, uint32_t ph,
     flush_stat(ctl, kStSinkFlow, sinkf);
 }
 
-// Apply phase of a pull-mode sub-round with parity p: a sweep of the first m
+// Apply phase of a pull-mode sub-round with phase ph: a sweep of the first m
 // bytes of the touched map (clearing it) in id order; every marked vertex
 // takes in what its neighbours pushed into it, relabels, and joins the next
 // active list when active.  Every vertex that holds excess after the push
