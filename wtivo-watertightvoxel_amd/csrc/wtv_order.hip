@@ -323,6 +323,9 @@ void pr_solve(uint32_t n, int32_t* d_nbr, uint32_t* d_res, int32_t* d_term, uint
     k_gather_rows<<<g, kOrdTPB, 0, s>>>(n, perm, w, nbr4, reinterpret_cast<const uint4*>(d_res), d_term, nbr_o, res_o,
                                         term_o);
     cko(hipGetLastError(), "order copy");
+    // the copy finished, so that t_order_ms holds the renumbering's whole time
+    // (the solve waits for it anyway)
+    cko(hipStreamSynchronize(s), "order copy");
     const double order_ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
     // 4. solve the copy and scatter back
     pr_solve_in_order(n, reinterpret_cast<int32_t*>(nbr_o), reinterpret_cast<uint32_t*>(res_o), term_o, part_o, true,
